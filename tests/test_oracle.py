@@ -1,0 +1,446 @@
+"""Pins the CPU oracle (oracle/nsp_oracle.cpp) against mathematics that fixes
+each of its parts independently of how the oracle computes them:
+polynomial exactness of the finite-difference tables, numpy's FFTs, dense
+solves, numpy's 4x4 inverse, analytic Helmholtz/Poisson solutions, the
+discrete Poiseuille fixed point, the closed-form decay of an m=0 swirl
+eigenvector, and a brute-force pseudo-spectral evaluation of the nonlinear
+term.  The reference ships no tests or golden vectors of its own
+(its repository holds none), so these are the pins.
+"""
+import numpy as np
+import pytest
+
+from oracle_lib import Oracle
+
+IW = 3
+TINY = dict(nr=8, nt=4, nz=5, lz=2 * np.pi, re=300.0, dt=0.01)
+SMALL = dict(nr=16, nt=8, nz=9, lz=2 * np.pi, re=500.0, dt=0.01)
+CFG1 = dict(nr=32, nt=64, nz=65, lz=4 * np.pi, re=2000.0, dt=0.005)
+
+
+@pytest.fixture(scope="module")
+def small():
+    return Oracle(**SMALL)
+
+
+@pytest.fixture(scope="module")
+def cfg1():
+    return Oracle(**CFG1)
+
+
+def stencil(nr, i):
+    """Columns of row i and the table slot offset (mod_radialFD.cu:10-30)."""
+    left, right = max(0, i - IW), min(i + IW, nr - 1)
+    return left, right, i + IW - right
+
+
+def npoints(nr, i):
+    """Number of Fornberg points behind row i (ghost points included)."""
+    return 7 if i + IW <= nr - 1 else nr + IW - i
+
+
+def apply_row(tab, p, i, f, nr):
+    left, right, i0 = stencil(nr, i)
+    return sum(tab[p, i, j + i0] * f[left + j] for j in range(right + 1 - left))
+
+
+def mode_index(o, m, l):
+    it = m if m >= 0 else m + o.nt
+    return it, l
+
+
+# ----------------------------------------------------------------- tables
+@pytest.mark.parametrize("cfg", [SMALL, CFG1])
+def test_grid_and_quadrature(cfg):
+    o = Oracle(**cfg)
+    T = o.tables()
+    r, rdr = T["r"], T["rdr"]
+    assert np.all(np.diff(r) > 0) and r[0] > 0
+    assert abs(r[-1] - 1.0) < 1e-14
+    # sum(rdr) telescopes to r[NR-1]*r[NR] - r[0]*r[-1] = 1
+    assert abs(rdr.sum() - 1.0) < 1e-14
+    # second-order quadrature of 2*int_0^1 f r dr
+    nr = cfg["nr"]
+    assert abs((rdr * (1 - r ** 2)).sum() - 0.5) < 2.0 / nr ** 2
+
+
+@pytest.mark.parametrize("cfg", [SMALL, CFG1])
+def test_fornberg_exactness_on_monomials(cfg):
+    o = Oracle(**cfg)
+    T = o.tables()
+    r, D, Lu, Lp = T["r"], T["D"], T["Lu"], T["Lp"]
+    nr = cfg["nr"]
+    for p in (0, 1):  # 0: even functions, 1: odd functions
+        for i in range(nr):
+            npt = npoints(nr, i)
+            for q in range(npt):
+                if i < IW and q % 2 != p:
+                    continue  # folded ghost points are exact only for the table's parity
+                f = r ** q
+                d_exact = q * r[i] ** (q - 1) if q > 0 else 0.0
+                scale = sum(abs(D[p, i, :])) * max(1.0, r.max() ** q)
+                assert abs(apply_row(D, p, i, f, nr) - d_exact) < 1e-12 * scale, (p, i, q)
+                if i < nr - 1:
+                    d2 = q * (q - 1) * r[i] ** (q - 2) if q > 1 else 0.0
+                    lap = d2 + d_exact / r[i]
+                    scale2 = sum(abs(Lu[p, i, :]))
+                    assert abs(apply_row(Lu, p, i, f, nr) - lap) < 1e-11 * scale2, (p, i, q)
+                    assert abs(apply_row(Lp, p, i, f, nr) - lap) < 1e-11 * scale2, (p, i, q)
+        # wall rows: Lu = identity (Dirichlet), Lp = D (Neumann)
+        assert np.array_equal(Lu[p, nr - 1], np.array([0, 0, 0, 0, 0, 0, 1.0]))
+        assert np.array_equal(Lp[p, nr - 1], D[p, nr - 1])
+    assert np.array_equal(T["dr1"], D[0, nr - 1])
+
+
+def test_dr0_interpolates_to_axis(cfg1):
+    T = cfg1.tables()
+    r, dr0 = T["r"], T["dr0"]
+    for q in range(4):
+        assert abs(np.dot(dr0, r[:4] ** q) - (1.0 if q == 0 else 0.0)) < 1e-12
+
+
+# ----------------------------------------------------------------- FFT
+@pytest.mark.parametrize("cfg", [TINY, SMALL, CFG1])
+def test_fft2_matches_numpy(cfg):
+    o = Oracle(**cfg)
+    rng = np.random.default_rng(1)
+    spec = rng.standard_normal((o.ntp, o.nzp)) + 1j * rng.standard_normal((o.ntp, o.nzp))
+    phys = o.fft2_inverse(spec)
+    ref = np.fft.irfft2(spec, s=(o.ntp, o.nzf)) * (o.ntp * o.nzf)
+    assert np.abs(phys - ref).max() < 1e-13 * np.abs(ref).max()
+    x = rng.standard_normal((o.ntp, o.nzf))
+    sp = o.fft2_forward(x)
+    refs = np.fft.rfft2(x)
+    assert np.abs(sp - refs).max() < 1e-13 * np.abs(refs).max()
+
+
+# ----------------------------------------------------------------- LU solves
+def dense_operator(o, T, kind, m, l):
+    """Assembled operator of LUsolve_{up,um,uz,p}_k (mod_linear.cu:55-646)."""
+    nr = o.nr
+    r = T["r"]
+    d_im = 0.51
+    tab = T["Lp"] if kind == 3 else T["Lu"]
+    fp = 1 if kind <= 1 else 0
+    p = abs(abs(m) % 2 - fp)
+    kz = 2 * np.pi / o.lz * l
+    kt = float(m)
+    A = np.zeros((nr, nr))
+    for i in range(nr):
+        left, right, i0 = stencil(nr, i)
+        for j in range(right + 1 - left):
+            A[i, left + j] = tab[p, i, j + i0]
+        if i < nr - 1:
+            r2 = r[i] ** 2
+            dterm = -kz * kz - kt * kt / r2
+            if kind == 0:
+                dterm += -1 / r2 - 2 * kt / r2
+            elif kind == 1:
+                dterm += -1 / r2 + 2 * kt / r2
+            A[i, i] += dterm
+            if kind < 3:
+                A[i, :] = -(d_im / o.re) * A[i, :]
+                A[i, i] += 1.0 / o.dt
+        elif kind == 3 and m == 0 and l == 0:
+            A[i, :] = 0.0
+            A[i, i] = 1.0
+    return A
+
+
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_lusolve_matches_dense_solve(small, kind):
+    o = small
+    T = o.tables()
+    rng = np.random.default_rng(kind)
+    rhs = rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)
+    sol = o.lusolve(kind, rhs)
+    for it in range(o.nt):
+        m = it if it < o.nt // 2 else it - o.nt
+        for l in range(o.nz):
+            A = dense_operator(o, T, kind, m, l)
+            x = np.linalg.solve(A, rhs[:, it, l])
+            scale = np.abs(x).max() * np.linalg.cond(A)
+            assert np.abs(sol[:, it, l] - x).max() < 1e-14 * scale, (kind, m, l)
+
+
+def test_lusolve_analytic_helmholtz(cfg1):
+    """Polynomial solutions inside the stencils' exactness: round-off recovery."""
+    o = cfg1
+    T = o.tables()
+    r = T["r"]
+    d_im, re, dt = 0.51, o.re, o.dt
+    for m, l in [(0, 0), (0, 3), (1, 2), (2, 1), (-2, 5)]:
+        it, _ = mode_index(o, m, l)
+        kz = 2 * np.pi / o.lz * l
+        # uz: x = r^|m| (1 - r^2), parity of |m|
+        q = abs(m)
+        x = r ** q * (1 - r ** 2)
+        lap = (q + 2) ** 2 * r ** q * 0 + (q * q * r ** (q - 2) if q else 0) - (q + 2) ** 2 * r ** q
+        lap = lap - m * m * x / r ** 2 - kz * kz * x
+        f = x / dt - d_im / re * lap
+        f[-1] = 0.0
+        rhs = o.zeros(); rhs[:, it, l] = f
+        sol = o.lusolve(2, rhs)[:, it, l]
+        assert np.abs(sol - x).max() < 1e-11, (m, l)
+        # u+: x = r^s (1 - r^2), s of parity |m|+1 (exact for degree <= 4 near the wall)
+        s = 0 if abs(m) % 2 == 1 else 1
+        x = r ** s * (1 - r ** 2)
+        d2x = (s * (s - 1) * r ** (s - 2) if s > 1 else 0) - (s + 2) * (s + 1) * r ** s
+        d1x = (s * r ** (s - 1) if s > 0 else 0) - (s + 2) * r ** (s + 1)
+        for kind, sg in [(0, -1), (1, +1)]:
+            lap = d2x + d1x / r + (-1 + sg * 2 * m - m * m) * x / r ** 2 - kz * kz * x
+            f = x / dt - d_im / re * lap
+            f[-1] = 0.0
+            rhs = o.zeros(); rhs[:, it, l] = f
+            sol = o.lusolve(kind, rhs)[:, it, l]
+            assert np.abs(sol - x).max() < 1e-11, (kind, m, l)
+    # pressure with Neumann wall row: m=1, p = r - r^3/3 (p'(1)=0)
+    m, l = 1, 2
+    it, _ = mode_index(o, m, l)
+    kz = 2 * np.pi / o.lz * l
+    p = r - r ** 3 / 3
+    f = -8 * r / 3 - kz * kz * p
+    f[-1] = 0.0
+    rhs = o.zeros(); rhs[:, it, l] = f
+    sol = o.lusolve(3, rhs)[:, it, l]
+    assert np.abs(sol - p).max() < 1e-9
+
+
+# ----------------------------------------------------------------- radial operators
+def test_derivr_parity_selection(cfg1):
+    o = cfg1
+    r = o.tables()["r"]
+    u = o.zeros()
+    # flag 1 (ur/ut): even m -> odd function, odd m -> even function
+    cases = {(0, 0): r, (1, 1): r ** 2, (2, 3): r ** 3, (-1, 2): 1 + 0 * r}
+    dexp = {(0, 0): 1 + 0 * r, (1, 1): 2 * r, (2, 3): 3 * r ** 2, (-1, 2): 0 * r}
+    for (m, l), f in cases.items():
+        it, _ = mode_index(o, m, l)
+        u[:, it, l] = f * (1 + 2j)
+    du = o.derivr(u, 1)
+    for (m, l), d in dexp.items():
+        it, _ = mode_index(o, m, l)
+        assert np.abs(du[:, it, l] - d * (1 + 2j)).max() < 1e-10
+    # flag 0 (uz/p): parity of |m|
+    u = o.zeros()
+    it, _ = mode_index(o, 3, 1)
+    u[:, it, 1] = r ** 3
+    it0, _ = mode_index(o, 2, 1)
+    u[:, it0, 1] = r ** 2
+    du = o.derivr(u, 0)
+    assert np.abs(du[:, it, 1] - 3 * r ** 2).max() < 1e-10
+    assert np.abs(du[:, it0, 1] - 2 * r).max() < 1e-10
+
+
+def test_divergence_and_addgrad_polynomial(cfg1):
+    o = cfg1
+    r = o.tables()["r"]
+    m, l = 1, 2
+    it, _ = mode_index(o, m, l)
+    kz = 2 * np.pi / o.lz * l
+    ur, ut, uz = o.zeros(), o.zeros(), o.zeros()
+    ur[:, it, l] = 1 - r ** 2
+    ut[:, it, l] = 0.5
+    uz[:, it, l] = 2 * r
+    dv = o.divergence(ur, ut, uz)[:, it, l]
+    exp = -2 * r + (1 - r ** 2) / r + 1j * m * 0.5 / r + 1j * kz * 2 * r
+    assert np.abs(dv - exp).max() < 1e-10
+    # wall divergence uses ur (not ur/r) at r = 1
+    dw = o.divwall(ur, ut, uz)[it, l]
+    assert abs(dw - exp[-1]) < 1e-10
+    # addGrad subtracts (dp/dr, i m p / r, i kz p); p of parity |m| (odd): p = r
+    p = o.zeros(); p[:, it, l] = r
+    rr, rt, rz = o.addgrad(o.zeros(), o.zeros(), o.zeros(), p)
+    assert np.abs(rr[:, it, l] + 1).max() < 1e-10
+    assert np.abs(rt[:, it, l] + 1j * m * r / r).max() < 1e-12
+    assert np.abs(rz[:, it, l] + 1j * kz * r).max() < 1e-12
+
+
+def test_addlap_polynomial(cfg1):
+    o = cfg1
+    r = o.tables()["r"]
+    d_im, re, dt = 0.51, o.re, o.dt
+    m, l = 2, 1
+    it, _ = mode_index(o, m, l)
+    kz = 2 * np.pi / o.lz * l
+    u = o.zeros()
+    x = r * (1 - r ** 2)  # odd: u+- at even m
+    u[:, it, l] = x
+    d1 = 1 - 3 * r ** 2
+    d2 = -6 * r
+    for flag, sg in [(0, -1), (1, +1)]:
+        out = o.addlap(o.zeros(), u, flag)[:, it, l]
+        lap = d2 + d1 / r + (-1 + sg * 2 * m - m * m) * x / r ** 2 - kz * kz * x
+        exp = x / dt + (1 - d_im) / re * lap
+        assert np.abs(out[:-1] - exp[:-1]).max() < 1e-9
+    u = o.zeros()
+    x = r ** 2 * (1 - r ** 2)
+    u[:, it, l] = x
+    out = o.addlap(o.zeros(), u, 2)[:, it, l]
+    lap = (4 - 16 * r ** 2) - m * m * x / r ** 2 - kz * kz * x
+    assert np.abs(out[:-1] - (x / dt + (1 - d_im) / re * lap)[:-1]).max() < 1e-9
+
+
+# ----------------------------------------------------------------- influence matrix
+def to_rt(up, um):
+    return 0.5 * (up + um), (up - um) / 2j
+
+
+def test_influence_inverse_matches_numpy(small):
+    o = small
+    col, (uhp, uhm, uhz, uPp, uPm, uPz) = o.influence()
+    zero = o.zeros()
+    d1 = o.divwall(*to_rt(uhp, zero), zero)
+    d2 = o.divwall(*to_rt(zero, uhm), zero)
+    d3 = o.divwall(zero, zero, uhz)
+    d4 = o.divwall(*to_rt(uPp, uPm), uPz)
+    for it in range(o.nt):
+        for l in range(o.nz):
+            M = np.array([[1, 0, 0, uPp[-1, it, l]],
+                          [0, 1, 0, uPm[-1, it, l]],
+                          [0, 0, 1j, uPz[-1, it, l]],
+                          [d1[it, l], d2[it, l], d3[it, l], d4[it, l]]], np.complex128)
+            inv = np.linalg.inv(M)
+            k = it * o.nz + l
+            assert np.abs(col[k] - inv[:, 3]).max() < 1e-12 * np.abs(inv).max(), (it, l)
+    # homogeneous velocity solutions take unit wall values (mod_boundary.cu:226-232)
+    assert np.allclose(uhp[-1], 1.0) and np.allclose(uhm[-1], 1.0) and np.allclose(uhz[-1], 1j)
+
+
+def test_corbcon_enforces_noslip_and_wall_divergence(cfg1):
+    o = cfg1
+    rng = np.random.default_rng(5)
+    f = [rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape) for _ in range(3)]
+    for x in f:
+        x[-1] = 0.0  # wall rows as written by write_bcon before the solve
+    up, um, uz = o.corbcon(*f)
+    ur, ut = to_rt(up, um)
+    scale = max(np.abs(x).max() for x in f)
+    for x in (ur, ut, uz):
+        assert np.abs(x[-1]).max() < 1e-12 * scale
+    dw = o.divwall(ur, ut, uz)
+    assert np.abs(dw).max() < 1e-9 * scale
+
+
+def test_adj_flux_sets_bulk_velocity(cfg1):
+    o = cfg1
+    T = o.tables()
+    rng = np.random.default_rng(2)
+    uz = rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape)
+    out = o.adj_flux(uz)
+    assert abs(np.dot(T["rdr"], out[:, 0, 0].real) - 0.5) < 1e-14
+    # only mode (0,0) is touched
+    diff = out - uz
+    diff[:, 0, 0] = 0
+    assert np.abs(diff).max() == 0.0
+    qc, uc = o.flux_qc()
+    assert abs(qc - np.dot(T["rdr"], uc)) < 1e-15
+
+
+# ----------------------------------------------------------------- whole step
+def test_poiseuille_is_discrete_fixed_point():
+    o = Oracle(nr=16, nt=8, nz=9, lz=2 * np.pi, re=800.0, dt=0.01)
+    T = o.tables()
+    r, rdr = T["r"], T["rdr"]
+    alpha = 0.5 / np.dot(rdr, 1 - r ** 2)
+    ur, ut, uz = o.zeros(), o.zeros(), o.zeros()
+    uz[:, 0, 0] = alpha * (1 - r ** 2)
+    o.set_state(ur, ut, uz)
+    n_it, err = o.step(100)
+    a, b, c = o.get_state()
+    assert np.all(n_it == 1)
+    assert np.abs(c - uz).max() < 1e-13
+    assert np.abs(a).max() < 1e-13 and np.abs(b).max() < 1e-13
+
+
+def test_swirl_eigenvector_decays_by_closed_form_factor():
+    o = Oracle(nr=32, nt=8, nz=9, lz=2 * np.pi, re=1000.0, dt=0.01)
+    T = o.tables()
+    r, rdr, Lu = T["r"], T["rdr"], T["Lu"]
+    nr = o.nr
+    # A = Lu (odd table, m = 0) - 1/r^2 on the interior rows, wall value 0
+    A = np.zeros((nr, nr))
+    for i in range(nr):
+        left, right, i0 = stencil(nr, i)
+        for j in range(right + 1 - left):
+            A[i, left + j] = Lu[1, i, j + i0]
+        A[i, i] -= 1 / r[i] ** 2
+    Ai = A[: nr - 1, : nr - 1]
+    mu, V = np.linalg.eig(Ai)
+    k = np.argmax(mu.real)
+    assert abs(mu[k].imag) < 1e-10
+    mu = mu[k].real
+    v = np.concatenate([V[:, k].real, [0.0]])
+    v /= np.abs(v).max()
+    eps = 1e-2
+    alpha = 0.5 / np.dot(rdr, 1 - r ** 2)
+    ur, ut, uz = o.zeros(), o.zeros(), o.zeros()
+    ut[:, 0, 0] = eps * v
+    uz[:, 0, 0] = alpha * (1 - r ** 2)
+    o.set_state(ur, ut, uz)
+    nsteps = 50
+    n_it, _ = o.step(nsteps)
+    d_im, re, dt = 0.51, o.re, o.dt
+    g = (1 + (1 - d_im) * mu * dt / re) / (1 - d_im * mu * dt / re)
+    _, b, _ = o.get_state()
+    assert np.abs(b[:, 0, 0] - g ** nsteps * eps * v).max() < 1e-12 * eps
+    # continuum limit: mu -> -j_{1,1}^2 = -14.6819...
+    assert abs(mu + 14.681970642123893) < 1e-3 * 14.68
+
+
+# ----------------------------------------------------------------- nonlinear term
+def test_nonlinear_brute_force():
+    """Direct Fourier sums on the 3/2 grid against the oracle's FFT pipeline
+    (mod_io_nonlinear.cu:380-409)."""
+    o = Oracle(**TINY)
+    nr, nt, nz, ntp, nzf = o.nr, o.nt, o.nz, o.ntp, o.nzf
+    T = o.tables()
+    r, D = T["r"], T["D"]
+    rng = np.random.default_rng(11)
+    u = [rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape) for _ in range(3)]
+    out = o.nonlinear(*u)
+    ms = np.array([it if it < nt // 2 else it - nt for it in range(nt)])
+    kz = 2 * np.pi / o.lz * np.arange(nz)
+    th = 2 * np.pi * np.arange(ntp) / ntp
+    z = o.lz * np.arange(nzf) / nzf
+
+    def dr(f, flag):
+        g = np.zeros_like(f)
+        for it in range(nt):
+            p = abs(abs(ms[it]) % 2 - flag)
+            for i in range(nr):
+                left, right, i0 = stencil(nr, i)
+                g[i, it, :] = sum(D[p, i, j + i0] * f[left + j, it, :] for j in range(right + 1 - left))
+        return g
+
+    def phys(c):  # c: (nt, nz) level spectrum -> (ntp, nzf) real field
+        E = np.exp(1j * (ms[:, None, None, None] * th[None, None, :, None]
+                         + kz[None, :, None, None] * z[None, None, None, :]))
+        terms = c[:, :, None, None] * E
+        f = terms[:, 0].sum(axis=0).real
+        f = f + 2 * terms[:, 1:].sum(axis=(0, 1)).real
+        return f
+
+    dur = [dr(u[0], 1), dr(u[1], 1), dr(u[2], 0)]
+    ref = [o.zeros() for _ in range(3)]
+    for i in range(nr):
+        U = [phys(x[i]) for x in u]
+        R = [phys(x[i]) for x in dur]
+        Th = [phys(1j * ms[:, None] * x[i] / r[i]) for x in u]
+        Z = [phys(1j * kz[None, :] * x[i]) for x in u]
+        ur, ut, uz = U
+        N = [-(ur * R[0] + ut * Th[0] + uz * Z[0] - ut * ut / r[i]),
+             -(ur * R[1] + ut * Th[1] + uz * Z[1] + ur * ut / r[i]),
+             -(ur * R[2] + ut * Th[2] + uz * Z[2])]
+        Einv = np.exp(-1j * (ms[:, None, None, None] * th[None, None, :, None]
+                             + kz[None, :, None, None] * z[None, None, None, :]))
+        for c in range(3):
+            co = (N[c][None, None] * Einv).sum(axis=(2, 3)) / (ntp * nzf)
+            co[nt // 2, :] = 0.0
+            co[:, nz - 1] = 0.0
+            ref[c][i] = co
+    # mean pressure gradient on mode (0,0) (mod_radialFD.cu:100-111)
+    dw = sum(D[0, nr - 1, 3 + j] * u[2][nr - 4 + j, 0, 0].real for j in range(4))
+    ref[2][:, 0, 0] -= 2.0 / o.re * dw
+    for c in range(3):
+        assert np.abs(out[c] - ref[c]).max() < 1e-12 * np.abs(ref[c]).max(), c

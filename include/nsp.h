@@ -1,0 +1,157 @@
+/* nsp.h -- C ABI of the MI355X nsPipe time step (libnsp.so).
+ *
+ * This boundary replaces the reference's compile-time configuration and the
+ * free functions its driver calls (Mordered/nsPipe-GPU, Main_Code/src).  Each
+ * entry point names the reference interface it stands in for.  Plain C types
+ * only: opaque handle, int status codes, pointers and sizes.
+ *
+ * Field layout (both directions, host or device memory): three arrays
+ * ur, ut, uz of interleaved (re, im) doubles, i.e. double2[NR][NT][NZ] with
+ * h = i*NT*NZ + it*NZ + l (head.h:73 vfield, main.cu:30-33).  Azimuthal
+ * wavenumber m = it for it < NT/2, it - NT otherwise (mod_deriv.cu:15); axial
+ * wavenumber kz = 2*pi*l/LZ (mod_deriv.cu:31).  Velocities are stored in the
+ * radial/azimuthal form (not u+/-), as in the reference's restart files.
+ *
+ * Errors are returned, never exit()ed (the reference exits, head.h:64-70).
+ * One handle = one HIP stream; handles are independent (thread-compatible,
+ * not thread-safe per handle).  The library owns all device memory; caller
+ * buffers are only copied.
+ */
+#ifndef NSP_H
+#define NSP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  NSP_OK = 0,
+  NSP_ERR_INVALID = 1,   /* bad argument or unsupported size */
+  NSP_ERR_HIP = 2,       /* HIP runtime error (no device, launch failure, ...) */
+  NSP_ERR_IO = 3,        /* restart file missing or short */
+  NSP_ERR_NOMEM = 4      /* device or host allocation failed */
+} nsp_status;
+
+typedef enum {
+  NSP_FP64 = 0,   /* every stage in FP64 (parity form) */
+  NSP_MIXED = 1   /* FP32 transforms/products and non-stiff solves, FP64 state */
+} nsp_precision;
+
+typedef enum { NSP_HOST = 0, NSP_DEVICE = 1 } nsp_where;
+
+/* Runtime parameters; replaces the #defines of head.h:14-61. */
+typedef struct {
+  int nr;          /* NR  radial points (head.h:16), >= 8 */
+  int nt;          /* NT  = 2*m_theta azimuthal modes, even (head.h:17) */
+  int nz;          /* NZ  = m_z+1 axial modes, odd (head.h:18) */
+  double lz;       /* LZ  pipe length in R (head.h:22) */
+  double re;       /* Re  (head.h:25) */
+  double dt;       /* dt  constant time step (head.h:46) */
+  double d_im;     /* implicit weight, 0.51 in the reference (head.h:47) */
+  int maxit;       /* corrector iterations cap, 10 (head.h:49) */
+  double tol;      /* corrector tolerance, 5e-8 (head.h:50) */
+  double qdes;     /* target bulk flux, 0.5 (mod_cublasflux.cu:92) */
+  int precision;   /* nsp_precision */
+} nsp_params;
+
+typedef struct nsp_handle nsp_handle;
+
+/* Per-step report (the reference prints err every dt_frc, mod_integrate.cu:136). */
+typedef struct {
+  int n_it;        /* corrector iterations used (mod_integrate.cu:120) */
+  double err;      /* final convergence measure err[0] (mod_cublasflux.cu:356) */
+  int flags;       /* bit0: err was NaN; bit1: stopped at maxit without err <= tol */
+} nsp_step_info;
+
+#define NSP_FLAG_NAN 1
+#define NSP_FLAG_MAXIT 2
+
+/* Fills the reference defaults (head.h:14-61: sten 7, d_im 0.51, maxit 10,
+ * tol 5e-8; Qdes 0.5 from mod_cublasflux.cu:92) for the given sizes. */
+nsp_status nsp_default_params(nsp_params* p, int nr, int nt, int nz, double lz, double re,
+                              double dt);
+
+/* Replaces main.cu:19-28 (init_fd, setFft, setLinear, setCublasFlux,
+ * setDeriv, setNonlinear, setIO, setInt, setBCon) and main.cu:30-33 (state
+ * allocation).  Selects HIP device `device`.  The state starts at zero. */
+nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out);
+
+/* Replaces the *Destroy functions (main.cu:46-50). */
+nsp_status nsp_destroy(nsp_handle* h);
+
+/* State in/out in the reference layout (rBufferBinary/wBufferBinary payload,
+ * mod_io_nonlinear.cu:528-565).  where = NSP_HOST or NSP_DEVICE. */
+nsp_status nsp_set_state(nsp_handle* h, const double* ur, const double* ut, const double* uz,
+                         int where);
+nsp_status nsp_get_state(nsp_handle* h, double* ur, double* ut, double* uz, int where);
+
+/* Reference initial condition for restart == 0 (initField, mod_io_nonlinear.cu:424-449):
+ * Hagen-Poiseuille uz00 = 1 - r^2 plus the A_P perturbation (head.h:26). */
+nsp_status nsp_reference_ic(nsp_handle* h, double a_p);
+
+/* Advances nsteps time steps: the body of integrate()'s loop
+ * (mod_integrate.cu:112-131), one nonlinear + predictor, then correctors
+ * until err <= tol or maxit.  info may be NULL, else receives nsteps entries. */
+nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info);
+
+/* nonlinear(u, du) (mod_io_nonlinear.cu:380-409) on caller fields:
+ * du = -(u.grad)u in r/theta form plus the mean pressure gradient on mode (0,0). */
+nsp_status nsp_nonlinear(nsp_handle* h, const double* ur, const double* ut, const double* uz,
+                         double* dr, double* dt, double* dz, int where);
+
+/* Diagnostics of the current state (wrt_frc_k, mod_io_nonlinear.cu:40-52, plus
+ * checks): out[0] centre-line velocity, out[1] friction velocity u_tau,
+ * out[2] kinetic energy 1/2 sum_modes w_l sum_r rdr |u|^2 (w_0 = 1, else 2),
+ * out[3] L2 divergence (weights |rdr|), out[4] max |u| on the wall,
+ * out[5] max |wall divergence|, out[6] bulk flux sum rdr Re uz00. */
+nsp_status nsp_diagnostics(nsp_handle* h, double out[7]);
+
+/* Radial tables (init_fd, mod_radialFD.cu:256-369; get_dr0 :373-407;
+ * get_dr1 :412-415): r[NR], D[2][NR][7], Lu[2][NR][7], Lp[2][NR][7],
+ * rdr[NR], dr0[4], dr1[7].  Any pointer may be NULL. */
+nsp_status nsp_tables(nsp_handle* h, double* r, double* D, double* Lu, double* Lp, double* rdr,
+                      double* dr0, double* dr1);
+
+/* Same tables without a handle or device (host-only setup code). */
+nsp_status nsp_radial_tables(int nr, double* r, double* D, double* Lu, double* Lp, double* rdr,
+                             double* dr0, double* dr1);
+
+/* Restart files, raw little-endian double2[NR][NT][NZ] with no header
+ * (rBufferBinary/wBufferBinary, mod_io_nonlinear.cu:528-565; the reference
+ * reads ur_s.bin/ut_s.bin/uz_s.bin, :426-429, and writes ur.bin/ut.bin/uz.bin,
+ * main.cu:42-44). */
+nsp_status nsp_read_state_files(nsp_handle* h, const char* ur_path, const char* ut_path,
+                                const char* uz_path);
+nsp_status nsp_write_state_files(nsp_handle* h, const char* ur_path, const char* ut_path,
+                                 const char* uz_path);
+
+/* Seeded synthetic initial condition (no reference counterpart: the
+ * reference's only IC is nsp_reference_ic).  Poiseuille alpha*(1-r^2) scaled
+ * to bulk flux qdes, plus an analytically divergence-free, no-slip
+ * perturbation on modes |m| <= mmax, 0 <= l <= lmax with complex Gaussian
+ * amplitudes (mt19937_64(seed), Box-Muller) times the envelope
+ * (kind 0: flat, kind 1: |k|^(-5/3) with |k|^2 = kz^2 + m^2), rescaled so the
+ * perturbation rms velocity is `rms`.  Written into the state. */
+nsp_status nsp_seeded_ic(nsp_handle* h, unsigned long long seed, int mmax, int lmax, int kind,
+                         double rms);
+
+/* Event-timed last step breakdown in milliseconds: out[0] nonlinear (all
+ * evaluations), out[1] linear sweeps, out[2] whole step.  Requires that
+ * timing was enabled with nsp_set_timing(h, 1) before the step. */
+nsp_status nsp_set_timing(nsp_handle* h, int enable);
+nsp_status nsp_last_timing(nsp_handle* h, double out[3]);
+
+/* HIP stream of the handle (a hipStream_t), for callers that order their own
+ * work against it. */
+void* nsp_stream(nsp_handle* h);
+
+/* Human-readable text of the last error on this thread. */
+const char* nsp_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* NSP_H */

@@ -1,0 +1,650 @@
+// C ABI of libnsp: handle lifetime, state transfer, the time-step loop,
+// restart files, initial conditions and diagnostics.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <random>
+
+#include "nsp_internal.h"
+
+using namespace nsp;
+
+static thread_local std::string g_err;
+
+static nsp_status fail(nsp_status st, const std::string& msg) {
+  g_err = msg;
+  return st;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess)                                                              \
+      return fail(NSP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+struct nsp_handle {
+  nsp_params P;
+  Geom g;
+  int device;
+  hipStream_t stream = nullptr;
+  RadialTables host;
+  DevTables t{};
+  DevInfluence b{};
+  NlBufs nb{};
+  // fields: groups of three components, N complex each
+  double2* u = nullptr;     // current state (r/theta)
+  double2* ui = nullptr;    // iterate
+  double2* un = nullptr;    // next iterate
+  double2* uw = nullptr;    // u^n (+/-)
+  double2* Nl = nullptr;    // nonlinear term
+  double2* Nw = nullptr;    // N(u^n) (+/-)
+  double2* rhs = nullptr;   // scratch rhs
+  double2* p = nullptr;     // scratch pressure (1 field)
+  double* U = nullptr;      // scratch LU factor (4 doubles per row and mode)
+  unsigned long long* norm = nullptr;  // device [2]
+  unsigned long long* norm_h = nullptr;  // pinned host [2]
+  double* part = nullptr;   // diagnostics partials [MODES][4]
+  std::vector<void*> allocs;
+  bool timing = false;
+  double last_ms[3] = {0, 0, 0};
+  hipEvent_t ev[8] = {};
+};
+
+namespace {
+
+template <typename T>
+nsp_status dalloc(nsp_handle* h, T** ptr, size_t count) {
+  void* p = nullptr;
+  hipError_t e = hipMalloc(&p, count * sizeof(T));
+  if (e != hipSuccess) return fail(NSP_ERR_NOMEM, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  h->allocs.push_back(p);
+  *ptr = reinterpret_cast<T*>(p);
+  return NSP_OK;
+}
+
+void free_all(nsp_handle* h) {
+  for (void* p : h->allocs) (void)hipFree(p);
+  h->allocs.clear();
+  if (h->norm_h) (void)hipHostFree(h->norm_h);
+  h->norm_h = nullptr;
+  for (auto& e : h->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  h->stream = nullptr;
+}
+
+nsp_status check_params(const nsp_params* p) {
+  if (!p) return fail(NSP_ERR_INVALID, "null params");
+  if (p->nr < 2 * IW + 2) return fail(NSP_ERR_INVALID, "nr must be >= 8");
+  if (p->nt < 2 || p->nt % 2) return fail(NSP_ERR_INVALID, "nt must be even and >= 2");
+  if (p->nz < 3 || p->nz % 2 == 0) return fail(NSP_ERR_INVALID, "nz must be odd and >= 3");
+  if (!(p->lz > 0) || !(p->re > 0) || !(p->dt > 0)) return fail(NSP_ERR_INVALID, "lz, re, dt must be > 0");
+  if (p->maxit < 1) return fail(NSP_ERR_INVALID, "maxit must be >= 1");
+  if (p->precision != NSP_FP64 && p->precision != NSP_MIXED) return fail(NSP_ERR_INVALID, "bad precision");
+  const int nzf = 3 * (p->nz - 1);
+  if (nzf > 36 * 256) return fail(NSP_ERR_INVALID, "3(nz-1) must be <= 9216");
+  return NSP_OK;
+}
+
+std::vector<double2> twiddles(int L, int count) {
+  std::vector<double2> w(count);
+  for (int e = 0; e < count; e++) {
+    const double a = -PI2 * double(e) / double(L);
+    w[e] = make_double2(std::cos(a), std::sin(a));
+  }
+  return w;
+}
+
+template <typename T>
+nsp_status upload(nsp_handle* h, T** dst, const std::vector<T>& src) {
+  nsp_status st = dalloc(h, dst, src.size());
+  if (st) return st;
+  HIPCHK(hipMemcpy(*dst, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+  return NSP_OK;
+}
+
+nsp_status copy_fields(nsp_handle* h, double2* dst, const double* a, const double* b, const double* c, int where) {
+  const size_t bytes = h->g.n * sizeof(double2);
+  const double* src[3] = {a, b, c};
+  const hipMemcpyKind kind = where == NSP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+  for (int q = 0; q < 3; q++) {
+    if (!src[q]) return fail(NSP_ERR_INVALID, "null field pointer");
+    HIPCHK(hipMemcpyAsync(dst + q * h->g.n, src[q], bytes, kind, h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return NSP_OK;
+}
+
+nsp_status fetch_fields(nsp_handle* h, const double2* src, double* a, double* b, double* c, int where) {
+  const size_t bytes = h->g.n * sizeof(double2);
+  double* dst[3] = {a, b, c};
+  const hipMemcpyKind kind = where == NSP_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+  for (int q = 0; q < 3; q++) {
+    if (!dst[q]) return fail(NSP_ERR_INVALID, "null field pointer");
+    HIPCHK(hipMemcpyAsync(dst[q], src + q * h->g.n, bytes, kind, h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return NSP_OK;
+}
+
+LinArgs lin_args(nsp_handle* h, const double2* N, const double2* u, double2* out, bool predictor) {
+  LinArgs a;
+  a.g = h->g;
+  a.t = h->t;
+  a.b = h->b;
+  a.N = N;
+  a.Nw = h->Nw;
+  a.u = u;
+  a.uw = h->uw;
+  a.out = out;
+  a.rhs = h->rhs;
+  a.p = h->p;
+  a.U = h->U;
+  a.norm = h->norm;
+  a.predictor = predictor ? 1 : 0;
+  return a;
+}
+
+double bits_to_double(unsigned long long b) {
+  double d;
+  std::memcpy(&d, &b, sizeof d);
+  return d;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* nsp_last_error(void) { return g_err.c_str(); }
+
+nsp_status nsp_default_params(nsp_params* p, int nr, int nt, int nz, double lz, double re, double dt) {
+  if (!p) return fail(NSP_ERR_INVALID, "null params");
+  p->nr = nr;
+  p->nt = nt;
+  p->nz = nz;
+  p->lz = lz;
+  p->re = re;
+  p->dt = dt;
+  p->d_im = 0.51;
+  p->maxit = 10;
+  p->tol = 5e-8;
+  p->qdes = 0.5;
+  p->precision = NSP_FP64;
+  return NSP_OK;
+}
+
+nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
+  if (!out) return fail(NSP_ERR_INVALID, "null handle pointer");
+  *out = nullptr;
+  nsp_status st = check_params(p);
+  if (st) return st;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(NSP_ERR_HIP, "no HIP device");
+  if (device < 0 || device >= ndev) return fail(NSP_ERR_INVALID, "device index out of range");
+  HIPCHK(hipSetDevice(device));
+
+  nsp_handle* h = new nsp_handle();
+  h->P = *p;
+  h->device = device;
+  Geom& g = h->g;
+  g.nr = p->nr;
+  g.nt = p->nt;
+  g.nz = p->nz;
+  g.modes = p->nt * p->nz;
+  g.ntp = 3 * p->nt / 2;
+  g.nzf = 3 * (p->nz - 1);
+  g.nh = g.nzf / 2;
+  g.n = size_t(g.nr) * g.modes;
+  g.kz0 = PI2 / p->lz;
+  g.re = p->re;
+  g.dt = p->dt;
+  g.d_im = p->d_im;
+  g.qdes = p->qdes;
+
+  auto bail = [&](nsp_status s) {
+    free_all(h);
+    delete h;
+    return s;
+  };
+#define TRY(x)                 \
+  do {                         \
+    nsp_status s_ = (x);       \
+    if (s_) return bail(s_);   \
+  } while (0)
+#define HTRY(x)                                                                                \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) return bail(fail(NSP_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(e_))); \
+  } while (0)
+
+  HTRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  for (auto& e : h->ev) HTRY(hipEventCreate(&e));
+
+  // radial tables
+  build_radial_tables(g.nr, h->host);
+  TRY(upload(h, &h->t.r, h->host.r));
+  TRY(upload(h, &h->t.D, h->host.D));
+  TRY(upload(h, &h->t.Lu, h->host.Lu));
+  TRY(upload(h, &h->t.Lp, h->host.Lp));
+  TRY(upload(h, &h->t.rdr, h->host.rdr));
+  TRY(upload(h, &h->t.dr0, h->host.dr0));
+  TRY(upload(h, &h->t.dr1, h->host.dr1));
+  TRY(dalloc(h, &h->t.uc, g.nr));
+  TRY(dalloc(h, &h->t.qc, 1));
+
+  // FFT plans
+  std::vector<int> perm_t, perm_h;
+  if (!make_fft_plan(g.ntp, h->nb.pt, perm_t) || !make_fft_plan(g.nh, h->nb.ph, perm_h))
+    return bail(fail(NSP_ERR_INVALID, "transform lengths must factor into 2, 3 and 5"));
+  TRY(upload(h, &h->nb.perm_t, perm_t));
+  TRY(upload(h, &h->nb.perm_h, perm_h));
+  TRY(upload(h, &h->nb.tw_t, twiddles(g.ntp, g.ntp)));
+  TRY(upload(h, &h->nb.tw_h, twiddles(g.nh, g.nh)));
+  TRY(upload(h, &h->nb.tw_z, twiddles(g.nzf, g.nh + 1)));
+  h->nb.wstride = size_t(g.nr) * g.ntp * g.nz;
+
+  // fields
+  const size_t n = g.n;
+  TRY(dalloc(h, &h->u, 3 * n));
+  TRY(dalloc(h, &h->ui, 3 * n));
+  TRY(dalloc(h, &h->un, 3 * n));
+  TRY(dalloc(h, &h->uw, 3 * n));
+  TRY(dalloc(h, &h->Nl, 3 * n));
+  TRY(dalloc(h, &h->Nw, 3 * n));
+  TRY(dalloc(h, &h->rhs, 3 * n));
+  TRY(dalloc(h, &h->p, n));
+  TRY(dalloc(h, &h->U, 4 * n));
+  TRY(dalloc(h, &h->b.uh, 3 * n));
+  TRY(dalloc(h, &h->b.uhP, 3 * n));
+  TRY(dalloc(h, &h->b.invc, 4 * size_t(g.modes)));
+  TRY(dalloc(h, &h->nb.dR, 3 * n));
+  TRY(dalloc(h, &h->nb.W, 9 * h->nb.wstride));
+  TRY(dalloc(h, &h->nb.Y, 3 * h->nb.wstride));
+  TRY(dalloc(h, &h->norm, 2));
+  TRY(dalloc(h, &h->part, 4 * size_t(g.modes)));
+  HTRY(hipHostMalloc((void**)&h->norm_h, 2 * sizeof(unsigned long long), hipHostMallocDefault));
+  HTRY(hipMemsetAsync(h->u, 0, 3 * n * sizeof(double2), h->stream));
+
+  // influence matrix, homogeneous solutions and flux profile (setBCon, setCublasFlux)
+  launch_setup_influence(g, h->t, h->b, h->p, h->U, h->stream);
+  HTRY(hipGetLastError());
+  HTRY(hipStreamSynchronize(h->stream));
+#undef TRY
+#undef HTRY
+  *out = h;
+  return NSP_OK;
+}
+
+nsp_status nsp_destroy(nsp_handle* h) {
+  if (!h) return NSP_OK;
+  (void)hipSetDevice(h->device);
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  free_all(h);
+  delete h;
+  return NSP_OK;
+}
+
+nsp_status nsp_set_state(nsp_handle* h, const double* ur, const double* ut, const double* uz, int where) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  return copy_fields(h, h->u, ur, ut, uz, where);
+}
+
+nsp_status nsp_get_state(nsp_handle* h, double* ur, double* ut, double* uz, int where) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  return fetch_fields(h, h->u, ur, ut, uz, where);
+}
+
+void* nsp_stream(nsp_handle* h) { return h ? (void*)h->stream : nullptr; }
+
+nsp_status nsp_nonlinear(nsp_handle* h, const double* ur, const double* ut, const double* uz, double* dr,
+                         double* dt, double* dz, int where) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  nsp_status st = copy_fields(h, h->ui, ur, ut, uz, where);
+  if (st) return st;
+  launch_nonlinear(h->g, h->t, h->nb, h->ui, h->Nl, h->stream);
+  HIPCHK(hipGetLastError());
+  return fetch_fields(h, h->Nl, dr, dt, dz, where);
+}
+
+nsp_status nsp_set_timing(nsp_handle* h, int enable) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  h->timing = enable != 0;
+  return NSP_OK;
+}
+
+nsp_status nsp_last_timing(nsp_handle* h, double out[3]) {
+  if (!h || !out) return fail(NSP_ERR_INVALID, "null argument");
+  for (int q = 0; q < 3; q++) out[q] = h->last_ms[q];
+  return NSP_OK;
+}
+
+// integrate() loop body (mod_integrate.cu:112-131)
+nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  if (nsteps < 0) return fail(NSP_ERR_INVALID, "nsteps < 0");
+  HIPCHK(hipSetDevice(h->device));
+  hipStream_t s = h->stream;
+  const Geom& g = h->g;
+  for (int step = 0; step < nsteps; step++) {
+    float t_nl = 0.f, t_lin = 0.f, t_all = 0.f, tmp = 0.f;
+    if (h->timing) HIPCHK(hipEventRecord(h->ev[0], s));
+    // nonlinear(u,rhs); predictor(u,rhs,uw,rhsw,ui)
+    if (h->timing) HIPCHK(hipEventRecord(h->ev[2], s));
+    launch_nonlinear(g, h->t, h->nb, h->u, h->Nl, s);
+    if (h->timing) HIPCHK(hipEventRecord(h->ev[3], s));
+    launch_linear_sweep(lin_args(h, h->Nl, h->u, h->ui, true), s);
+    if (h->timing) HIPCHK(hipEventRecord(h->ev[4], s));
+    HIPCHK(hipGetLastError());
+    if (h->timing) {
+      HIPCHK(hipEventSynchronize(h->ev[4]));
+      HIPCHK(hipEventElapsedTime(&tmp, h->ev[2], h->ev[3])); t_nl += tmp;
+      HIPCHK(hipEventElapsedTime(&tmp, h->ev[3], h->ev[4])); t_lin += tmp;
+    }
+    double err = 1e98;
+    int ite = 0;
+    while (ite < h->P.maxit && err > h->P.tol) {
+      ite += 1;
+      if (h->timing) HIPCHK(hipEventRecord(h->ev[2], s));
+      launch_nonlinear(g, h->t, h->nb, h->ui, h->Nl, s);
+      if (h->timing) HIPCHK(hipEventRecord(h->ev[3], s));
+      launch_norm_reset(h->norm, s);
+      launch_linear_sweep(lin_args(h, h->Nl, h->ui, h->un, false), s);
+      if (h->timing) HIPCHK(hipEventRecord(h->ev[4], s));
+      HIPCHK(hipGetLastError());
+      HIPCHK(hipMemcpyAsync(h->norm_h, h->norm, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+      HIPCHK(hipStreamSynchronize(s));
+      if (h->timing) {
+        HIPCHK(hipEventElapsedTime(&tmp, h->ev[2], h->ev[3])); t_nl += tmp;
+        HIPCHK(hipEventElapsedTime(&tmp, h->ev[3], h->ev[4])); t_lin += tmp;
+      }
+      err = bits_to_double(h->norm_h[0]) / bits_to_double(h->norm_h[1]);
+      std::swap(h->ui, h->un);
+    }
+    std::swap(h->u, h->ui);  // the last iterate becomes the state
+    if (h->timing) {
+      HIPCHK(hipEventRecord(h->ev[1], s));
+      HIPCHK(hipEventSynchronize(h->ev[1]));
+      HIPCHK(hipEventElapsedTime(&t_all, h->ev[0], h->ev[1]));
+      h->last_ms[0] = t_nl;
+      h->last_ms[1] = t_lin;
+      h->last_ms[2] = t_all;
+    }
+    if (info) {
+      info[step].n_it = ite;
+      info[step].err = err;
+      info[step].flags = (std::isnan(err) ? NSP_FLAG_NAN : 0) | ((!std::isnan(err) && err > h->P.tol) ? NSP_FLAG_MAXIT : 0);
+    }
+  }
+  return NSP_OK;
+}
+
+nsp_status nsp_tables(nsp_handle* h, double* r, double* D, double* Lu, double* Lp, double* rdr, double* dr0,
+                      double* dr1) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  const RadialTables& T = h->host;
+  auto cp = [](double* dst, const std::vector<double>& v) {
+    if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  };
+  cp(r, T.r);
+  cp(D, T.D);
+  cp(Lu, T.Lu);
+  cp(Lp, T.Lp);
+  cp(rdr, T.rdr);
+  cp(dr0, T.dr0);
+  cp(dr1, T.dr1);
+  return NSP_OK;
+}
+
+nsp_status nsp_radial_tables(int nr, double* r, double* D, double* Lu, double* Lp, double* rdr, double* dr0,
+                             double* dr1) {
+  if (nr < 2 * IW + 2) return fail(NSP_ERR_INVALID, "nr must be >= 8");
+  RadialTables T;
+  build_radial_tables(nr, T);
+  auto cp = [](double* dst, const std::vector<double>& v) {
+    if (dst) std::memcpy(dst, v.data(), v.size() * sizeof(double));
+  };
+  cp(r, T.r);
+  cp(D, T.D);
+  cp(Lu, T.Lu);
+  cp(Lp, T.Lp);
+  cp(rdr, T.rdr);
+  cp(dr0, T.dr0);
+  cp(dr1, T.dr1);
+  return NSP_OK;
+}
+
+nsp_status nsp_diagnostics(nsp_handle* h, double out[7]) {
+  if (!h || !out) return fail(NSP_ERR_INVALID, "null argument");
+  HIPCHK(hipSetDevice(h->device));
+  const Geom& g = h->g;
+  launch_diagnostics(g, h->t, h->u, h->part, h->stream);
+  HIPCHK(hipGetLastError());
+  std::vector<double> part(4 * size_t(g.modes));
+  std::vector<double2> col(3 * size_t(g.nr));
+  HIPCHK(hipMemcpyAsync(part.data(), h->part, part.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  // mode (0,0) uz column for the centre-line, friction and flux values
+  HIPCHK(hipMemcpy2DAsync(col.data(), sizeof(double2), h->u + 2 * g.n, g.modes * sizeof(double2), sizeof(double2),
+                          g.nr, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  const RadialTables& T = h->host;
+  double uc = 0.0, du = 0.0, q = 0.0;
+  for (int i = 0; i < IW + 1; i++) uc += col[i].x * T.dr0[i];  // mod_io_nonlinear.cu:45
+  for (int i = 0; i < STEN; i++) du += col[g.nr - STEN + i].x * T.dr1[i];  // :47
+  du = std::sqrt(std::fabs(du) / g.re);
+  for (int i = 0; i < g.nr; i++) q += T.rdr[i] * col[i].x;
+  double ke = 0.0, d2 = 0.0, wall = 0.0, wdiv = 0.0;
+  for (int k = 0; k < g.modes; k++) {
+    ke += part[4 * size_t(k)];
+    d2 += part[4 * size_t(k) + 1];
+    wall = std::max(wall, part[4 * size_t(k) + 2]);
+    wdiv = std::max(wdiv, part[4 * size_t(k) + 3]);
+  }
+  out[0] = uc;
+  out[1] = du;
+  out[2] = ke;
+  out[3] = std::sqrt(d2);
+  out[4] = wall;
+  out[5] = wdiv;
+  out[6] = q;
+  return NSP_OK;
+}
+
+// ---------------------------------------------------------------- restart files
+nsp_status nsp_write_state_files(nsp_handle* h, const char* a, const char* b, const char* c) {
+  if (!h || !a || !b || !c) return fail(NSP_ERR_INVALID, "null argument");
+  std::vector<double> buf(2 * h->g.n);
+  const char* names[3] = {a, b, c};
+  HIPCHK(hipSetDevice(h->device));
+  for (int q = 0; q < 3; q++) {
+    HIPCHK(hipMemcpy(buf.data(), h->u + q * h->g.n, h->g.n * sizeof(double2), hipMemcpyDeviceToHost));
+    FILE* f = std::fopen(names[q], "wb");
+    if (!f) return fail(NSP_ERR_IO, std::string("writing error: ") + names[q]);
+    const size_t w = std::fwrite(buf.data(), sizeof(double2), h->g.n, f);
+    std::fclose(f);
+    if (w != h->g.n) return fail(NSP_ERR_IO, std::string("writing error: ") + names[q]);
+  }
+  return NSP_OK;
+}
+
+nsp_status nsp_read_state_files(nsp_handle* h, const char* a, const char* b, const char* c) {
+  if (!h || !a || !b || !c) return fail(NSP_ERR_INVALID, "null argument");
+  std::vector<double> buf(6 * h->g.n);
+  const char* names[3] = {a, b, c};
+  for (int q = 0; q < 3; q++) {
+    FILE* f = std::fopen(names[q], "rb");
+    if (!f) return fail(NSP_ERR_IO, std::string("reading error: ") + names[q]);
+    const size_t r = std::fread(buf.data() + 2 * q * h->g.n, sizeof(double2), h->g.n, f);
+    std::fclose(f);
+    if (r != h->g.n) return fail(NSP_ERR_IO, std::string("reading error (short file): ") + names[q]);
+  }
+  return nsp_set_state(h, buf.data(), buf.data() + 2 * h->g.n, buf.data() + 4 * h->g.n, NSP_HOST);
+}
+
+// ---------------------------------------------------------------- initial conditions
+// Reference IC (initField, restart == 0, mod_io_nonlinear.cu:424-449): the
+// physical perturbation of init_per is projected on the retained modes by an
+// exact direct transform on the host (separable: sin/cos(theta) times a
+// z profile), equal to the reference's D2Z + normalize + padBackward.
+nsp_status nsp_reference_ic(nsp_handle* h, double a_p) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  const Geom& g = h->g;
+  const int ntp = g.ntp, nzf = g.nzf;
+  const std::vector<double>& r = h->host.r;
+  std::vector<double2> f(3 * g.n, make_double2(0.0, 0.0));
+  // z profile e^{-10 sin^2(pi z/L)} sampled at z_l = l/nzf, forward DFT
+  std::vector<double> prof(nzf);
+  for (int l = 0; l < nzf; l++) {
+    const double z = double(l) / double(nzf);
+    double ux = std::sin(0.5 * PI2 * z);
+    ux = ux * ux;
+    prof[l] = std::exp(-10.0 * ux);
+  }
+  std::vector<double2> zc(g.nz);
+  for (int l = 0; l < g.nz; l++) {
+    double sr = 0.0, si = 0.0;
+    for (int n = 0; n < nzf; n++) {
+      const double a = -PI2 * double((long long)l * n % nzf) / double(nzf);
+      sr += prof[n] * std::cos(a);
+      si += prof[n] * std::sin(a);
+    }
+    zc[l] = make_double2(sr / nzf, si / nzf);
+  }
+  // sin(theta) -> m=+1: -i/2, m=-1: +i/2 ; cos(theta) -> m=+-1: 1/2
+  for (int i = 0; i < g.nr; i++) {
+    const double rr = r[i];
+    const double ur = (1.0 - rr * rr) * (1.0 - rr * rr);
+    const double ut = ur - 4.0 * rr * rr * (1.0 - rr * rr);
+    for (int l = 0; l < g.nz - 1; l++) {
+      const double2 z = zc[l];
+      // m = +1 (it = 1) and m = -1 (it = nt-1)
+      const size_t hp = (size_t)i * g.modes + 1 * g.nz + l;
+      const size_t hm = (size_t)i * g.modes + (size_t)(g.nt - 1) * g.nz + l;
+      f[hp] = make_double2(a_p * ur * 0.5 * z.y, -a_p * ur * 0.5 * z.x);        // (-i/2) z
+      f[hm] = make_double2(-a_p * ur * 0.5 * z.y, a_p * ur * 0.5 * z.x);        // (+i/2) z
+      f[g.n + hp] = make_double2(a_p * ut * 0.5 * z.x, a_p * ut * 0.5 * z.y);
+      f[g.n + hm] = make_double2(a_p * ut * 0.5 * z.x, a_p * ut * 0.5 * z.y);
+    }
+    f[2 * g.n + (size_t)i * g.modes].x = 1.0 - rr * rr;  // base_k
+  }
+  if (g.nt == 2) {  // m=+-1 would alias onto the zeroed -NT/2 row
+    for (size_t q = 0; q < 2 * g.n; q++) f[q] = make_double2(0.0, 0.0);
+  }
+  (void)ntp;
+  std::vector<double> host(6 * g.n);
+  return nsp_set_state(h, (const double*)f.data(), (const double*)(f.data() + g.n),
+                       (const double*)(f.data() + 2 * g.n), NSP_HOST);
+}
+
+// Seeded, analytically divergence-free, no-slip perturbation on top of the
+// flux-matched Poiseuille profile (see nsp.h).  For a mode (m, kz) with
+// q = |m| and w(r) = (1 - r^2)^2:
+//   kz != 0, q >= 1: ur = a r^(q-1) w, ut = i sgn(m) a r^(q-1) w,
+//                    uz = -4 i a r^q (1 - r^2) / kz;  (1/r)(r ur)' + i m ut/r = -4 a r^q (1-r^2)
+//   kz != 0, q == 0: ur = a r w, ut = b r w, uz = i a (2w + r w') / kz
+//   kz == 0, q >= 1: ur = a r^(q-1) w, ut = i (r ur)'/m, uz = c r^q (1 - r^2)
+//   kz == 0, q == 0: ur = 0, ut = b r w (real), uz = 0
+// Modes with l = 0, m < 0 are the conjugates of m > 0 (real field).
+nsp_status nsp_seeded_ic(nsp_handle* h, unsigned long long seed, int mmax, int lmax, int kind, double rms) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  const Geom& g = h->g;
+  if (mmax < 0 || lmax < 0 || kind < 0 || kind > 1) return fail(NSP_ERR_INVALID, "bad IC arguments");
+  mmax = std::min(mmax, g.nt / 2 - 1);
+  lmax = std::min(lmax, g.nz - 2);
+  const std::vector<double>& r = h->host.r;
+  const std::vector<double>& rdr = h->host.rdr;
+  std::mt19937_64 rng(seed);
+  auto gauss = [&]() -> double2 {  // Box-Muller
+    const double u1 = (double(rng() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const double u2 = (double(rng() >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+    const double rad = std::sqrt(-2.0 * std::log(u1)), ang = PI2 * u2;
+    return make_double2(rad * std::cos(ang), rad * std::sin(ang));
+  };
+  std::vector<double2> f(3 * g.n, make_double2(0.0, 0.0));
+  auto cm = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+  for (int l = 0; l <= lmax; l++) {
+    for (int m = -mmax; m <= mmax; m++) {
+      if (l == 0 && m < 0) continue;
+      const double kz = g.kz0 * double(l);
+      const int q = std::abs(m);
+      double env = 1.0;
+      if (kind == 1) {
+        const double kk = std::sqrt(kz * kz + double(m * m));
+        env = kk > 0 ? std::pow(kk, -5.0 / 3.0) : 0.0;
+      }
+      double2 a = gauss(), b = gauss(), c = gauss();
+      a = make_double2(a.x * env, a.y * env);
+      b = make_double2(b.x * env, b.y * env);
+      c = make_double2(c.x * env, c.y * env);
+      if (l == 0) {  // kz = 0 modes of a real field: keep a real radial structure
+        if (m == 0) { a = make_double2(0, 0); b.y = 0.0; c = make_double2(0, 0); }
+      }
+      const int it = m >= 0 ? m : m + g.nt;
+      for (int i = 0; i < g.nr; i++) {
+        const double rr = r[i], w = (1 - rr * rr) * (1 - rr * rr), wp = -4.0 * rr * (1 - rr * rr);
+        double2 ur, ut, uz;
+        if (l > 0 && q >= 1) {
+          const double s = std::pow(rr, q - 1) * w;
+          ur = make_double2(a.x * s, a.y * s);
+          const double sg = m > 0 ? 1.0 : -1.0;
+          ut = make_double2(-sg * a.y * s, sg * a.x * s);
+          const double t = -4.0 * std::pow(rr, q) * (1 - rr * rr) / kz;
+          uz = make_double2(-a.y * t, a.x * t);  // i * a * t
+        } else if (l > 0) {
+          ur = make_double2(a.x * rr * w, a.y * rr * w);
+          ut = make_double2(b.x * rr * w, b.y * rr * w);
+          const double t = (2 * w + rr * wp) / kz;
+          uz = make_double2(-a.y * t, a.x * t);
+        } else if (q >= 1) {
+          const double s = std::pow(rr, q - 1) * w;
+          ur = make_double2(a.x * s, a.y * s);
+          // (r ur)' = a [q r^(q-1) w + r^q w']
+          const double dd = (q * std::pow(rr, q - 1) * w + std::pow(rr, q) * wp) / double(m);
+          ut = make_double2(-a.y * dd, a.x * dd);  // i a dd
+          const double t = std::pow(rr, q) * (1 - rr * rr);
+          uz = make_double2(c.x * t, c.y * t);
+        } else {
+          ur = make_double2(0, 0);
+          ut = make_double2(b.x * rr * w, 0.0);
+          uz = make_double2(0, 0);
+        }
+        const size_t hh = (size_t)i * g.modes + (size_t)it * g.nz + l;
+        f[hh] = ur;
+        f[g.n + hh] = ut;
+        f[2 * g.n + hh] = uz;
+        if (l == 0 && m > 0) {  // conjugate partner
+          const size_t hc = (size_t)i * g.modes + (size_t)(g.nt - m) * g.nz;
+          f[hc] = make_double2(ur.x, -ur.y);
+          f[g.n + hc] = make_double2(ut.x, -ut.y);
+          f[2 * g.n + hc] = make_double2(uz.x, -uz.y);
+        }
+      }
+      (void)cm;
+    }
+  }
+  // rms of the perturbation: sqrt(sum_modes w_l sum_r rdr |u|^2)
+  double e = 0.0;
+  for (int i = 0; i < g.nr; i++)
+    for (int k = 0; k < g.modes; k++) {
+      const double wl = (k % g.nz == 0) ? 1.0 : 2.0;
+      for (int c2 = 0; c2 < 3; c2++) {
+        const double2 v = f[c2 * g.n + (size_t)i * g.modes + k];
+        e += wl * rdr[i] * (v.x * v.x + v.y * v.y);
+      }
+    }
+  const double scale = e > 0 ? rms / std::sqrt(e) : 0.0;
+  for (auto& v : f) v = make_double2(v.x * scale, v.y * scale);
+  // flux-matched Poiseuille
+  double q0 = 0.0;
+  for (int i = 0; i < g.nr; i++) q0 += rdr[i] * (1 - r[i] * r[i]);
+  const double alpha = h->P.qdes / q0;
+  for (int i = 0; i < g.nr; i++) f[2 * g.n + (size_t)i * g.modes].x += alpha * (1 - r[i] * r[i]);
+  return nsp_set_state(h, (const double*)f.data(), (const double*)(f.data() + g.n),
+                       (const double*)(f.data() + 2 * g.n), NSP_HOST);
+}
+
+}  // extern "C"

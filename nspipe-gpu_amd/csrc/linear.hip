@@ -1,0 +1,540 @@
+// Linear phase of the time step: one thread per Fourier mode (it, l) walks
+// the whole radial column, fusing what the reference launches as ~25
+// separate whole-field kernels per predictor/corrector
+// (mod_integrate.cu:164-228):
+//   rhs assembly (decoupleForward, addRHS_k, addLap_k), decoupleBackward,
+//   div_k + write_bcon, LUsolve_p_k, addGrad_k, decoupleForward, write_bcon,
+//   LUsolve_up/um/uz_k, corBCon (divWall_k, compcoef_k, addBCon_k),
+//   decoupleBackward, modeZZ_k, adj_flux (mode (0,0) only), and for the
+//   corrector the mcor_k / iden_max convergence measure as a device max.
+// Column vectors live in mode-fastest global arrays, so every access of a
+// wavefront is one contiguous 1 KiB (FP64 complex) line.
+#include "nsp_internal.h"
+
+namespace nsp {
+
+struct Mode {
+  int k, it, l, m;
+  double kt, kz;
+};
+
+__device__ __forceinline__ Mode mode_of(const Geom& g, int k) {
+  Mode q;
+  q.k = k;
+  q.it = k / g.nz;
+  q.l = k - q.it * g.nz;
+  q.m = q.it < g.nt / 2 ? q.it : q.it - g.nt;
+  q.kt = double(q.m);  // (PI2/LT)*it with LT = PI2 (head.h:21)
+  q.kz = g.kz0 * double(q.l);
+  return q;
+}
+
+__device__ __forceinline__ size_t at(const Geom& g, int i, int k) { return (size_t)i * g.modes + k; }
+
+// Radial stencil  sum_j tab[ip0 + 7i + j + i0] f[left+j]  (derivr_k, mod_radialFD.cu:10-30)
+__device__ __forceinline__ double2 stencil(const Geom& g, const double* tab, int ip0, int i,
+                                           const double2* f, int k) {
+  const int left = max(0, i - IW), right = min(i + IW, g.nr - 1);
+  const int len = right + 1 - left, i0 = i + IW - right;
+  double px = 0.0, py = 0.0;
+  for (int j = 0; j < len; j++) {
+    const double de = tab[ip0 + STEN * i + j + i0];
+    const double2 a = f[at(g, left + j, k)];
+    px += de * a.x;
+    py += de * a.y;
+  }
+  return make_double2(px, py);
+}
+
+// Banded LU without pivoting (Doolittle, half band IW) fused with forward
+// elimination, then back substitution: LUsolve_{up,um,uz,p}_k
+// (mod_linear.cu:55-646).  kind 0: u+, 1: u-, 2: uz, 3: p.  x may alias rhs.
+// U factor rows live in the mode-fastest scratch array Us.
+__device__ void lu_solve(const Geom& g, const DevTables& t, double* Us, int kind, const Mode& q,
+                         const double2* rhs, double2* x) {
+  const double* LHS = (kind == 3) ? t.Lp : t.Lu;
+  const int fp = (kind <= 1) ? 1 : 0;
+  const int ip0 = abs((abs(q.m) % 2) - fp) * STEN * g.nr;
+  const bool vel = kind < 3;
+  const bool p00 = (kind == 3) && (q.l == 0 && q.m == 0);
+  const double a = g.d_im / g.re, idt = 1.0 / g.dt;
+  const double kt = q.kt, kz = q.kz;
+  const int k = q.k;
+  const int nr = g.nr;
+#define UF(row, j) Us[(size_t)((row) * (1 + IW) + (j)) * g.modes + k]
+  for (int i = 0; i < nr; i++) {
+    const int left = max(0, i - IW), right = min(i + IW, nr - 1);
+    const int i0 = i + IW - right;
+    const int nl = i - left;
+    const double r2 = t.r[i] * t.r[i];
+    const bool last = (i == nr - 1);
+    double2 xi = rhs[at(g, i, k)];
+    double L[IW];
+#pragma unroll
+    for (int j = 0; j < IW; j++) {
+      if (j < nl) {
+        double XM = LHS[ip0 + j + i0 + STEN * i];
+        if (!last) {
+          if (vel) XM = -a * XM;
+        } else if (p00) {
+          XM = 0.0;
+        }
+        double Lj = XM;
+#pragma unroll
+        for (int h = 0; h < IW; h++)
+          if (h < j) Lj -= L[h] * UF(h + left, j - h);
+        Lj = Lj / UF(j + left, 0);
+        L[j] = Lj;
+        const double2 xj = x[at(g, j + left, k)];
+        xi.x -= Lj * xj.x;
+        xi.y -= Lj * xj.y;
+      }
+    }
+    int coun = nl;
+    for (int j = 0; j < right - i + 1; j++) {
+      double XM = LHS[ip0 + STEN * i + i0 + coun];
+      if (!last) {
+        if (j == 0) {
+          if (kind == 0) XM += -(1.0 / r2) - (2.0 * kt / r2) - kz * kz - kt * kt / r2;
+          else if (kind == 1) XM += -(1.0 / r2) + (2.0 * kt / r2) - kz * kz - kt * kt / r2;
+          else XM += -kz * kz - kt * kt / r2;
+          if (vel) XM = idt - a * XM;
+        } else if (vel) {
+          XM = -a * XM;
+        }
+      } else if (p00) {
+        XM = (j == 0) ? 1.0 : 0.0;
+      }
+      const int hleft = max(0, coun - IW);
+      coun += 1;
+      double Uij = XM;
+#pragma unroll
+      for (int h = 0; h < IW; h++)
+        if (h >= hleft && h < nl) Uij -= L[h] * UF(h + left, j + nl - h);
+      UF(i, j) = Uij;
+    }
+    x[at(g, i, k)] = xi;
+  }
+  {
+    const int i = nr - 1;
+    double2 xi = x[at(g, i, k)];
+    const double d = UF(i, 0);
+    xi.x = xi.x / d;
+    xi.y = xi.y / d;
+    x[at(g, i, k)] = xi;
+  }
+  for (int i = nr - 2; i > -1; i--) {
+    const int right = min(i + IW, nr - 1);
+    double2 xi = x[at(g, i, k)];
+    for (int j = 1; j < right - i + 1; j++) {
+      const double u = UF(i, j);
+      const double2 xj = x[at(g, i + j, k)];
+      xi.x -= u * xj.x;
+      xi.y -= u * xj.y;
+    }
+    const double d = UF(i, 0);
+    xi.x = xi.x / d;
+    xi.y = xi.y / d;
+    x[at(g, i, k)] = xi;
+  }
+#undef UF
+}
+
+// divWall_k (mod_radialFD.cu:34-63) on one column in r/theta form
+__device__ __forceinline__ double2 div_wall(const Geom& g, const DevTables& t, const Mode& q,
+                                            const double2* ur, const double2* ut, const double2* uz) {
+  const int i = g.nr - 1;
+  const int ip0 = abs((abs(q.m) % 2) - 1) * STEN * g.nr;
+  double2 prod = stencil(g, t.D, ip0, i, ur, q.k);
+  const size_t h = at(g, i, q.k);
+  prod.x += ur[h].x;
+  prod.y += ur[h].y;
+  prod.x += -q.kt * ut[h].y;
+  prod.y += q.kt * ut[h].x;
+  prod.x += -q.kz * uz[h].y;
+  prod.y += q.kz * uz[h].x;
+  return prod;
+}
+
+__device__ __forceinline__ void to_pm(double2& a, double2& b) {  // decoupleForward_k
+  const double2 ur = a, ut = b;
+  a = make_double2(ur.x - ut.y, ur.y + ut.x);
+  b = make_double2(ur.x + ut.y, ur.y - ut.x);
+}
+__device__ __forceinline__ void to_rt(double2& a, double2& b) {  // decoupleBackward_k
+  const double2 up = a, um = b;
+  a = make_double2(0.5 * (up.x + um.x), 0.5 * (up.y + um.y));
+  b = make_double2(0.5 * (up.y - um.y), -0.5 * (up.x - um.x));
+}
+__device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+
+// ------------------------------------------------------------------ sweep
+__global__ __launch_bounds__(256) void linear_sweep_kernel(LinArgs A) {
+  const Geom& g = A.g;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.modes) return;
+  const Mode q = mode_of(g, k);
+  const size_t n = g.n;
+  const int nr = g.nr;
+  double2* R0 = A.rhs;
+  double2* R1 = A.rhs + n;
+  double2* R2 = A.rhs + 2 * n;
+  const double d_im = g.d_im;
+
+  // (1) rhs = N(u) [predictor] or d_im N(u^k) + (1-d_im) N(u^n) [corrector], +/- form
+  for (int i = 0; i < nr; i++) {
+    const size_t h = at(g, i, k);
+    double2 nr_ = A.N[h], nt_ = A.N[n + h];
+    const double2 nz_ = A.N[2 * n + h];
+    to_pm(nr_, nt_);
+    if (A.predictor) {
+      A.Nw[h] = nr_;
+      A.Nw[n + h] = nt_;
+      A.Nw[2 * n + h] = nz_;
+      double2 a = A.u[h], b = A.u[n + h];
+      to_pm(a, b);
+      A.uw[h] = a;
+      A.uw[n + h] = b;
+      A.uw[2 * n + h] = A.u[2 * n + h];
+      R0[h] = nr_;
+      R1[h] = nt_;
+      R2[h] = nz_;
+    } else {  // addRHS_k (mod_integrate.cu:43-59)
+      const double2 w0 = A.Nw[h], w1 = A.Nw[n + h], w2 = A.Nw[2 * n + h];
+      R0[h] = make_double2(d_im * nr_.x + (1.0 - d_im) * w0.x, d_im * nr_.y + (1.0 - d_im) * w0.y);
+      R1[h] = make_double2(d_im * nt_.x + (1.0 - d_im) * w1.x, d_im * nt_.y + (1.0 - d_im) * w1.y);
+      R2[h] = make_double2(d_im * nz_.x + (1.0 - d_im) * w2.x, d_im * nz_.y + (1.0 - d_im) * w2.y);
+    }
+  }
+  // (2) explicit Crank-Nicolson part, addLap_k (mod_linear.cu:12-51)
+  {
+    const double cn = (1.0 - d_im) / g.re, idt = 1.0 / g.dt;
+    for (int i = 0; i < nr; i++) {
+      const size_t h = at(g, i, k);
+      const double r2 = A.t.r[i] * A.t.r[i];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const int fp = (c < 2) ? 1 : 0;
+        const int ip0 = abs((abs(q.m) % 2) - fp) * STEN * nr;
+        const double2* uwc = A.uw + c * n;
+        double2 prod = stencil(g, A.t.Lu, ip0, i, uwc, k);
+        const double2 aux = uwc[h];
+        double de = -q.kz * q.kz - q.kt * q.kt / r2;
+        if (c == 0) de += -(1.0 / r2) - (2.0 * q.kt / r2);
+        else if (c == 1) de += -(1.0 / r2) + (2.0 * q.kt / r2);
+        prod.x += de * aux.x;
+        prod.y += de * aux.y;
+        prod.x *= cn;
+        prod.y *= cn;
+        prod.x += idt * aux.x;
+        prod.y += idt * aux.y;
+        double2* Rc = A.rhs + c * n;
+        double2 rv = Rc[h];
+        rv.x += prod.x;
+        rv.y += prod.y;
+        Rc[h] = rv;
+      }
+    }
+  }
+  // (3) back to r/theta, divergence with zero wall row (div_k, write_bcon_k)
+  for (int i = 0; i < nr; i++) {
+    const size_t h = at(g, i, k);
+    double2 a = R0[h], b = R1[h];
+    to_rt(a, b);
+    R0[h] = a;
+    R1[h] = b;
+  }
+  {
+    const int ip0 = abs((abs(q.m) % 2) - 1) * STEN * nr;
+    for (int i = 0; i < nr - 1; i++) {
+      const size_t h = at(g, i, k);
+      const double rK = A.t.r[i];
+      double2 prod = stencil(g, A.t.D, ip0, i, R0, k);
+      const double2 ur = R0[h], ut = R1[h], uz = R2[h];
+      prod.x += ur.x / rK;
+      prod.y += ur.y / rK;
+      prod.x += -q.kt * ut.y / rK;
+      prod.y += q.kt * ut.x / rK;
+      prod.x += -q.kz * uz.y;
+      prod.y += q.kz * uz.x;
+      A.p[h] = prod;
+    }
+    A.p[at(g, nr - 1, k)] = make_double2(0.0, 0.0);
+  }
+  // (4) pressure Poisson solve, in place
+  lu_solve(g, A.t, A.U, 3, q, A.p, A.p);
+  // (5) rhs -= grad p (addGrad_k), to +/-, zero wall rows
+  {
+    const int ip0 = abs((abs(q.m) % 2) - 0) * STEN * nr;
+    for (int i = 0; i < nr; i++) {
+      const size_t h = at(g, i, k);
+      const double rK = A.t.r[i];
+      const double2 prod = stencil(g, A.t.D, ip0, i, A.p, k);
+      const double2 ph = A.p[h];
+      double2 a = R0[h], b = R1[h], c = R2[h];
+      a.x -= prod.x;
+      a.y -= prod.y;
+      b.x -= -q.kt * ph.y / rK;
+      b.y -= q.kt * ph.x / rK;
+      c.x -= -q.kz * ph.y;
+      c.y -= q.kz * ph.x;
+      to_pm(a, b);
+      if (i == nr - 1) {
+        a = b = c = make_double2(0.0, 0.0);
+      }
+      R0[h] = a;
+      R1[h] = b;
+      R2[h] = c;
+    }
+  }
+  // (6) Helmholtz solves for u+, u-, uz
+  double2* O0 = A.out;
+  double2* O1 = A.out + n;
+  double2* O2 = A.out + 2 * n;
+  lu_solve(g, A.t, A.U, 0, q, R0, O0);
+  lu_solve(g, A.t, A.U, 1, q, R1, O1);
+  lu_solve(g, A.t, A.U, 2, q, R2, O2);
+  // (7) influence-matrix correction, corBCon (mod_boundary.cu:269-278)
+  for (int i = 0; i < nr; i++) {  // decoupleBackward / decoupleForward round trip
+    const size_t h = at(g, i, k);
+    double2 a = O0[h], b = O1[h];
+    to_rt(a, b);
+    O0[h] = a;
+    O1[h] = b;
+  }
+  const double2 dv = div_wall(g, A.t, q, O0, O1, O2);
+  for (int i = 0; i < nr; i++) {
+    const size_t h = at(g, i, k);
+    double2 a = O0[h], b = O1[h];
+    to_pm(a, b);
+    O0[h] = a;
+    O1[h] = b;
+  }
+  {
+    const double2 ndv = make_double2(-dv.x, -dv.y);  // compcoef_k
+    double2 cf[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) cf[j] = cmul2(ndv, A.b.invc[(size_t)k * 4 + j]);
+    for (int i = 0; i < nr; i++) {  // addBCon_k, then decoupleBackward
+      const size_t h = at(g, i, k);
+      double2 o[3];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double2 hh = A.b.uh[c * n + h], hp = A.b.uhP[c * n + h];
+        double2 v = A.out[c * n + h];
+        const double ax = cf[c].x * hh.x - cf[c].y * hh.y + cf[3].x * hp.x - cf[3].y * hp.y;
+        const double ay = cf[c].x * hh.y + cf[c].y * hh.x + cf[3].x * hp.y + cf[3].y * hp.x;
+        v.x += ax;
+        v.y += ay;
+        o[c] = v;
+      }
+      to_rt(o[0], o[1]);
+      O0[h] = o[0];
+      O1[h] = o[1];
+      O2[h] = o[2];
+    }
+  }
+  // (8) mode (0,0): modeZZ_k and adj_flux (mod_integrate.cu:63-68, mod_cublasflux.cu:331-344)
+  if (k == 0) {
+    double qa = 0.0;
+    for (int i = 0; i < nr; i++) {
+      const size_t h = at(g, i, 0);
+      O0[h] = make_double2(0.0, 0.0);
+      O1[h].y = 0.0;
+      O2[h].y = 0.0;
+    }
+    for (int i = 0; i < nr; i++) qa += A.t.rdr[i] * O2[at(g, i, 0)].x;
+    const double corr = (g.qdes - qa) / A.t.qc[0];
+    for (int i = 0; i < nr; i++) O2[at(g, i, 0)].x += corr * A.t.uc[i];
+  }
+  // (9) corrector convergence measure (mcor_k, iden_max)
+  if (!A.predictor) {
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < nr; i++) {
+      const size_t h = at(g, i, k);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double2 u1 = A.u[c * n + h], u2 = A.out[c * n + h];
+        const double dx = u1.x - u2.x, dy = u1.y - u2.y;
+        const double a1 = dx * dx + dy * dy, a2 = u1.x * u1.x + u1.y * u1.y;
+        // NaN-propagating max: a NaN entry makes the measure NaN
+        num = (a1 > num || a1 != a1) ? a1 : num;
+        den = (a2 > den || a2 != a2) ? a2 : den;
+      }
+    }
+    // non-negative doubles (and NaN) order like their bit patterns
+    atomicMax(&A.norm[0], (unsigned long long)__double_as_longlong(num));
+    atomicMax(&A.norm[1], (unsigned long long)__double_as_longlong(den));
+  }
+}
+
+__global__ void norm_reset_kernel(unsigned long long* norm) {
+  if (threadIdx.x < 2) norm[threadIdx.x] = 0ull;
+}
+
+// ------------------------------------------------------------------ setup
+// setBCon (mod_boundary.cu:205-255) and setCublasFlux (mod_cublasflux.cu:297-321),
+// one thread per mode.
+__global__ __launch_bounds__(256) void setup_influence_kernel(Geom g, DevTables t, DevInfluence b,
+                                                              double2* P, double* Us) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.modes) return;
+  const Mode q = mode_of(g, k);
+  const size_t n = g.n;
+  const int nr = g.nr;
+  auto unit_rhs = [&](double2 wall) {
+    for (int i = 0; i < nr - 1; i++) P[at(g, i, k)] = make_double2(0.0, 0.0);
+    P[at(g, nr - 1, k)] = wall;
+  };
+  unit_rhs(make_double2(1.0, 0.0));
+  lu_solve(g, t, Us, 0, q, P, b.uh);
+  unit_rhs(make_double2(1.0, 0.0));
+  lu_solve(g, t, Us, 1, q, P, b.uh + n);
+  unit_rhs(make_double2(0.0, 1.0));
+  lu_solve(g, t, Us, 2, q, P, b.uh + 2 * n);
+  unit_rhs(make_double2(-1.0, 0.0));
+  lu_solve(g, t, Us, 3, q, P, P);  // P now holds the homogeneous pressure p_h
+  // wall divergences of the four homogeneous solutions (in r/theta form)
+  const size_t hw = at(g, nr - 1, k);
+  double2 d1, d2, d3, d4;
+  {
+    const int ip0 = abs((abs(q.m) % 2) - 1) * STEN * nr;
+    const int i = nr - 1, left = i - IW;
+    // DIV1: (u_h+, 0) -> r/theta: ur = up/2, ut = (up.y/2, -up.x/2)
+    double px = 0, py = 0, sx = 0, sy = 0;
+    for (int j = 0; j < IW + 1; j++) {
+      const double de = t.D[ip0 + STEN * i + j + IW];
+      double2 a = b.uh[at(g, left + j, k)], z0 = make_double2(0.0, 0.0);
+      to_rt(a, z0);
+      px += de * a.x;
+      py += de * a.y;
+      double2 c0 = make_double2(0.0, 0.0), bm = b.uh[n + at(g, left + j, k)];
+      to_rt(c0, bm);
+      sx += de * c0.x;
+      sy += de * c0.y;
+    }
+    double2 a = b.uh[hw], z0 = make_double2(0.0, 0.0);
+    to_rt(a, z0);
+    d1 = make_double2(px + a.x - q.kt * z0.y, py + a.y + q.kt * z0.x);
+    double2 c0 = make_double2(0.0, 0.0), bm = b.uh[n + hw];
+    to_rt(c0, bm);
+    d2 = make_double2(sx + c0.x - q.kt * bm.y, sy + c0.y + q.kt * bm.x);
+    // DIV3: (0, 0, u_hz)
+    const double2 uzw = b.uh[2 * n + hw];
+    d3 = make_double2(0.0 + 0.0 - q.kz * uzw.y, 0.0 + 0.0 + q.kz * uzw.x);
+  }
+  // u_hP = grad p_h (derivr flag 0, derivt, derivz) in r/theta form
+  {
+    const int ip0 = abs((abs(q.m) % 2) - 0) * STEN * nr;
+    for (int i = 0; i < nr; i++) {
+      const size_t h = at(g, i, k);
+      const double2 dr = stencil(g, t.D, ip0, i, P, k);
+      const double2 ph = P[h];
+      b.uhP[h] = dr;
+      b.uhP[n + h] = make_double2(-q.kt * ph.y / t.r[i], q.kt * ph.x / t.r[i]);
+      b.uhP[2 * n + h] = make_double2(-q.kz * ph.y, q.kz * ph.x);
+    }
+    d4 = div_wall(g, t, q, b.uhP, b.uhP + n, b.uhP + 2 * n);
+    for (int i = 0; i < nr; i++) {
+      const size_t h = at(g, i, k);
+      double2 a = b.uhP[h], c = b.uhP[n + h];
+      to_pm(a, c);
+      b.uhP[h] = a;
+      b.uhP[n + h] = c;
+    }
+  }
+  // column 3 of the inverse of [[1,0,0,a14],[0,1,0,a24],[0,0,i,a34],[a41,a42,a43,a44]]
+  // (calc_invM_k, mod_boundary.cu:66-196: entries 3, 7, 11, 15)
+  {
+    const double2 a14 = b.uhP[hw], a24 = b.uhP[n + hw], a34 = b.uhP[2 * n + hw];
+    const double2 a41 = d1, a42 = d2, a43 = d3, a44 = d4;
+    double2 aux, Det;
+    aux.x = a44.x - a24.x * a42.x + a24.y * a42.y - a14.x * a41.x + a14.y * a41.y;
+    aux.y = a44.y - a24.x * a42.y - a24.y * a42.x - a14.x * a41.y - a14.y * a41.x;
+    Det.x = -aux.y - a34.x * a43.x + a34.y * a43.y;
+    Det.y = aux.x - a34.x * a43.y - a34.y * a43.x;
+    const double D1 = Det.x * Det.x + Det.y * Det.y;
+    Det.x = Det.x / D1;
+    Det.y = -Det.y / D1;
+    double2 IA[4];
+    IA[0] = make_double2(a14.y, -a14.x);
+    IA[1] = make_double2(a24.y, -a24.x);
+    IA[2] = make_double2(-a34.x, -a34.y);
+    IA[3] = make_double2(0.0, 1.0);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const double ax = IA[j].x * Det.x - IA[j].y * Det.y;
+      const double ay = IA[j].x * Det.y + IA[j].y * Det.x;
+      b.invc[(size_t)k * 4 + j] = make_double2(ax, ay);
+    }
+  }
+  // flux profile: mode (0,0) uz-Helmholtz solution for rhs 1 with wall value 0
+  if (k == 0) {
+    for (int i = 0; i < nr - 1; i++) P[at(g, i, 0)] = make_double2(1.0, 0.0);
+    P[at(g, nr - 1, 0)] = make_double2(0.0, 0.0);
+    lu_solve(g, t, Us, 2, q, P, P);
+    double qc = 0.0;
+    for (int i = 0; i < nr; i++) {
+      t.uc[i] = P[at(g, i, 0)].x;
+      qc += t.rdr[i] * t.uc[i];
+    }
+    t.qc[0] = qc;
+  }
+}
+
+// ------------------------------------------------------------------ diagnostics
+// Per-mode partial sums: [0] KE, [1] |rdr|-weighted |div|^2, [2] max wall |u|,
+// [3] |wall divergence|.
+__global__ __launch_bounds__(256) void diag_kernel(Geom g, DevTables t, const double2* u, double* part) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.modes) return;
+  const Mode q = mode_of(g, k);
+  const size_t n = g.n;
+  const double w = (q.l == 0) ? 1.0 : 2.0;
+  double ke = 0.0, d2 = 0.0;
+  const int ip0 = abs((abs(q.m) % 2) - 1) * STEN * g.nr;
+  for (int i = 0; i < g.nr; i++) {
+    const size_t h = at(g, i, k);
+    for (int c = 0; c < 3; c++) {
+      const double2 a = u[c * n + h];
+      ke += 0.5 * w * t.rdr[i] * (a.x * a.x + a.y * a.y);
+    }
+    const double rK = t.r[i];
+    double2 prod = stencil(g, t.D, ip0, i, u, k);
+    const double2 ur = u[h], ut = u[n + h], uz = u[2 * n + h];
+    prod.x += ur.x / rK;
+    prod.y += ur.y / rK;
+    prod.x += -q.kt * ut.y / rK;
+    prod.y += q.kt * ut.x / rK;
+    prod.x += -q.kz * uz.y;
+    prod.y += q.kz * uz.x;
+    d2 += w * fabs(t.rdr[i]) * (prod.x * prod.x + prod.y * prod.y);
+  }
+  double wall = 0.0;
+  for (int c = 0; c < 3; c++) {
+    const double2 a = u[c * n + at(g, g.nr - 1, k)];
+    wall = fmax(wall, sqrt(a.x * a.x + a.y * a.y));
+  }
+  const double2 dw = div_wall(g, t, q, u, u + n, u + 2 * n);
+  part[(size_t)k * 4 + 0] = ke;
+  part[(size_t)k * 4 + 1] = d2;
+  part[(size_t)k * 4 + 2] = wall;
+  part[(size_t)k * 4 + 3] = sqrt(dw.x * dw.x + dw.y * dw.y);
+}
+
+// ------------------------------------------------------------------ launchers
+void launch_linear_sweep(const LinArgs& a, hipStream_t s) {
+  linear_sweep_kernel<<<(a.g.modes + 255) / 256, 256, 0, s>>>(a);
+}
+void launch_norm_reset(unsigned long long* norm, hipStream_t s) { norm_reset_kernel<<<1, 64, 0, s>>>(norm); }
+void launch_setup_influence(const Geom& g, const DevTables& t, const DevInfluence& b, double2* P, double* U,
+                            hipStream_t s) {
+  setup_influence_kernel<<<(g.modes + 255) / 256, 256, 0, s>>>(g, t, b, P, U);
+}
+void launch_diagnostics(const Geom& g, const DevTables& t, const double2* u, double* part, hipStream_t s) {
+  diag_kernel<<<(g.modes + 255) / 256, 256, 0, s>>>(g, t, u, part);
+}
+
+}  // namespace nsp

@@ -1,0 +1,108 @@
+// Internal declarations shared by the host and HIP sources of libnsp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nsp.h"
+
+namespace nsp {
+
+constexpr double PI2 = 6.283185307179586;  // head.h:12
+constexpr int STEN = 7;                    // head.h:42
+constexpr int IW = 3;                      // head.h:43
+constexpr int MAXRAD = 16;
+
+// ----------------------------------------------------------------- tables
+// Host FP64 radial tables (mod_radialFD.cu:256-415).
+struct RadialTables {
+  std::vector<double> r, D, Lu, Lp, rdr, dr0, dr1;
+};
+void build_radial_tables(int nr, RadialTables& T);
+
+// ----------------------------------------------------------------- FFT plan
+// In-place decimation-in-time mixed-radix plan; input is placed at perm[n].
+struct FftPlan {
+  int L;
+  int nrad;
+  int rad[MAXRAD];
+};
+bool make_fft_plan(int L, FftPlan& p, std::vector<int>& perm);
+
+template <typename T> struct cpx { T x, y; };
+
+// Geometry and constants shared by all kernels of one pipe.
+struct Geom {
+  int nr, nt, nz, modes;   // NR, NT, NZ, NT*NZ
+  int ntp, nzf, nh;        // 3NT/2, 3(NZ-1), nzf/2
+  size_t n;                // NR*NT*NZ (field stride)
+  double kz0;              // 2*pi/LZ as the reference forms it (PI2/LZ)
+  double re, dt, d_im, qdes;
+};
+
+// Device tables of one pipe.
+struct DevTables {
+  double* r;     // [NR]
+  double* D;     // [2][NR][7]
+  double* Lu;    // [2][NR][7]
+  double* Lp;    // [2][NR][7]
+  double* rdr;   // [NR]
+  double* dr0;   // [4]
+  double* dr1;   // [7]
+  double* uc;    // [NR] flux profile (mod_cublasflux.cu:310-317)
+  double* qc;    // [1]  Qc = rdr . uc
+};
+
+// Influence-matrix data (mod_boundary.cu:200), +/- form.
+struct DevInfluence {
+  double2* uh;    // [3][N] homogeneous velocity solutions
+  double2* uhP;   // [3][N] gradient of the homogeneous pressure solution
+  double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
+};
+
+// Per-mode linear sweep (predictor / corrector) argument block.
+struct LinArgs {
+  Geom g;
+  DevTables t;
+  DevInfluence b;
+  const double2* N;     // [3] nonlinear term of the current iterate, r/theta form
+  double2* Nw;          // [3] nonlinear term of u^n, +/- form (rhsw)
+  const double2* u;     // [3] predictor: u^n (r/theta); corrector: u^k (norm)
+  double2* uw;          // [3] u^n in +/- form
+  double2* out;         // [3] new iterate (r/theta)
+  double2* rhs;         // [3] scratch rhs
+  double2* p;           // [1] scratch pressure / LU vector
+  double* U;            // scratch U factor [NR*4][MODES]
+  unsigned long long* norm;  // [2] max |u^k - u^{k+1}|^2, max |u^k|^2 (bits)
+  int predictor;
+};
+
+// Nonlinear-phase buffers and FFT data.
+struct NlBufs {
+  FftPlan pt, ph;      // theta (length NTP) and half-length z (length NZF/2) plans
+  int* perm_t;         // [NTP]
+  int* perm_h;         // [NH]
+  double2* tw_t;       // [NTP] exp(-2 pi i e/NTP)
+  double2* tw_h;       // [NH]  exp(-2 pi i e/NH)
+  double2* tw_z;       // [NH+1] exp(-2 pi i k/NZF)
+  double2* dR;         // [3][N] radial derivatives
+  double2* W;          // [9][NR*NTP*NZ] theta-inverse intermediates
+  double2* Y;          // [3][NR*NTP*NZ] z-forward intermediates
+  size_t wstride;      // NR*NTP*NZ
+};
+
+// Launchers (HIP sources).
+void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u,
+                      double2* N, hipStream_t s);
+void launch_linear_sweep(const LinArgs& a, hipStream_t s);
+void launch_norm_reset(unsigned long long* norm, hipStream_t s);
+void launch_setup_influence(const Geom& g, const DevTables& t, const DevInfluence& b,
+                            double2* scratch_p, double* scratch_U, hipStream_t s);
+void launch_diagnostics(const Geom& g, const DevTables& t, const double2* u, double* partial,
+                        hipStream_t s);
+
+}  // namespace nsp

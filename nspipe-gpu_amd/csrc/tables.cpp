@@ -1,0 +1,207 @@
+// Host setup: radial grid, finite-difference tables, quadrature weights and
+// FFT plans.  Runs once per pipe in FP64 on the host.
+#include <algorithm>
+#include <cmath>
+
+#include "nsp_internal.h"
+
+namespace nsp {
+
+// Fornberg weights of derivatives 0..m at rau[i0+IW] from the points
+// rau[i0 .. i0+right-left] (mod_radialFD.cu:422-452).  c[j + n*k] holds the
+// weight of point j for derivative k, n = number of points.
+static void fornberg(double* c, const double* rau, int i0, int left, int right, int m) {
+  const int n = 1 + right - left;
+  const double x0 = rau[i0 + IW];
+  double x[STEN];
+  for (int q = 0; q < STEN * STEN; q++) c[q] = 0.0;
+  for (int q = 0; q < n; q++) x[q] = rau[q + i0];
+  c[0] = 1.0;
+  double c1 = 1.0, c4 = x[0] - x0;
+  for (int i = 1; i < n; i++) {
+    const int mn = std::min(i, m);
+    double c2 = 1.0;
+    const double c5 = c4;
+    c4 = x[i] - x0;
+    for (int j = 0; j < i; j++) {
+      const double c3 = x[i] - x[j];
+      c2 = c2 * c3;
+      if (j == i - 1) {
+        for (int k = mn; k > 0; k--)
+          c[i + n * k] = c1 * (double(k) * c[i - 1 + n * (k - 1)] - c5 * c[i - 1 + n * k]) / c2;
+        c[i] = -c1 * c5 * c[i - 1] / c2;
+      }
+      for (int k = mn; k > 0; k--) c[j + n * k] = (c4 * c[j + n * k] - double(k) * c[j + (k - 1) * n]) / c3;
+      c[j] = c4 * c[j] / c3;
+    }
+    c1 = c2;
+  }
+}
+
+// Radial grid, parity-folded derivative tables, quadrature weights and the
+// Laplacians of the velocity (Dirichlet wall row) and pressure (Neumann wall
+// row): mod_radialFD.cu:256-369.  Table index: [parity][row][7], parity 0
+// for even functions of r (ghost points folded with +), 1 for odd (-).
+// The reference's quadrature loop runs one derivative order past the weights
+// it computed (mod_radialFD.cu:329 reads c[ir+len*len], beyond its 49-double
+// buffer); that term is of size dr^8/8! and is omitted here.
+void build_radial_tables(int nr, RadialTables& T) {
+  const int hD = STEN * nr;
+  T.r.assign(nr, 0.0);
+  T.D.assign(2 * hD, 0.0);
+  T.Lu.assign(2 * hD, 0.0);
+  T.Lp.assign(2 * hD, 0.0);
+  T.rdr.assign(nr, 0.0);
+  std::vector<double> D2(2 * hD, 0.0), rau(nr + IW), ri(nr + 1);
+  double c[STEN * STEN];
+  double* r = T.r.data();
+
+  // Cosine grid clustered to the wall, then stretched so the axis sits
+  // midway to its mirror point (mod_radialFD.cu:268-273).
+  const int np = nr + int(std::sqrt(double(nr)));
+  for (int i = np - nr; i < np; i++) {
+    const int ir = np - i - 1;
+    r[i - np + nr] = 0.5 * (1.0 + std::cos(0.5 * PI2 * (double(ir) / double(np))));
+  }
+  for (int it = 0; it < 10; it++) {
+    const double dr = 1.5 * r[0] - 0.5 * r[1];
+    for (int j = 0; j < nr; j++) r[j] = r[j] * (1.0 + dr) - dr;
+  }
+  ri[0] = 0.0;
+  for (int i = 0; i < nr; i++) ri[i + 1] = r[i];
+  for (int i = 0; i < nr + IW; i++) rau[i] = (i < IW) ? -r[IW - i - 1] : r[i - IW];
+
+  // First and second derivative rows (mod_radialFD.cu:278-309).
+  int ghosts = IW;
+  for (int i = 0; i < nr; i++) {
+    const int left = i, right = std::min(2 * IW + i, nr + IW - 1), len = 1 + right - left;
+    fornberg(c, rau.data(), i, left, right, 2);
+    if (i < IW) {
+      for (int j = 0; j < len; j++) {
+        if (j < ghosts) {  // ghost point -r_q folded onto r_q
+          const int col = ghosts - j - 1;
+          T.D[col + STEN * i] += c[len + j];
+          T.D[hD + col + STEN * i] -= c[len + j];
+          D2[col + STEN * i] += c[2 * len + j];
+          D2[hD + col + STEN * i] -= c[2 * len + j];
+        } else {
+          const int col = j - ghosts;
+          T.D[col + STEN * i] += c[len + j];
+          T.D[hD + col + STEN * i] += c[len + j];
+          D2[col + STEN * i] += c[2 * len + j];
+          D2[hD + col + STEN * i] += c[2 * len + j];
+        }
+      }
+      ghosts -= 1;
+    } else {
+      const int off = STEN - len;  // short near-wall rows are right-aligned
+      for (int j = 0; j < len; j++) {
+        const int col = off + j;
+        T.D[col + STEN * i] += c[len + j];
+        T.D[hD + col + STEN * i] += c[len + j];
+        D2[col + STEN * i] += c[2 * len + j];
+        D2[hD + col + STEN * i] += c[2 * len + j];
+      }
+    }
+  }
+
+  // Quadrature weights: rdr_j such that sum_j rdr_j f_j ~ 2 int_0^1 f r dr,
+  // from Taylor expansions about each r_i over [r_{i-1}, r_{i+1}] with
+  // r_{-1} = 0 and r_{NR} = r_{NR-1} (mod_radialFD.cu:312-337).
+  ghosts = IW;
+  for (int i = 0; i < nr; i++) {
+    const int left = i, right = std::min(2 * IW + i, nr + IW - 1), len = 1 + right - left;
+    const int left2 = std::max(0, i - IW);
+    fornberg(c, rau.data(), i, left, right, len - 1);
+    int off = 0;
+    double e1 = 1.0, e2 = 1.0;
+    if (i < IW) {
+      off = ghosts;
+      for (int j = 0; j < ghosts; j++) {
+        const int col = 2 * ghosts - 1 - j;
+        for (int k = 0; k < len; k++) c[col + k * len] += c[j + k * len];
+      }
+      ghosts -= 1;
+    }
+    for (int k = 0; k < len; k++) {
+      e1 = e1 * (ri[std::min(i + 2, nr)] - ri[i + 1]) / (double(k + 1));
+      e2 = e2 * (ri[i] - ri[i + 1]) / (double(k + 1));
+      for (int j = 0; j < len - off; j++) T.rdr[left2 + j] += (e1 - e2) * r[i] * c[off + j + k * len];
+    }
+  }
+
+  // Radial Laplacians (mod_radialFD.cu:339-359).
+  for (int i = 0; i < nr; i++) {
+    for (int j = 0; j < STEN; j++) {
+      const int q = j + STEN * i;
+      if (i < nr - 1) {
+        T.Lu[q] = D2[q] + T.D[q] / r[i];
+        T.Lu[hD + q] = D2[hD + q] + T.D[hD + q] / r[i];
+        T.Lp[q] = D2[q] + T.D[q] / r[i];
+        T.Lp[hD + q] = D2[hD + q] + T.D[hD + q] / r[i];
+      } else {
+        T.Lu[q] = (j < STEN - 1) ? 0.0 : 1.0;
+        T.Lu[hD + q] = T.Lu[q];
+        T.Lp[q] = T.D[q];
+        T.Lp[hD + q] = T.D[hD + q];
+      }
+    }
+  }
+
+  // Interpolation weights to r = 0 from r[0..IW] (get_dr0, mod_radialFD.cu:373-407):
+  // Lagrange weights l_j(0) = prod_{q != j} (0 - r_q)/(r_j - r_q), in the
+  // reference's Fornberg recurrence order.
+  {
+    const int n = IW + 1;
+    double cc[IW + 1], x[IW + 1];
+    for (int q = 0; q < n; q++) { cc[q] = 0.0; x[q] = r[q]; }
+    cc[0] = 1.0;
+    double c1 = 1.0, c4 = x[0];
+    for (int i = 1; i < n; i++) {
+      double c2 = 1.0;
+      const double c5 = c4;
+      c4 = x[i];
+      for (int j = 0; j < i; j++) {
+        const double c3 = x[i] - x[j];
+        c2 = c2 * c3;
+        if (j == i - 1) cc[i] = -c1 * c5 * cc[i - 1] / c2;
+        cc[j] = c4 * cc[j] / c3;
+      }
+      c1 = c2;
+    }
+    T.dr0.assign(cc, cc + n);
+  }
+  // Wall row of the even derivative table (get_dr1, mod_radialFD.cu:412-415).
+  T.dr1.assign(T.D.begin() + (nr - 1) * STEN, T.D.begin() + nr * STEN);
+}
+
+// In-place DIT plan: radices 4, 2, 3, 5 in that order; perm[n] is the LDS
+// position of input sample n (mixed-radix digit reversal: the last pass's
+// radix digit of n selects the outermost block).
+bool make_fft_plan(int L, FftPlan& p, std::vector<int>& perm) {
+  p.L = L;
+  p.nrad = 0;
+  int m = L;
+  const int order[4] = {4, 2, 3, 5};
+  for (int o = 0; o < 4; o++)
+    while (m % order[o] == 0) {
+      if (p.nrad >= MAXRAD) return false;
+      p.rad[p.nrad++] = order[o];
+      m /= order[o];
+    }
+  if (m != 1) return false;
+  perm.assign(L, 0);
+  for (int n = 0; n < L; n++) {
+    int pos = 0, rem = n, len = L;
+    for (int q = p.nrad - 1; q >= 0; q--) {
+      const int rq = p.rad[q];
+      len /= rq;
+      pos += (rem % rq) * len;
+      rem /= rq;
+    }
+    perm[n] = pos;
+  }
+  return true;
+}
+
+}  // namespace nsp

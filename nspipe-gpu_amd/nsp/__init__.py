@@ -1,0 +1,200 @@
+"""Python binding of libnsp.so, the MI355X nsPipe time step (C ABI in include/nsp.h).
+
+`Pipe` mirrors the reference driver (Main_Code/src/main.cu): construction is
+the setter sequence (init_fd ... setBCon), `step` is the body of integrate()
+(mod_integrate.cu:112-131), `nonlinear` is nonlinear() (mod_io_nonlinear.cu:380),
+`read_state_files`/`write_state_files` are rBufferBinary/wBufferBinary.
+Fields are numpy complex128 arrays of shape (NR, NT, NZ) in the reference
+layout, or device pointers (int) with where=DEVICE.
+
+There is no fallback: if libnsp.so is missing or no HIP device is present the
+constructor raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(HERE), "libnsp.so")
+
+FP64, MIXED = 0, 1
+HOST, DEVICE = 0, 1
+FLAG_NAN, FLAG_MAXIT = 1, 2
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("nr", ctypes.c_int), ("nt", ctypes.c_int), ("nz", ctypes.c_int),
+                ("lz", ctypes.c_double), ("re", ctypes.c_double), ("dt", ctypes.c_double),
+                ("d_im", ctypes.c_double), ("maxit", ctypes.c_int), ("tol", ctypes.c_double),
+                ("qdes", ctypes.c_double), ("precision", ctypes.c_int)]
+
+
+class StepInfo(ctypes.Structure):
+    _fields_ = [("n_it", ctypes.c_int), ("err", ctypes.c_double), ("flags", ctypes.c_int)]
+
+
+_lib = None
+EXPORTS = ["nsp_default_params", "nsp_create", "nsp_destroy", "nsp_set_state", "nsp_get_state",
+           "nsp_reference_ic", "nsp_step", "nsp_nonlinear", "nsp_diagnostics", "nsp_tables",
+           "nsp_radial_tables", "nsp_read_state_files", "nsp_write_state_files", "nsp_seeded_ic",
+           "nsp_set_timing", "nsp_last_timing", "nsp_stream", "nsp_last_error"]
+
+
+class NspError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load libnsp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise NspError(f"libnsp.so not built at {LIB_PATH}; run `make -C nspipe-gpu_amd`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+    P = ctypes.POINTER(ctypes.c_double)
+    L.nsp_last_error.restype = ctypes.c_char_p
+    L.nsp_default_params.argtypes = [ctypes.POINTER(Params), ip, ip, ip, dp, dp, dp]
+    L.nsp_create.argtypes = [ctypes.POINTER(Params), ip, ctypes.POINTER(vp)]
+    L.nsp_destroy.argtypes = [vp]
+    L.nsp_set_state.argtypes = [vp, vp, vp, vp, ip]
+    L.nsp_get_state.argtypes = [vp, vp, vp, vp, ip]
+    L.nsp_reference_ic.argtypes = [vp, dp]
+    L.nsp_step.argtypes = [vp, ip, ctypes.POINTER(StepInfo)]
+    L.nsp_nonlinear.argtypes = [vp, vp, vp, vp, vp, vp, vp, ip]
+    L.nsp_diagnostics.argtypes = [vp, P]
+    L.nsp_tables.argtypes = [vp, P, P, P, P, P, P, P]
+    L.nsp_radial_tables.argtypes = [ip, P, P, P, P, P, P, P]
+    L.nsp_read_state_files.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
+    L.nsp_write_state_files.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
+    L.nsp_seeded_ic.argtypes = [vp, ctypes.c_ulonglong, ip, ip, ip, dp]
+    L.nsp_set_timing.argtypes = [vp, ip]
+    L.nsp_last_timing.argtypes = [vp, P]
+    L.nsp_stream.restype = vp
+    L.nsp_stream.argtypes = [vp]
+    for name in EXPORTS:
+        getattr(L, name).restype = getattr(L, name).restype if name in (
+            "nsp_last_error", "nsp_stream") else ctypes.c_int
+    _lib = L
+    return L
+
+
+def _check(st):
+    if st != 0:
+        raise NspError(f"nsp status {st}: {lib().nsp_last_error().decode()}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def radial_tables(nr):
+    """Host-only radial tables (init_fd, get_dr0, get_dr1) for NR = nr."""
+    r = np.zeros(nr); D = np.zeros(14 * nr); Lu = np.zeros(14 * nr); Lp = np.zeros(14 * nr)
+    rdr = np.zeros(nr); dr0 = np.zeros(4); dr1 = np.zeros(7)
+    _check(lib().nsp_radial_tables(nr, _dp(r), _dp(D), _dp(Lu), _dp(Lp), _dp(rdr), _dp(dr0), _dp(dr1)))
+    return dict(r=r, D=D.reshape(2, nr, 7), Lu=Lu.reshape(2, nr, 7), Lp=Lp.reshape(2, nr, 7),
+                rdr=rdr, dr0=dr0, dr1=dr1)
+
+
+class Pipe:
+    """One pipe on one HIP device."""
+
+    def __init__(self, nr, nt, nz, lz, re, dt, precision=FP64, device=0, d_im=0.51, maxit=10,
+                 tol=5e-8, qdes=0.5):
+        p = Params()
+        _check(lib().nsp_default_params(ctypes.byref(p), nr, nt, nz, lz, re, dt))
+        p.d_im, p.maxit, p.tol, p.qdes, p.precision = d_im, maxit, tol, qdes, precision
+        self.params = p
+        self.nr, self.nt, self.nz = nr, nt, nz
+        h = ctypes.c_void_p()
+        _check(lib().nsp_create(ctypes.byref(p), device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().nsp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    @property
+    def shape(self):
+        return (self.nr, self.nt, self.nz)
+
+    def _host(self, arrs):
+        out = []
+        for a in arrs:
+            a = np.ascontiguousarray(a, np.complex128)
+            if a.shape != self.shape:
+                raise ValueError(f"field shape {a.shape} != {self.shape}")
+            out.append(a)
+        return out
+
+    def set_state(self, ur, ut, uz):
+        a = self._host((ur, ut, uz))
+        _check(lib().nsp_set_state(self.h, *[x.ctypes.data for x in a], HOST))
+
+    def get_state(self):
+        a = [np.zeros(self.shape, np.complex128) for _ in range(3)]
+        _check(lib().nsp_get_state(self.h, *[x.ctypes.data for x in a], HOST))
+        return a
+
+    def set_state_device(self, pur, put, puz):
+        _check(lib().nsp_set_state(self.h, pur, put, puz, DEVICE))
+
+    def get_state_device(self, pur, put, puz):
+        _check(lib().nsp_get_state(self.h, pur, put, puz, DEVICE))
+
+    def reference_ic(self, a_p=0.3):
+        _check(lib().nsp_reference_ic(self.h, a_p))
+
+    def seeded_ic(self, seed, mmax=4, lmax=4, kind=0, rms=1e-3):
+        _check(lib().nsp_seeded_ic(self.h, seed, mmax, lmax, kind, rms))
+
+    def step(self, nsteps=1):
+        info = (StepInfo * max(nsteps, 1))()
+        _check(lib().nsp_step(self.h, nsteps, info))
+        return (np.array([info[s].n_it for s in range(nsteps)], np.int32),
+                np.array([info[s].err for s in range(nsteps)]),
+                np.array([info[s].flags for s in range(nsteps)], np.int32))
+
+    def nonlinear(self, ur, ut, uz):
+        a = self._host((ur, ut, uz))
+        b = [np.zeros(self.shape, np.complex128) for _ in range(3)]
+        _check(lib().nsp_nonlinear(self.h, *[x.ctypes.data for x in a], *[x.ctypes.data for x in b], HOST))
+        return b
+
+    def diagnostics(self):
+        out = np.zeros(7)
+        _check(lib().nsp_diagnostics(self.h, _dp(out)))
+        return dict(u_cl=out[0], u_tau=out[1], ke=out[2], div_l2=out[3], wall_max=out[4],
+                    wall_div_max=out[5], flux=out[6])
+
+    def tables(self):
+        nr = self.nr
+        r = np.zeros(nr); D = np.zeros(14 * nr); Lu = np.zeros(14 * nr); Lp = np.zeros(14 * nr)
+        rdr = np.zeros(nr); dr0 = np.zeros(4); dr1 = np.zeros(7)
+        _check(lib().nsp_tables(self.h, _dp(r), _dp(D), _dp(Lu), _dp(Lp), _dp(rdr), _dp(dr0), _dp(dr1)))
+        return dict(r=r, D=D.reshape(2, nr, 7), Lu=Lu.reshape(2, nr, 7), Lp=Lp.reshape(2, nr, 7),
+                    rdr=rdr, dr0=dr0, dr1=dr1)
+
+    def read_state_files(self, ur, ut, uz):
+        _check(lib().nsp_read_state_files(self.h, ur.encode(), ut.encode(), uz.encode()))
+
+    def write_state_files(self, ur, ut, uz):
+        _check(lib().nsp_write_state_files(self.h, ur.encode(), ut.encode(), uz.encode()))
+
+    def set_timing(self, on=True):
+        _check(lib().nsp_set_timing(self.h, 1 if on else 0))
+
+    def last_timing(self):
+        out = np.zeros(3)
+        _check(lib().nsp_last_timing(self.h, _dp(out)))
+        return dict(nonlinear_ms=out[0], linear_ms=out[1], step_ms=out[2])
+
+    def stream(self):
+        return lib().nsp_stream(self.h)

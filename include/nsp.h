@@ -137,15 +137,23 @@ nsp_status nsp_write_state_files(nsp_handle* h, const char* ur_path, const char*
 nsp_status nsp_seeded_ic(nsp_handle* h, unsigned long long seed, int mmax, int lmax, int kind,
                          double rms);
 
-/* Event-timed last step breakdown in milliseconds: out[0] nonlinear (all
- * evaluations), out[1] linear sweeps, out[2] whole step.  Requires that
- * timing was enabled with nsp_set_timing(h, 1) before the step. */
-nsp_status nsp_set_timing(nsp_handle* h, int enable);
-nsp_status nsp_last_timing(nsp_handle* h, double out[3]);
+/* Per-kernel timing with HIP events recorded on the handle's stream around
+ * every launch of nsp_step (no reference counterpart: the reference only
+ * reports total process CPU time, main.cu:16,54-55).  nsp_profile_enable
+ * turns it on/off and clears the totals; nsp_profile_read fills
+ * nsp_profile_count() entries of accumulated milliseconds and launch counts,
+ * indexed like nsp_profile_name(i). */
+nsp_status nsp_profile_enable(nsp_handle* h, int enable);
+int nsp_profile_count(void);
+const char* nsp_profile_name(int i);
+nsp_status nsp_profile_read(nsp_handle* h, double* total_ms, long long* launches);
 
 /* HIP stream of the handle (a hipStream_t), for callers that order their own
  * work against it. */
 void* nsp_stream(nsp_handle* h);
+
+/* Waits for all work of the handle's device (hipDeviceSynchronize). */
+nsp_status nsp_synchronize(nsp_handle* h);
 
 /* Human-readable text of the last error on this thread. */
 const char* nsp_last_error(void);

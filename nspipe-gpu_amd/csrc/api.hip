@@ -47,9 +47,7 @@ struct nsp_handle {
   unsigned long long* norm_h = nullptr;  // pinned host [2]
   double* part = nullptr;   // diagnostics partials [MODES][4]
   std::vector<void*> allocs;
-  bool timing = false;
-  double last_ms[3] = {0, 0, 0};
-  hipEvent_t ev[8] = {};
+  Profiler prof;
 };
 
 namespace {
@@ -69,8 +67,6 @@ void free_all(nsp_handle* h) {
   h->allocs.clear();
   if (h->norm_h) (void)hipHostFree(h->norm_h);
   h->norm_h = nullptr;
-  for (auto& e : h->ev)
-    if (e) (void)hipEventDestroy(e);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   h->stream = nullptr;
 }
@@ -83,6 +79,7 @@ nsp_status check_params(const nsp_params* p) {
   if (!(p->lz > 0) || !(p->re > 0) || !(p->dt > 0)) return fail(NSP_ERR_INVALID, "lz, re, dt must be > 0");
   if (p->maxit < 1) return fail(NSP_ERR_INVALID, "maxit must be >= 1");
   if (p->precision != NSP_FP64 && p->precision != NSP_MIXED) return fail(NSP_ERR_INVALID, "bad precision");
+  if (p->precision == NSP_MIXED) return fail(NSP_ERR_INVALID, "the mixed-precision form is not built yet");
   const int nzf = 3 * (p->nz - 1);
   if (nzf > 36 * 256) return fail(NSP_ERR_INVALID, "3(nz-1) must be <= 9216");
   return NSP_OK;
@@ -220,7 +217,6 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   } while (0)
 
   HTRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-  for (auto& e : h->ev) HTRY(hipEventCreate(&e));
 
   // radial tables
   build_radial_tables(g.nr, h->host);
@@ -300,26 +296,41 @@ nsp_status nsp_get_state(nsp_handle* h, double* ur, double* ut, double* uz, int 
 
 void* nsp_stream(nsp_handle* h) { return h ? (void*)h->stream : nullptr; }
 
+nsp_status nsp_synchronize(nsp_handle* h) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipDeviceSynchronize());
+  return NSP_OK;
+}
+
 nsp_status nsp_nonlinear(nsp_handle* h, const double* ur, const double* ut, const double* uz, double* dr,
                          double* dt, double* dz, int where) {
   if (!h) return fail(NSP_ERR_INVALID, "null handle");
   HIPCHK(hipSetDevice(h->device));
   nsp_status st = copy_fields(h, h->ui, ur, ut, uz, where);
   if (st) return st;
-  launch_nonlinear(h->g, h->t, h->nb, h->ui, h->Nl, h->stream);
+  launch_nonlinear(h->g, h->t, h->nb, h->ui, h->Nl, h->stream, nullptr);
   HIPCHK(hipGetLastError());
   return fetch_fields(h, h->Nl, dr, dt, dz, where);
 }
 
-nsp_status nsp_set_timing(nsp_handle* h, int enable) {
+nsp_status nsp_profile_enable(nsp_handle* h, int enable) {
   if (!h) return fail(NSP_ERR_INVALID, "null handle");
-  h->timing = enable != 0;
+  h->prof.on = enable != 0;
+  h->prof.reset();
   return NSP_OK;
 }
 
-nsp_status nsp_last_timing(nsp_handle* h, double out[3]) {
-  if (!h || !out) return fail(NSP_ERR_INVALID, "null argument");
-  for (int q = 0; q < 3; q++) out[q] = h->last_ms[q];
+int nsp_profile_count(void) { return K_COUNT; }
+
+const char* nsp_profile_name(int i) { return kernel_name(i); }
+
+nsp_status nsp_profile_read(nsp_handle* h, double* total_ms, long long* launches) {
+  if (!h || !total_ms || !launches) return fail(NSP_ERR_INVALID, "null argument");
+  for (int q = 0; q < K_COUNT; q++) {
+    total_ms[q] = h->prof.total_ms[q];
+    launches[q] = h->prof.launches[q];
+  }
   return NSP_OK;
 }
 
@@ -330,50 +341,31 @@ nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {
   HIPCHK(hipSetDevice(h->device));
   hipStream_t s = h->stream;
   const Geom& g = h->g;
+  Profiler* pf = h->prof.on ? &h->prof : nullptr;
   for (int step = 0; step < nsteps; step++) {
-    float t_nl = 0.f, t_lin = 0.f, t_all = 0.f, tmp = 0.f;
-    if (h->timing) HIPCHK(hipEventRecord(h->ev[0], s));
     // nonlinear(u,rhs); predictor(u,rhs,uw,rhsw,ui)
-    if (h->timing) HIPCHK(hipEventRecord(h->ev[2], s));
-    launch_nonlinear(g, h->t, h->nb, h->u, h->Nl, s);
-    if (h->timing) HIPCHK(hipEventRecord(h->ev[3], s));
+    launch_nonlinear(g, h->t, h->nb, h->u, h->Nl, s, pf);
+    if (pf) pf->begin(K_LIN_PRED, s);
     launch_linear_sweep(lin_args(h, h->Nl, h->u, h->ui, true), s);
-    if (h->timing) HIPCHK(hipEventRecord(h->ev[4], s));
+    if (pf) pf->end(s);
     HIPCHK(hipGetLastError());
-    if (h->timing) {
-      HIPCHK(hipEventSynchronize(h->ev[4]));
-      HIPCHK(hipEventElapsedTime(&tmp, h->ev[2], h->ev[3])); t_nl += tmp;
-      HIPCHK(hipEventElapsedTime(&tmp, h->ev[3], h->ev[4])); t_lin += tmp;
-    }
     double err = 1e98;
     int ite = 0;
     while (ite < h->P.maxit && err > h->P.tol) {
       ite += 1;
-      if (h->timing) HIPCHK(hipEventRecord(h->ev[2], s));
-      launch_nonlinear(g, h->t, h->nb, h->ui, h->Nl, s);
-      if (h->timing) HIPCHK(hipEventRecord(h->ev[3], s));
+      launch_nonlinear(g, h->t, h->nb, h->ui, h->Nl, s, pf);
       launch_norm_reset(h->norm, s);
+      if (pf) pf->begin(K_LIN_CORR, s);
       launch_linear_sweep(lin_args(h, h->Nl, h->ui, h->un, false), s);
-      if (h->timing) HIPCHK(hipEventRecord(h->ev[4], s));
+      if (pf) pf->end(s);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(h->norm_h, h->norm, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
       HIPCHK(hipStreamSynchronize(s));
-      if (h->timing) {
-        HIPCHK(hipEventElapsedTime(&tmp, h->ev[2], h->ev[3])); t_nl += tmp;
-        HIPCHK(hipEventElapsedTime(&tmp, h->ev[3], h->ev[4])); t_lin += tmp;
-      }
       err = bits_to_double(h->norm_h[0]) / bits_to_double(h->norm_h[1]);
       std::swap(h->ui, h->un);
     }
     std::swap(h->u, h->ui);  // the last iterate becomes the state
-    if (h->timing) {
-      HIPCHK(hipEventRecord(h->ev[1], s));
-      HIPCHK(hipEventSynchronize(h->ev[1]));
-      HIPCHK(hipEventElapsedTime(&t_all, h->ev[0], h->ev[1]));
-      h->last_ms[0] = t_nl;
-      h->last_ms[1] = t_lin;
-      h->last_ms[2] = t_all;
-    }
+    if (pf) pf->collect();
     if (info) {
       info[step].n_it = ite;
       info[step].err = err;

@@ -297,11 +297,22 @@ static void run_z(const Geom& g, const DevTables& t, const NlBufs& nb, hipStream
                                                     t.r, nb.W, nb.wstride, nb.Y);
 }
 
+const char* kernel_name(int id) {
+  static const char* names[K_COUNT] = {"dr3_kernel", "theta_inv_kernel", "zpass_kernel", "theta_fwd_kernel",
+                                       "linear_sweep_kernel(predictor)", "linear_sweep_kernel(corrector)"};
+  return (id >= 0 && id < K_COUNT) ? names[id] : "";
+}
+
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u, double2* N,
-                      hipStream_t s) {
+                      hipStream_t s, Profiler* pf) {
   dim3 gd((g.modes + 255) / 256, g.nr);
+  if (pf) pf->begin(K_DR3, s);
   dr3_kernel<<<gd, 256, 0, s>>>(g, t.D, u, nb.dR);
+  if (pf) pf->end(s);
+  if (pf) pf->begin(K_THETA_INV, s);
   theta_dispatch(g, t, nb, u, N, s, true);
+  if (pf) pf->end(s);
+  if (pf) pf->begin(K_ZPASS, s);
   const int mpt = (g.nzf + 255) / 256;
   if (mpt <= 1) run_z<1>(g, t, nb, s);
   else if (mpt <= 2) run_z<2>(g, t, nb, s);
@@ -312,7 +323,10 @@ void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const
   else if (mpt <= 18) run_z<18>(g, t, nb, s);
   else if (mpt <= 24) run_z<24>(g, t, nb, s);
   else run_z<36>(g, t, nb, s);
+  if (pf) pf->end(s);
+  if (pf) pf->begin(K_THETA_FWD, s);
   theta_dispatch(g, t, nb, u, N, s, false);
+  if (pf) pf->end(s);
 }
 
 }  // namespace nsp

@@ -95,9 +95,58 @@ struct NlBufs {
   size_t wstride;      // NR*NTP*NZ
 };
 
+// ----------------------------------------------------------------- profiler
+// Per-kernel HIP-event timing on the launching stream (enabled on demand).
+enum KernelId { K_DR3 = 0, K_THETA_INV, K_ZPASS, K_THETA_FWD, K_LIN_PRED, K_LIN_CORR, K_COUNT };
+const char* kernel_name(int id);
+
+struct Profiler {
+  bool on = false;
+  std::vector<hipEvent_t> pool;
+  std::vector<int> ids;
+  int used = 0;
+  double total_ms[K_COUNT] = {};
+  long long launches[K_COUNT] = {};
+  void reset() {
+    for (int q = 0; q < K_COUNT; q++) { total_ms[q] = 0.0; launches[q] = 0; }
+    used = 0;
+    ids.clear();
+  }
+  void begin(int id, hipStream_t s) {
+    if (!on) return;
+    while ((int)pool.size() < used + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+      pool.push_back(e);
+    }
+    ids.push_back(id);
+    (void)hipEventRecord(pool[used], s);
+  }
+  void end(hipStream_t s) {
+    if (!on) return;
+    (void)hipEventRecord(pool[used + 1], s);
+    used += 2;
+  }
+  // call after the stream has been synchronised
+  void collect() {
+    for (int q = 0; q < used / 2; q++) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, pool[2 * q], pool[2 * q + 1]) == hipSuccess) {
+        total_ms[ids[q]] += ms;
+        launches[ids[q]] += 1;
+      }
+    }
+    used = 0;
+    ids.clear();
+  }
+  ~Profiler() {
+    for (auto e : pool) (void)hipEventDestroy(e);
+  }
+};
+
 // Launchers (HIP sources).
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u,
-                      double2* N, hipStream_t s);
+                      double2* N, hipStream_t s, Profiler* pf);
 void launch_linear_sweep(const LinArgs& a, hipStream_t s);
 void launch_norm_reset(unsigned long long* norm, hipStream_t s);
 void launch_setup_influence(const Geom& g, const DevTables& t, const DevInfluence& b,

@@ -38,7 +38,8 @@ _lib = None
 EXPORTS = ["nsp_default_params", "nsp_create", "nsp_destroy", "nsp_set_state", "nsp_get_state",
            "nsp_reference_ic", "nsp_step", "nsp_nonlinear", "nsp_diagnostics", "nsp_tables",
            "nsp_radial_tables", "nsp_read_state_files", "nsp_write_state_files", "nsp_seeded_ic",
-           "nsp_set_timing", "nsp_last_timing", "nsp_stream", "nsp_last_error"]
+           "nsp_profile_enable", "nsp_profile_count", "nsp_profile_name", "nsp_profile_read",
+           "nsp_stream", "nsp_synchronize", "nsp_last_error"]
 
 
 class NspError(RuntimeError):
@@ -70,13 +71,17 @@ def lib():
     L.nsp_read_state_files.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
     L.nsp_write_state_files.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]
     L.nsp_seeded_ic.argtypes = [vp, ctypes.c_ulonglong, ip, ip, ip, dp]
-    L.nsp_set_timing.argtypes = [vp, ip]
-    L.nsp_last_timing.argtypes = [vp, P]
+    L.nsp_profile_enable.argtypes = [vp, ip]
+    L.nsp_profile_count.argtypes = []
+    L.nsp_profile_name.restype = ctypes.c_char_p
+    L.nsp_profile_name.argtypes = [ip]
+    L.nsp_profile_read.argtypes = [vp, P, ctypes.POINTER(ctypes.c_longlong)]
     L.nsp_stream.restype = vp
     L.nsp_stream.argtypes = [vp]
+    L.nsp_synchronize.argtypes = [vp]
     for name in EXPORTS:
-        getattr(L, name).restype = getattr(L, name).restype if name in (
-            "nsp_last_error", "nsp_stream") else ctypes.c_int
+        if name not in ("nsp_last_error", "nsp_stream", "nsp_profile_name"):
+            getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
 
@@ -188,13 +193,21 @@ class Pipe:
     def write_state_files(self, ur, ut, uz):
         _check(lib().nsp_write_state_files(self.h, ur.encode(), ut.encode(), uz.encode()))
 
-    def set_timing(self, on=True):
-        _check(lib().nsp_set_timing(self.h, 1 if on else 0))
+    def profile_enable(self, on=True):
+        """Per-kernel HIP-event timing on the handle's stream; clears totals."""
+        _check(lib().nsp_profile_enable(self.h, 1 if on else 0))
 
-    def last_timing(self):
-        out = np.zeros(3)
-        _check(lib().nsp_last_timing(self.h, _dp(out)))
-        return dict(nonlinear_ms=out[0], linear_ms=out[1], step_ms=out[2])
+    def profile_read(self):
+        """{kernel name: (total ms, launches)} accumulated since profile_enable."""
+        n = lib().nsp_profile_count()
+        ms = np.zeros(n)
+        cnt = (ctypes.c_longlong * n)()
+        _check(lib().nsp_profile_read(self.h, _dp(ms), cnt))
+        return {lib().nsp_profile_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+    def synchronize(self):
+        """hipDeviceSynchronize on the pipe's device."""
+        _check(lib().nsp_synchronize(self.h))
 
     def stream(self):
         return lib().nsp_stream(self.h)

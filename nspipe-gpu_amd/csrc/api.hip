@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <random>
 
 #include "nsp_internal.h"
@@ -240,6 +241,12 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   TRY(upload(h, &h->nb.tw_h, twiddles(g.nh, g.nh)));
   TRY(upload(h, &h->nb.tw_z, twiddles(g.nzf, g.nh + 1)));
   h->nb.wstride = size_t(g.nr) * g.ntp * g.nz;
+  h->nb.use_ct = false;
+  if (ct_has_theta(g.ntp) && ct_has_zhalf(g.nh) && !getenv("NSP_GENERIC_FFT")) {
+    TRY(upload(h, &h->nb.ct_tw_t, pass_twiddles(ct_radices(g.ntp, true))));
+    TRY(upload(h, &h->nb.ct_tw_h, pass_twiddles(ct_radices(g.nh, false))));
+    h->nb.use_ct = true;
+  }
 
   // fields
   const size_t n = g.n;

@@ -1,0 +1,240 @@
+// Compile-time specialised FFT building blocks for LDS-resident transforms.
+//
+// Plan<R0, R1, ..., R_{P-1}>: length L = prod(R), in-place decimation in
+// time.  Sample n is placed at LDS position Plan::digitrev(n); pass p (radix
+// R_p, sub-length S_p = R_0...R_{p-1}) combines R_p sub-transforms of length
+// S_p into one of length S_p*R_p; the result comes out in natural order.
+// All radices, strides and divisors are compile-time constants, so index
+// arithmetic reduces to shifts and multiply-high sequences and the butterfly
+// loops unroll.  Butterflies of radix 6, 8, 9, 12, 16 are composite
+// in-register DFTs (R = R1*R2) with constant twiddles evaluated by the
+// compiler.
+//
+// Twiddles between passes come from a compact table (host-built, see
+// pass_twiddles) holding, for each pass in order, S*(R-1) entries
+// tw[k*(R-1) + q-1] = exp(-2 pi i q k / (S R)), k < S, 1 <= q < R.
+#pragma once
+
+#include "nsp_internal.h"
+
+namespace nsp {
+
+// ---------------------------------------------------------------- constexpr trig
+constexpr long double kPiL = 3.141592653589793238462643383279502884L;
+
+constexpr long double ct_sin_small(long double x) {
+  long double term = x, sum = x;
+  for (int n = 1; n < 40; ++n) {
+    term *= -x * x / ((2.0L * n) * (2.0L * n + 1.0L));
+    sum += term;
+  }
+  return sum;
+}
+// cos and sin of 2 pi m / R, exact to long double rounding, reduced to [-pi/4, pi/4]
+constexpr long double ct_cos2pi(int m, int R) {
+  m %= R;
+  if (m < 0) m += R;
+  // angle = 2 pi m / R; use octant reduction through sin/cos identities
+  const long double a = 2.0L * kPiL * (long double)m / (long double)R;
+  // cos(a) = sin(pi/2 - a); reduce into [-pi, pi]
+  long double x = kPiL / 2.0L - a;
+  while (x > kPiL) x -= 2.0L * kPiL;
+  while (x < -kPiL) x += 2.0L * kPiL;
+  if (x > kPiL / 2.0L) x = kPiL - x;
+  if (x < -kPiL / 2.0L) x = -kPiL - x;
+  return ct_sin_small(x);
+}
+constexpr long double ct_sin2pi(int m, int R) { return ct_cos2pi(m * 4 - R, 4 * R); }
+
+// cos/sin(2 pi m / R) for m < R, evaluated by the compiler
+template <int R>
+struct TwTab {
+  double c[R], s[R];
+  constexpr TwTab() : c(), s() {
+    for (int m = 0; m < R; ++m) {
+      c[m] = double(ct_cos2pi(m, R));
+      s[m] = double(ct_sin2pi(m, R));
+    }
+  }
+};
+template <int R>
+inline constexpr TwTab<R> kTw{};
+
+// ---------------------------------------------------------------- complex helpers
+template <typename T>
+__device__ __forceinline__ cpx<T> cx_mul(cpx<T> a, cpx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> cx_mulc(cpx<T> a, cpx<T> b) {  // a * conj(b)
+  return {a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y};
+}
+
+// ---------------------------------------------------------------- small DFTs
+// In-place DFT of v[0..R) in natural order, sign SIGN (-1 forward, +1 inverse).
+template <typename T, int R, int SIGN>
+struct DFT;
+
+template <typename T, int SIGN>
+struct DFT<T, 2, SIGN> {
+  __device__ __forceinline__ static void run(cpx<T>* v) {
+    const cpx<T> a = v[0], b = v[1];
+    v[0] = {a.x + b.x, a.y + b.y};
+    v[1] = {a.x - b.x, a.y - b.y};
+  }
+};
+template <typename T, int SIGN>
+struct DFT<T, 3, SIGN> {
+  __device__ __forceinline__ static void run(cpx<T>* v) {
+    constexpr T h = T(-0.5);
+    constexpr T sn = T(SIGN) * T(0.86602540378443864676372317075294L);
+    const cpx<T> a = v[0];
+    const cpx<T> t1{v[1].x + v[2].x, v[1].y + v[2].y}, t2{v[1].x - v[2].x, v[1].y - v[2].y};
+    const cpx<T> m{a.x + h * t1.x, a.y + h * t1.y};
+    v[0] = {a.x + t1.x, a.y + t1.y};
+    v[1] = {m.x - sn * t2.y, m.y + sn * t2.x};
+    v[2] = {m.x + sn * t2.y, m.y - sn * t2.x};
+  }
+};
+template <typename T, int SIGN>
+struct DFT<T, 4, SIGN> {
+  __device__ __forceinline__ static void run(cpx<T>* v) {
+    constexpr T s = T(SIGN);
+    const cpx<T> a = v[0], b = v[1], c = v[2], d = v[3];
+    const cpx<T> ac{a.x + c.x, a.y + c.y}, amc{a.x - c.x, a.y - c.y};
+    const cpx<T> bd{b.x + d.x, b.y + d.y};
+    const cpx<T> bmd{-s * (b.y - d.y), s * (b.x - d.x)};  // s*i*(b-d)
+    v[0] = {ac.x + bd.x, ac.y + bd.y};
+    v[1] = {amc.x + bmd.x, amc.y + bmd.y};
+    v[2] = {ac.x - bd.x, ac.y - bd.y};
+    v[3] = {amc.x - bmd.x, amc.y - bmd.y};
+  }
+};
+template <typename T, int SIGN>
+struct DFT<T, 5, SIGN> {
+  __device__ __forceinline__ static void run(cpx<T>* v) {
+    constexpr T c1 = T(0.30901699437494742410229341718282L), c2 = T(-0.80901699437494742410229341718282L);
+    constexpr T s1 = T(SIGN) * T(0.95105651629515357211643933337938L);
+    constexpr T s2 = T(SIGN) * T(0.58778525229247312916870595463907L);
+    const cpx<T> a = v[0];
+    const cpx<T> be{v[1].x + v[4].x, v[1].y + v[4].y}, bme{v[1].x - v[4].x, v[1].y - v[4].y};
+    const cpx<T> cd{v[2].x + v[3].x, v[2].y + v[3].y}, cmd{v[2].x - v[3].x, v[2].y - v[3].y};
+    const cpx<T> m1{a.x + c1 * be.x + c2 * cd.x, a.y + c1 * be.y + c2 * cd.y};
+    const cpx<T> m2{a.x + c2 * be.x + c1 * cd.x, a.y + c2 * be.y + c1 * cd.y};
+    const cpx<T> q1{s1 * bme.x + s2 * cmd.x, s1 * bme.y + s2 * cmd.y};
+    const cpx<T> q2{s2 * bme.x - s1 * cmd.x, s2 * bme.y - s1 * cmd.y};
+    v[0] = {a.x + be.x + cd.x, a.y + be.y + cd.y};
+    v[1] = {m1.x - q1.y, m1.y + q1.x};
+    v[4] = {m1.x + q1.y, m1.y - q1.x};
+    v[2] = {m2.x - q2.y, m2.y + q2.x};
+    v[3] = {m2.x + q2.y, m2.y - q2.x};
+  }
+};
+
+// Composite R = R1*R2: n = R2*n1 + n2, k = k1 + R1*k2,
+// X[k] = sum_n2 W_R^{n2 k1} W_R2^{n2 k2} sum_n1 x[R2 n1 + n2] W_R1^{n1 k1}
+template <typename T, int R1, int R2, int SIGN>
+__device__ __forceinline__ void dft_composite(cpx<T>* v) {
+  constexpr int R = R1 * R2;
+  cpx<T> y[R];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; ++n2) {
+    cpx<T> a[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; ++n1) a[n1] = v[R2 * n1 + n2];
+    DFT<T, R1, SIGN>::run(a);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; ++k1) y[n2 * R1 + k1] = a[k1];
+  }
+  // constant twiddles W_R^{n2 k1} (compile-time values after unrolling)
+#pragma unroll
+  for (int n2 = 1; n2 < R2; ++n2)
+#pragma unroll
+    for (int k1 = 1; k1 < R1; ++k1) {
+      const int e = (n2 * k1) % R;
+      const T c = T(kTw<R>.c[e]);
+      const T s = T(SIGN) * T(kTw<R>.s[e]);
+      const cpx<T> a = y[n2 * R1 + k1];
+      y[n2 * R1 + k1] = {a.x * c - a.y * s, a.x * s + a.y * c};
+    }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; ++k1) {
+    cpx<T> b[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; ++n2) b[n2] = y[n2 * R1 + k1];
+    DFT<T, R2, SIGN>::run(b);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; ++k2) v[k1 + R1 * k2] = b[k2];
+  }
+}
+template <typename T, int SIGN> struct DFT<T, 6, SIGN> {
+  __device__ __forceinline__ static void run(cpx<T>* v) { dft_composite<T, 3, 2, SIGN>(v); } };
+template <typename T, int SIGN> struct DFT<T, 8, SIGN> {
+  __device__ __forceinline__ static void run(cpx<T>* v) { dft_composite<T, 4, 2, SIGN>(v); } };
+template <typename T, int SIGN> struct DFT<T, 9, SIGN> {
+  __device__ __forceinline__ static void run(cpx<T>* v) { dft_composite<T, 3, 3, SIGN>(v); } };
+template <typename T, int SIGN> struct DFT<T, 12, SIGN> {
+  __device__ __forceinline__ static void run(cpx<T>* v) { dft_composite<T, 4, 3, SIGN>(v); } };
+template <typename T, int SIGN> struct DFT<T, 16, SIGN> {
+  __device__ __forceinline__ static void run(cpx<T>* v) { dft_composite<T, 4, 4, SIGN>(v); } };
+
+// ---------------------------------------------------------------- plans
+template <int... R>
+struct Plan {
+  static constexpr int P = sizeof...(R);
+  static constexpr int rad[P] = {R...};
+  static constexpr int L = (1 * ... * R);
+  // LDS position of input sample n (mixed-radix digit reversal)
+  __device__ __forceinline__ static int digitrev(int n) {
+    int pos = 0, len = L;
+#pragma unroll
+    for (int q = P - 1; q >= 0; --q) {
+      len /= rad[q];
+      pos += (n % rad[q]) * len;
+      n /= rad[q];
+    }
+    return pos;
+  }
+};
+
+// One DIT pass and the remaining ones.  NCOL interleaved columns (element n of
+// column c at buf[c + n*NCOL]); NTHR threads cooperate; ends with a barrier.
+template <typename T, int L, int NCOL, int NTHR, int SIGN, int S, int TWO, int R, int... Rest>
+__device__ __forceinline__ void dit_passes(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
+  constexpr int NB = L / R;
+  constexpr int TOT = NB * NCOL;
+  constexpr int ITER = (TOT + NTHR - 1) / NTHR;
+#pragma unroll
+  for (int it = 0; it < ITER; ++it) {
+    const int b = tid + it * NTHR;
+    if ((TOT % NTHR) == 0 || b < TOT) {
+      const int col = b % NCOL;
+      const int bb = b / NCOL;
+      const int k = bb % S;
+      const int base = (bb - k) * R + k;
+      cpx<T> v[R];
+#pragma unroll
+      for (int q = 0; q < R; ++q) v[q] = buf[col + (base + q * S) * NCOL];
+      if constexpr (S > 1) {
+#pragma unroll
+        for (int q = 1; q < R; ++q) {
+          const cpx<T> w = tw[TWO + k * (R - 1) + (q - 1)];
+          v[q] = (SIGN < 0) ? cx_mul(v[q], w) : cx_mulc(v[q], w);
+        }
+      }
+      DFT<T, R, SIGN>::run(v);
+#pragma unroll
+      for (int q = 0; q < R; ++q) buf[col + (base + q * S) * NCOL] = v[q];
+    }
+  }
+  __syncthreads();
+  if constexpr (sizeof...(Rest) > 0)
+    dit_passes<T, L, NCOL, NTHR, SIGN, S * R, TWO + S * (R - 1), Rest...>(buf, tw, tid);
+}
+
+template <typename T, int NCOL, int NTHR, int SIGN, int... R>
+__device__ __forceinline__ void fft_run(Plan<R...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
+  dit_passes<T, Plan<R...>::L, NCOL, NTHR, SIGN, 1, 0, R...>(buf, tw, tid);
+}
+
+}  // namespace nsp

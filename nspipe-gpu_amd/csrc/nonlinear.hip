@@ -309,6 +309,13 @@ void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const
   if (pf) pf->begin(K_DR3, s);
   dr3_kernel<<<gd, 256, 0, s>>>(g, t.D, u, nb.dR);
   if (pf) pf->end(s);
+  if (nb.use_ct) {
+    const CtTables<double> tb{(const cpx<double>*)nb.ct_tw_t, (const cpx<double>*)nb.ct_tw_h,
+                              (const cpx<double>*)nb.tw_z};
+    if (launch_nonlinear_ct<double>(g, t, u, nb.dR, (cpx<double>*)nb.W, (cpx<double>*)nb.Y, nb.wstride, tb, N, s,
+                                    pf))
+      return;
+  }
   if (pf) pf->begin(K_THETA_INV, s);
   theta_dispatch(g, t, nb, u, N, s, true);
   if (pf) pf->end(s);

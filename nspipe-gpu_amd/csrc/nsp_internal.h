@@ -93,6 +93,10 @@ struct NlBufs {
   double2* W;          // [9][NR*NTP*NZ] theta-inverse intermediates
   double2* Y;          // [3][NR*NTP*NZ] z-forward intermediates
   size_t wstride;      // NR*NTP*NZ
+  // compile-time plans (when the lengths have one): per-pass twiddles
+  bool use_ct;
+  double2* ct_tw_t;    // [NTP-1]
+  double2* ct_tw_h;    // [NH-1]
 };
 
 // ----------------------------------------------------------------- profiler
@@ -143,6 +147,23 @@ struct Profiler {
     for (auto e : pool) (void)hipEventDestroy(e);
   }
 };
+
+// Compiled-plan support (nonlinear_ct.hip).
+template <typename T>
+struct CtTables {
+  const cpx<T>* tw_t;  // pass twiddles of the theta plan
+  const cpx<T>* tw_h;  // pass twiddles of the half-z plan
+  const cpx<T>* tw_z;  // exp(-2 pi i k / NZF), k <= NH
+};
+bool ct_has_theta(int L);
+bool ct_has_zhalf(int L);
+std::vector<int> ct_radices(int L, bool theta);
+// per-pass twiddles of a plan: for each pass (radix R, sub-length S) S*(R-1)
+// entries exp(-2 pi i q k / (S R)) at [k*(R-1) + q-1]
+std::vector<double2> pass_twiddles(const std::vector<int>& radices);
+template <typename T>
+bool launch_nonlinear_ct(const Geom& g, const DevTables& t, const double2* u, const double2* dR, cpx<T>* W,
+                         cpx<T>* Y, size_t ws, const CtTables<T>& tb, double2* N, hipStream_t s, Profiler* pf);
 
 // Launchers (HIP sources).
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u,

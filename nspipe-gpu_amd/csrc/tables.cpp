@@ -204,4 +204,19 @@ bool make_fft_plan(int L, FftPlan& p, std::vector<int>& perm) {
   return true;
 }
 
+std::vector<double2> pass_twiddles(const std::vector<int>& radices) {
+  std::vector<double2> tw;
+  int S = 1;
+  for (int R : radices) {
+    for (int k = 0; k < S; k++)
+      for (int q = 1; q < R; q++) {
+        const long long e = (long long)q * k;
+        const double a = -PI2 * double(e % (S * R)) / double(S * R);
+        tw.push_back(make_double2(std::cos(a), std::sin(a)));
+      }
+    S *= R;
+  }
+  return tw;
+}
+
 }  // namespace nsp

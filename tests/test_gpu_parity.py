@@ -83,6 +83,43 @@ def test_nonlinear_parity(shape):
     p.close()
 
 
+# shapes whose transform lengths (3NT/2 and 3(NZ-1)/2) have compiled plans
+COMPILED = [
+    dict(nr=8, nt=96, nz=513, lz=50.0, re=2000.0, dt=0.005),      # 144, 768
+    dict(nr=8, nt=128, nz=1025, lz=50.0, re=2000.0, dt=0.005),    # 192, 1536
+    dict(nr=8, nt=192, nz=385, lz=2 * np.pi, re=5300.0, dt=0.002),  # 288, 576
+    dict(nr=8, nt=384, nz=1537, lz=10 * np.pi, re=11700.0, dt=0.001),  # 576, 2304
+]
+
+
+@pytest.mark.parametrize("shape", COMPILED, ids=lambda s: f"{s['nr']}x{s['nt']}x{s['nz']}")
+def test_nonlinear_parity_compiled_plans(shape):
+    print(f"[gpu] nonlinear parity (compiled plans) {shape}", flush=True)
+    p = nsp.Pipe(**shape)
+    o = Oracle(**shape)
+    p.seeded_ic(SEED1, 6, 6, 1, 0.1)
+    u = p.get_state()
+    rng = np.random.default_rng(4)
+    for c in range(3):  # broadband content on top of the seeded field
+        u[c] = u[c] + 1e-3 * (rng.standard_normal(p.shape) + 1j * rng.standard_normal(p.shape))
+    a = p.nonlinear(*u)
+    b = o.nonlinear(*u)
+    fields_close(a, b, 1e-12)
+    p.close()
+
+
+def test_generic_and_compiled_transforms_agree(monkeypatch):
+    shape = COMPILED[3]
+    p1 = nsp.Pipe(**shape)
+    monkeypatch.setenv("NSP_GENERIC_FFT", "1")
+    p2 = nsp.Pipe(**shape)
+    p1.seeded_ic(SEED1, 4, 4, 0, 0.075)
+    u = p1.get_state()
+    fields_close(p1.nonlinear(*u), p2.nonlinear(*u), 1e-13)
+    p1.close()
+    p2.close()
+
+
 def run_pair(shape, nsteps, ic, tol):
     print(f"[gpu] step parity {shape} steps={nsteps} ic={ic}", flush=True)
     p = nsp.Pipe(**shape)
@@ -121,6 +158,10 @@ def test_one_step_parity_cfg1_reference_ic():
 
 def test_hundred_steps_parity_cfg1():
     run_pair(CFG1, 100, 1e-3, 1e-9)
+
+
+def test_one_step_parity_headline_transforms():
+    run_pair(COMPILED[3], 1, 0.075, 1e-12)
 
 
 def test_poiseuille_fixed_point_gpu():

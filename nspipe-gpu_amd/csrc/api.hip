@@ -38,12 +38,14 @@ struct nsp_handle {
   double2* u = nullptr;     // current state (r/theta)
   double2* ui = nullptr;    // iterate
   double2* un = nullptr;    // next iterate
-  double2* uw = nullptr;    // u^n (+/-)
   double2* Nl = nullptr;    // nonlinear term
-  double2* Nw = nullptr;    // N(u^n) (+/-)
-  double2* rhs = nullptr;   // scratch rhs
-  double2* p = nullptr;     // scratch pressure (1 field)
-  double* U = nullptr;      // scratch LU factor (4 doubles per row and mode)
+  double2* E = nullptr;     // explicit part of the step (+/-)
+  double2* SR = nullptr;    // sweep scratch (3 fields)
+  double2* SP = nullptr;    // sweep scratch (1 field)
+  double* F = nullptr;      // banded LU factors [4][NR][7][MODES]
+  double* PH = nullptr;     // homogeneous pressure data [NR][2][MODES]
+  double* Dc = nullptr;     // centred D table
+  double* Luc = nullptr;    // centred Lu table
   unsigned long long* norm = nullptr;  // device [2]
   unsigned long long* norm_h = nullptr;  // pinned host [2]
   double* part = nullptr;   // diagnostics partials [MODES][4]
@@ -127,19 +129,24 @@ nsp_status fetch_fields(nsp_handle* h, const double2* src, double* a, double* b,
   return NSP_OK;
 }
 
-LinArgs lin_args(nsp_handle* h, const double2* N, const double2* u, double2* out, bool predictor) {
-  LinArgs a;
+Lin2Args lin_args(nsp_handle* h, const double2* N, const double2* u, double2* out, bool predictor) {
+  Lin2Args a;
   a.g = h->g;
-  a.t = h->t;
-  a.b = h->b;
+  a.r = h->t.r;
+  a.rdr = h->t.rdr;
+  a.uc = h->t.uc;
+  a.qc = h->t.qc;
+  a.Dc = h->Dc;
+  a.Luc = h->Luc;
+  a.F = h->F;
+  a.PH = h->PH;
+  a.invc = h->b.invc;
   a.N = N;
-  a.Nw = h->Nw;
+  a.E = h->E;
   a.u = u;
-  a.uw = h->uw;
   a.out = out;
-  a.rhs = h->rhs;
-  a.p = h->p;
-  a.U = h->U;
+  a.SR = h->SR;
+  a.SP = h->SP;
   a.norm = h->norm;
   a.predictor = predictor ? 1 : 0;
   return a;
@@ -253,14 +260,14 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   TRY(dalloc(h, &h->u, 3 * n));
   TRY(dalloc(h, &h->ui, 3 * n));
   TRY(dalloc(h, &h->un, 3 * n));
-  TRY(dalloc(h, &h->uw, 3 * n));
   TRY(dalloc(h, &h->Nl, 3 * n));
-  TRY(dalloc(h, &h->Nw, 3 * n));
-  TRY(dalloc(h, &h->rhs, 3 * n));
-  TRY(dalloc(h, &h->p, n));
-  TRY(dalloc(h, &h->U, 4 * n));
-  TRY(dalloc(h, &h->b.uh, 3 * n));
-  TRY(dalloc(h, &h->b.uhP, 3 * n));
+  TRY(dalloc(h, &h->E, 3 * n));
+  TRY(dalloc(h, &h->SR, 3 * n));
+  TRY(dalloc(h, &h->SP, n));
+  TRY(dalloc(h, &h->F, 4 * 7 * n));
+  TRY(dalloc(h, &h->PH, 2 * n));
+  TRY(upload(h, &h->Dc, centred_table(g.nr, h->host.D)));
+  TRY(upload(h, &h->Luc, centred_table(g.nr, h->host.Lu)));
   TRY(dalloc(h, &h->b.invc, 4 * size_t(g.modes)));
   TRY(dalloc(h, &h->nb.dR, 3 * n));
   TRY(dalloc(h, &h->nb.W, 9 * h->nb.wstride));
@@ -270,10 +277,31 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   HTRY(hipHostMalloc((void**)&h->norm_h, 2 * sizeof(unsigned long long), hipHostMallocDefault));
   HTRY(hipMemsetAsync(h->u, 0, 3 * n * sizeof(double2), h->stream));
 
-  // influence matrix, homogeneous solutions and flux profile (setBCon, setCublasFlux)
-  launch_setup_influence(g, h->t, h->b, h->p, h->U, h->stream);
-  HTRY(hipGetLastError());
-  HTRY(hipStreamSynchronize(h->stream));
+  // influence matrix, homogeneous solutions and flux profile (setBCon,
+  // setCublasFlux) with the reference's solver order; its full-field
+  // homogeneous solutions are only needed here
+  {
+    double2 *uh = nullptr, *uhP = nullptr, *P = nullptr;
+    double* U = nullptr;
+    hipError_t e1 = hipMalloc(&uh, 3 * n * sizeof(double2));
+    hipError_t e2 = hipMalloc(&uhP, 3 * n * sizeof(double2));
+    hipError_t e3 = hipMalloc(&P, n * sizeof(double2));
+    hipError_t e4 = hipMalloc(&U, 4 * n * sizeof(double));
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess || e4 != hipSuccess) {
+      (void)hipFree(uh); (void)hipFree(uhP); (void)hipFree(P); (void)hipFree(U);
+      return bail(fail(NSP_ERR_NOMEM, "hipMalloc failed (setup buffers)"));
+    }
+    h->b.uh = uh;
+    h->b.uhP = uhP;
+    launch_setup_influence(g, h->t, h->b, P, U, h->PH, h->stream);
+    launch_factor(g, h->t.r, h->t.Lu, h->t.Lp, h->F, h->stream);
+    hipError_t e5 = hipGetLastError();
+    hipError_t e6 = hipStreamSynchronize(h->stream);
+    (void)hipFree(uh); (void)hipFree(uhP); (void)hipFree(P); (void)hipFree(U);
+    h->b.uh = h->b.uhP = nullptr;
+    if (e5 != hipSuccess || e6 != hipSuccess)
+      return bail(fail(NSP_ERR_HIP, std::string("setup: ") + hipGetErrorString(e5 != hipSuccess ? e5 : e6)));
+  }
 #undef TRY
 #undef HTRY
   *out = h;
@@ -353,7 +381,7 @@ nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {
     // nonlinear(u,rhs); predictor(u,rhs,uw,rhsw,ui)
     launch_nonlinear(g, h->t, h->nb, h->u, h->Nl, s, pf);
     if (pf) pf->begin(K_LIN_PRED, s);
-    launch_linear_sweep(lin_args(h, h->Nl, h->u, h->ui, true), s);
+    launch_sweep2(lin_args(h, h->Nl, h->u, h->ui, true), s);
     if (pf) pf->end(s);
     HIPCHK(hipGetLastError());
     double err = 1e98;
@@ -363,7 +391,7 @@ nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {
       launch_nonlinear(g, h->t, h->nb, h->ui, h->Nl, s, pf);
       launch_norm_reset(h->norm, s);
       if (pf) pf->begin(K_LIN_CORR, s);
-      launch_linear_sweep(lin_args(h, h->Nl, h->ui, h->un, false), s);
+      launch_sweep2(lin_args(h, h->Nl, h->ui, h->un, false), s);
       if (pf) pf->end(s);
       HIPCHK(hipGetLastError());
       HIPCHK(hipMemcpyAsync(h->norm_h, h->norm, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));

@@ -378,7 +378,7 @@ __global__ void norm_reset_kernel(unsigned long long* norm) {
 // setBCon (mod_boundary.cu:205-255) and setCublasFlux (mod_cublasflux.cu:297-321),
 // one thread per mode.
 __global__ __launch_bounds__(256) void setup_influence_kernel(Geom g, DevTables t, DevInfluence b,
-                                                              double2* P, double* Us) {
+                                                              double2* P, double* Us, double* PH) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= g.modes) return;
   const Mode q = mode_of(g, k);
@@ -432,6 +432,8 @@ __global__ __launch_bounds__(256) void setup_influence_kernel(Geom g, DevTables 
       const size_t h = at(g, i, k);
       const double2 dr = stencil(g, t.D, ip0, i, P, k);
       const double2 ph = P[h];
+      PH[(size_t)(2 * i) * g.modes + k] = dr.x;  // p_h is real: its d/dr and value
+      PH[(size_t)(2 * i + 1) * g.modes + k] = ph.x;
       b.uhP[h] = dr;
       b.uhP[n + h] = make_double2(-q.kt * ph.y / t.r[i], q.kt * ph.x / t.r[i]);
       b.uhP[2 * n + h] = make_double2(-q.kz * ph.y, q.kz * ph.x);
@@ -530,8 +532,8 @@ void launch_linear_sweep(const LinArgs& a, hipStream_t s) {
 }
 void launch_norm_reset(unsigned long long* norm, hipStream_t s) { norm_reset_kernel<<<1, 64, 0, s>>>(norm); }
 void launch_setup_influence(const Geom& g, const DevTables& t, const DevInfluence& b, double2* P, double* U,
-                            hipStream_t s) {
-  setup_influence_kernel<<<(g.modes + 255) / 256, 256, 0, s>>>(g, t, b, P, U);
+                            double* PH, hipStream_t s) {
+  setup_influence_kernel<<<(g.modes + 255) / 256, 256, 0, s>>>(g, t, b, P, U, PH);
 }
 void launch_diagnostics(const Geom& g, const DevTables& t, const double2* u, double* part, hipStream_t s) {
   diag_kernel<<<(g.modes + 255) / 256, 256, 0, s>>>(g, t, u, part);

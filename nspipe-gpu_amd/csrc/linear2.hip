@@ -1,0 +1,542 @@
+// Linear phase, streaming form: one thread per Fourier mode, four passes
+// over the radial column with sliding register windows.
+//
+//   A (up)   rhs = E + d_im N (corrector) or N + CN part of u^n (predictor,
+//            which also stores the explicit part E); to r/theta; divergence
+//            (div_k, zero wall row); forward elimination of the pressure system
+//   B (down) pressure back substitution; rhs -= grad p three rows behind
+//            (addGrad_k); to +/-; zero wall rows (write_bcon_k)
+//   C (up)   forward elimination of the u+, u-, uz systems
+//   D (down) back substitution of u+, u-, uz together with the homogeneous
+//            wall-unit solutions; after the four wall rows the wall
+//            divergence fixes the influence coefficients (corBCon), which
+//            are then applied row by row; to r/theta; convergence measure
+//            (mcor_k) and output.  Mode (0,0) then takes modeZZ_k and the
+//            flux correction (adj_flux).
+//
+// The banded LU factors of the four operators depend only on (m, kz), dt and
+// Re, so they are computed once at setup with the reference's recurrence
+// (LUsolve_*_k, mod_linear.cu:55-646) and stored mode-fastest; every access of
+// a wavefront is one contiguous line.  Scratch SR (3 complex per row) and SP
+// (1 complex per row) carry the column between passes.
+#include "nsp_internal.h"
+
+namespace nsp {
+
+struct ModeQ {
+  int k, it, l, m;
+  double kt, kz;
+};
+
+__device__ __forceinline__ ModeQ modeq(const Geom& g, int k) {
+  ModeQ q;
+  q.k = k;
+  q.it = k / g.nz;
+  q.l = k - q.it * g.nz;
+  q.m = q.it < g.nt / 2 ? q.it : q.it - g.nt;
+  q.kt = double(q.m);
+  q.kz = g.kz0 * double(q.l);
+  return q;
+}
+
+__device__ __forceinline__ void pm_of(double2& a, double2& b) {  // decoupleForward_k
+  const double2 ur = a, ut = b;
+  a = make_double2(ur.x - ut.y, ur.y + ut.x);
+  b = make_double2(ur.x + ut.y, ur.y - ut.x);
+}
+__device__ __forceinline__ void rt_of(double2& a, double2& b) {  // decoupleBackward_k
+  const double2 up = a, um = b;
+  a = make_double2(0.5 * (up.x + um.x), 0.5 * (up.y + um.y));
+  b = make_double2(0.5 * (up.y - um.y), -0.5 * (up.x - um.x));
+}
+
+// factor slot c of operator `kind` at row i: c 0..2 = L multipliers of columns
+// i-3..i-1, 3 = 1/U_ii, 4..6 = U_{i,i+1..i+3}
+__device__ __forceinline__ size_t fidx(const Geom& g, int kind, int i, int c, int k) {
+  return ((size_t)(kind * g.nr + i) * 7 + c) * g.modes + k;
+}
+
+// ------------------------------------------------------------------ factors
+// Reference recurrence of LUsolve_*_k without the right-hand side.
+__global__ __launch_bounds__(256) void factor_kernel(Geom g, const double* __restrict__ r,
+                                                     const double* __restrict__ Lu,
+                                                     const double* __restrict__ Lp, double* __restrict__ F) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int kind = blockIdx.y;
+  if (k >= g.modes) return;
+  const ModeQ q = modeq(g, k);
+  const double* LHS = (kind == 3) ? Lp : Lu;
+  const int fp = (kind <= 1) ? 1 : 0;
+  const int ip0 = abs((abs(q.m) % 2) - fp) * STEN * g.nr;
+  const bool vel = kind < 3;
+  const bool p00 = (kind == 3) && (q.l == 0 && q.m == 0);
+  const double a = g.d_im / g.re, idt = 1.0 / g.dt;
+  const double kt = q.kt, kz = q.kz;
+  const int nr = g.nr;
+  double Uw[IW][IW + 1];  // U rows i-3..i-1 (Uw[2] = row i-1)
+#pragma unroll
+  for (int s = 0; s < IW; s++)
+#pragma unroll
+    for (int j = 0; j <= IW; j++) Uw[s][j] = 0.0;
+  for (int i = 0; i < nr; i++) {
+    const int left = max(0, i - IW), right = min(i + IW, nr - 1);
+    const int i0 = i + IW - right;
+    const int nl = i - left;
+    const double r2 = r[i] * r[i];
+    const bool last = (i == nr - 1);
+    double L[IW];
+#pragma unroll
+    for (int j = 0; j < IW; j++) {
+      L[j] = 0.0;
+      if (j < nl) {
+        double XM = LHS[ip0 + j + i0 + STEN * i];
+        if (!last) {
+          if (vel) XM = -a * XM;
+        } else if (p00) {
+          XM = 0.0;
+        }
+        double Lj = XM;
+        // U row of column left+h lives at Uw[IW - nl + h]
+#pragma unroll
+        for (int h = 0; h < IW; h++)
+          if (h < j) Lj -= L[h] * Uw[IW - nl + h][j - h];
+        Lj = Lj / Uw[IW - nl + j][0];
+        L[j] = Lj;
+      }
+    }
+    double Ui[IW + 1] = {0.0, 0.0, 0.0, 0.0};
+    int coun = nl;
+#pragma unroll
+    for (int j = 0; j <= IW; j++) {
+      if (j < right - i + 1) {
+        double XM = LHS[ip0 + STEN * i + i0 + coun];
+        if (!last) {
+          if (j == 0) {
+            if (kind == 0) XM += -(1.0 / r2) - (2.0 * kt / r2) - kz * kz - kt * kt / r2;
+            else if (kind == 1) XM += -(1.0 / r2) + (2.0 * kt / r2) - kz * kz - kt * kt / r2;
+            else XM += -kz * kz - kt * kt / r2;
+            if (vel) XM = idt - a * XM;
+          } else if (vel) {
+            XM = -a * XM;
+          }
+        } else if (p00) {
+          XM = (j == 0) ? 1.0 : 0.0;
+        }
+        const int hleft = max(0, coun - IW);
+        coun += 1;
+        double Uij = XM;
+#pragma unroll
+        for (int h = 0; h < IW; h++)
+          if (h >= hleft && h < nl) Uij -= L[h] * Uw[IW - nl + h][j + nl - h];
+        Ui[j] = Uij;
+      }
+    }
+    // store: L aligned to columns i-3..i-1
+#pragma unroll
+    for (int c = 0; c < IW; c++) {
+      const int j = c - (IW - nl);  // multiplier index of column i-3+c
+      F[fidx(g, kind, i, c, k)] = (j >= 0) ? L[j] : 0.0;
+    }
+    F[fidx(g, kind, i, 3, k)] = 1.0 / Ui[0];
+#pragma unroll
+    for (int j = 1; j <= IW; j++) F[fidx(g, kind, i, 3 + j, k)] = Ui[j];
+    // shift the U window
+#pragma unroll
+    for (int s = 0; s < IW - 1; s++)
+#pragma unroll
+      for (int j = 0; j <= IW; j++) Uw[s][j] = Uw[s + 1][j];
+#pragma unroll
+    for (int j = 0; j <= IW; j++) Uw[IW - 1][j] = Ui[j];
+  }
+}
+
+// ------------------------------------------------------------------ sweep
+template <int PRED>
+__global__ __launch_bounds__(256) void sweep_kernel(Lin2Args A) {
+  const Geom& g = A.g;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.modes) return;
+  const ModeQ q = modeq(g, k);
+  const size_t n = g.n;
+  const int nr = g.nr;
+  const int MS = g.modes;
+  const double d_im = g.d_im;
+  const int par_u = abs((abs(q.m) % 2) - 1);  // parity table of ur, ut (u+, u-)
+  const int par_p = abs(q.m) % 2;             // parity table of uz, p
+  const double* Dcu = A.Dc + par_u * STEN * nr;
+  const double* Dcp = A.Dc + par_p * STEN * nr;
+  const double2 z2 = make_double2(0.0, 0.0);
+
+  // ============================== pass A (up)
+  {
+    constexpr int LAG = PRED ? 6 : 3;
+    double2 uw[3][7];     // predictor: u^n +/- rows t-6..t
+    double2 Rr[7];        // R_r rows i-3..i+3
+    double2 Rt[4], Rz[4]; // R_t, R_z rows i..i+3
+    double2 yp[3];        // pressure elimination rows i-3..i-1
+#pragma unroll
+    for (int w = 0; w < 7; w++) {
+      Rr[w] = z2;
+#pragma unroll
+      for (int c = 0; c < 3; c++) uw[c][w] = z2;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; w++) Rt[w] = Rz[w] = z2;
+#pragma unroll
+    for (int w = 0; w < 3; w++) yp[w] = z2;
+    const double cn = (1.0 - d_im) / g.re, idt = 1.0 / g.dt;
+    for (int t = 0; t < nr + LAG; t++) {
+      if (PRED) {
+        double2 a = z2, b = z2, c = z2;
+        if (t < nr) {
+          const size_t h = (size_t)t * MS + k;
+          a = A.u[h];
+          b = A.u[n + h];
+          c = A.u[2 * n + h];
+          pm_of(a, b);
+        }
+#pragma unroll
+        for (int w = 0; w < 6; w++) {
+          uw[0][w] = uw[0][w + 1];
+          uw[1][w] = uw[1][w + 1];
+          uw[2][w] = uw[2][w + 1];
+        }
+        uw[0][6] = a;
+        uw[1][6] = b;
+        uw[2][6] = c;
+      }
+      // rhs row j
+      const int j = PRED ? t - 3 : t;
+      double2 R0 = z2, R1 = z2, R2 = z2;
+      if (j >= 0 && j < nr) {
+        const size_t h = (size_t)j * MS + k;
+        double2 n0 = A.N[h], n1 = A.N[n + h];
+        const double2 n2 = A.N[2 * n + h];
+        pm_of(n0, n1);
+        double2 rh[3];
+        if (PRED) {
+          // addLap_k (mod_linear.cu:12-51) on u^n, then rhs = N + that
+          const double rj = A.r[j], r2 = rj * rj;
+          const double2 nn[3] = {n0, n1, n2};
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const double* Lc = A.Luc + ((c < 2) ? par_u : par_p) * STEN * nr + STEN * j;
+            double px = 0.0, py = 0.0;
+#pragma unroll
+            for (int w = 0; w < 7; w++) {
+              px += Lc[w] * uw[c][w].x;
+              py += Lc[w] * uw[c][w].y;
+            }
+            const double2 aux = uw[c][3];
+            double de = -q.kz * q.kz - q.kt * q.kt / r2;
+            if (c == 0) de += -(1.0 / r2) - (2.0 * q.kt / r2);
+            else if (c == 1) de += -(1.0 / r2) + (2.0 * q.kt / r2);
+            px += de * aux.x;
+            py += de * aux.y;
+            px *= cn;
+            py *= cn;
+            px += idt * aux.x;
+            py += idt * aux.y;
+            rh[c] = make_double2(nn[c].x + px, nn[c].y + py);
+            A.E[c * n + h] = make_double2(px + (1.0 - d_im) * nn[c].x, py + (1.0 - d_im) * nn[c].y);
+          }
+        } else {
+          const double2 e0 = A.E[h], e1 = A.E[n + h], e2 = A.E[2 * n + h];
+          rh[0] = make_double2(e0.x + d_im * n0.x, e0.y + d_im * n0.y);
+          rh[1] = make_double2(e1.x + d_im * n1.x, e1.y + d_im * n1.y);
+          rh[2] = make_double2(e2.x + d_im * n2.x, e2.y + d_im * n2.y);
+        }
+        rt_of(rh[0], rh[1]);
+        R0 = rh[0];
+        R1 = rh[1];
+        R2 = rh[2];
+        A.SR[h] = R0;
+        A.SR[n + h] = R1;
+        A.SR[2 * n + h] = R2;
+      }
+#pragma unroll
+      for (int w = 0; w < 6; w++) Rr[w] = Rr[w + 1];
+      Rr[6] = R0;
+#pragma unroll
+      for (int w = 0; w < 3; w++) {
+        Rt[w] = Rt[w + 1];
+        Rz[w] = Rz[w + 1];
+      }
+      Rt[3] = R1;
+      Rz[3] = R2;
+      // divergence and pressure elimination at row i = j - 3
+      const int i = j - 3;
+      if (i >= 0 && i < nr) {
+        double2 dv = z2;
+        if (i < nr - 1) {  // div_k (mod_radialFD.cu:67-96), wall row zero
+          const double* Dr = Dcu + STEN * i;
+          double px = 0.0, py = 0.0;
+#pragma unroll
+          for (int w = 0; w < 7; w++) {
+            px += Dr[w] * Rr[w].x;
+            py += Dr[w] * Rr[w].y;
+          }
+          const double rK = A.r[i];
+          const double2 ur = Rr[3], ut = Rt[0], uz = Rz[0];
+          px += ur.x / rK;
+          py += ur.y / rK;
+          px += -q.kt * ut.y / rK;
+          py += q.kt * ut.x / rK;
+          px += -q.kz * uz.y;
+          py += q.kz * uz.x;
+          dv = make_double2(px, py);
+        }
+        const double l0 = A.F[fidx(g, 3, i, 0, k)], l1 = A.F[fidx(g, 3, i, 1, k)], l2 = A.F[fidx(g, 3, i, 2, k)];
+        dv.x -= l0 * yp[0].x;
+        dv.y -= l0 * yp[0].y;
+        dv.x -= l1 * yp[1].x;
+        dv.y -= l1 * yp[1].y;
+        dv.x -= l2 * yp[2].x;
+        dv.y -= l2 * yp[2].y;
+        A.SP[(size_t)i * MS + k] = dv;
+        yp[0] = yp[1];
+        yp[1] = yp[2];
+        yp[2] = dv;
+      }
+    }
+  }
+
+  // ============================== pass B (down)
+  {
+    double2 pw[7];  // p rows t..t+6
+#pragma unroll
+    for (int w = 0; w < 7; w++) pw[w] = z2;
+    for (int t = nr - 1; t >= -IW; t--) {
+      double2 pn = z2;
+      if (t >= 0) {
+        const double ru = A.F[fidx(g, 3, t, 3, k)];
+        const double u1 = A.F[fidx(g, 3, t, 4, k)], u2 = A.F[fidx(g, 3, t, 5, k)], u3 = A.F[fidx(g, 3, t, 6, k)];
+        double2 x = A.SP[(size_t)t * MS + k];
+        x.x -= u1 * pw[0].x;
+        x.y -= u1 * pw[0].y;
+        x.x -= u2 * pw[1].x;
+        x.y -= u2 * pw[1].y;
+        x.x -= u3 * pw[2].x;
+        x.y -= u3 * pw[2].y;
+        pn = make_double2(x.x * ru, x.y * ru);
+      }
+#pragma unroll
+      for (int w = 6; w > 0; w--) pw[w] = pw[w - 1];
+      pw[0] = pn;
+      const int j = t + IW;
+      if (j >= 0 && j < nr) {  // addGrad_k (mod_radialFD.cu:115-143)
+        const size_t h = (size_t)j * MS + k;
+        const double* Dr = Dcp + STEN * j;
+        double px = 0.0, py = 0.0;
+#pragma unroll
+        for (int w = 0; w < 7; w++) {
+          px += Dr[w] * pw[w].x;
+          py += Dr[w] * pw[w].y;
+        }
+        const double rK = A.r[j];
+        const double2 ph = pw[3];
+        double2 a = A.SR[h], b = A.SR[n + h], c = A.SR[2 * n + h];
+        a.x -= px;
+        a.y -= py;
+        b.x -= -q.kt * ph.y / rK;
+        b.y -= q.kt * ph.x / rK;
+        c.x -= -q.kz * ph.y;
+        c.y -= q.kz * ph.x;
+        pm_of(a, b);
+        if (j == nr - 1) a = b = c = z2;
+        A.SR[h] = a;
+        A.SR[n + h] = b;
+        A.SR[2 * n + h] = c;
+      }
+    }
+  }
+
+  // ============================== pass C (up)
+  {
+    double2 yw[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int w = 0; w < 3; w++) yw[c][w] = z2;
+    for (int i = 0; i < nr; i++) {
+      const size_t h = (size_t)i * MS + k;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double l0 = A.F[fidx(g, c, i, 0, k)], l1 = A.F[fidx(g, c, i, 1, k)], l2 = A.F[fidx(g, c, i, 2, k)];
+        double2 y = A.SR[c * n + h];
+        y.x -= l0 * yw[c][0].x;
+        y.y -= l0 * yw[c][0].y;
+        y.x -= l1 * yw[c][1].x;
+        y.y -= l1 * yw[c][1].y;
+        y.x -= l2 * yw[c][2].x;
+        y.y -= l2 * yw[c][2].y;
+        A.SR[c * n + h] = y;
+        yw[c][0] = yw[c][1];
+        yw[c][1] = yw[c][2];
+        yw[c][2] = y;
+      }
+    }
+  }
+
+  // ============================== pass D (down)
+  {
+    double2 xw[3][3];   // x rows i+1..i+3
+    double xh[3][3];    // homogeneous wall-unit solutions rows i+1..i+3
+#pragma unroll
+    for (int c = 0; c < 3; c++)
+#pragma unroll
+      for (int w = 0; w < 3; w++) {
+        xw[c][w] = z2;
+        xh[c][w] = 0.0;
+      }
+    double2 xs[4][3];   // wall rows NR-1..NR-4
+    double xhs[4][3];
+    auto backsub = [&](int i, double2* xo, double* ho) {
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        const double ru = A.F[fidx(g, c, i, 3, k)];
+        const double u1 = A.F[fidx(g, c, i, 4, k)], u2 = A.F[fidx(g, c, i, 5, k)], u3 = A.F[fidx(g, c, i, 6, k)];
+        double2 x = A.SR[c * n + (size_t)i * MS + k];
+        x.x -= u1 * xw[c][0].x;
+        x.y -= u1 * xw[c][0].y;
+        x.x -= u2 * xw[c][1].x;
+        x.y -= u2 * xw[c][1].y;
+        x.x -= u3 * xw[c][2].x;
+        x.y -= u3 * xw[c][2].y;
+        x = make_double2(x.x * ru, x.y * ru);
+        double hh = (i == nr - 1) ? 1.0 : 0.0;
+        hh -= u1 * xh[c][0];
+        hh -= u2 * xh[c][1];
+        hh -= u3 * xh[c][2];
+        hh *= ru;
+        xw[c][2] = xw[c][1];
+        xw[c][1] = xw[c][0];
+        xw[c][0] = x;
+        xh[c][2] = xh[c][1];
+        xh[c][1] = xh[c][0];
+        xh[c][0] = hh;
+        xo[c] = x;
+        ho[c] = hh;
+      }
+    };
+#pragma unroll
+    for (int s = 0; s < 4; s++) backsub(nr - 1 - s, xs[s], xhs[s]);
+    // wall divergence of the provisional solution (divWall_k on r/theta form)
+    double2 urw[4], utw = z2, uzw = xs[0][2];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+      double2 a = xs[s][0], b = xs[s][1];
+      rt_of(a, b);
+      urw[s] = a;
+      if (s == 0) utw = b;
+    }
+    double2 dv;
+    {
+      const double* Dr = Dcu + STEN * (nr - 1);
+      double px = 0.0, py = 0.0;
+#pragma unroll
+      for (int w = 0; w < 4; w++) {  // columns NR-4..NR-1 = urw[3..0]
+        px += Dr[w] * urw[3 - w].x;
+        py += Dr[w] * urw[3 - w].y;
+      }
+      px += urw[0].x;
+      py += urw[0].y;
+      px += -q.kt * utw.y;
+      py += q.kt * utw.x;
+      px += -q.kz * uzw.y;
+      py += q.kz * uzw.x;
+      dv = make_double2(px, py);
+    }
+    double2 cf[4];  // compcoef_k
+    {
+      const double2 nd = make_double2(-dv.x, -dv.y);
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const double2 im = A.invc[(size_t)k * 4 + j];
+        cf[j] = make_double2(nd.x * im.x - nd.y * im.y, nd.x * im.y + nd.y * im.x);
+      }
+    }
+    double num = 0.0, den = 0.0;
+    auto emit = [&](int i, const double2* xo, const double* ho) {
+      const size_t h = (size_t)i * MS + k;
+      // pressure-gradient homogeneous solution: (dr p_h -+ kt p_h / r, 0), (0, kz p_h)
+      const double dph = A.PH[(size_t)(2 * i) * MS + k], ph = A.PH[(size_t)(2 * i + 1) * MS + k];
+      const double tq = q.kt * ph / A.r[i];
+      const double2 hp[3] = {make_double2(dph - tq, 0.0), make_double2(dph + tq, 0.0), make_double2(-0.0, q.kz * ph)};
+      const double2 hv[3] = {make_double2(ho[0], 0.0), make_double2(ho[1], 0.0), make_double2(0.0, ho[2])};
+      double2 o[3] = {xo[0], xo[1], xo[2]};
+      rt_of(o[0], o[1]);  // decoupleBackward / decoupleForward round trip of corBCon
+      pm_of(o[0], o[1]);
+#pragma unroll
+      for (int c = 0; c < 3; c++) {  // addBCon_k
+        const double ax = cf[c].x * hv[c].x - cf[c].y * hv[c].y + cf[3].x * hp[c].x - cf[3].y * hp[c].y;
+        const double ay = cf[c].x * hv[c].y + cf[c].y * hv[c].x + cf[3].x * hp[c].y + cf[3].y * hp[c].x;
+        o[c] = make_double2(o[c].x + ax, o[c].y + ay);
+      }
+      rt_of(o[0], o[1]);
+      if (!PRED && k != 0) {  // mcor_k; mode (0,0) is measured after its fix-up
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const double2 u1 = A.u[c * n + h];
+          const double dx = u1.x - o[c].x, dy = u1.y - o[c].y;
+          const double a1 = dx * dx + dy * dy, a2 = u1.x * u1.x + u1.y * u1.y;
+          num = (a1 > num || a1 != a1) ? a1 : num;
+          den = (a2 > den || a2 != a2) ? a2 : den;
+        }
+      }
+      A.out[h] = o[0];
+      A.out[n + h] = o[1];
+      A.out[2 * n + h] = o[2];
+    };
+#pragma unroll
+    for (int s = 0; s < 4; s++) emit(nr - 1 - s, xs[s], xhs[s]);
+    for (int i = nr - 5; i >= 0; i--) {
+      double2 xo[3];
+      double ho[3];
+      backsub(i, xo, ho);
+      emit(i, xo, ho);
+    }
+    if (k == 0) {  // modeZZ_k, adj_flux (mod_integrate.cu:63-68, mod_cublasflux.cu:331-344)
+      double qa = 0.0;
+      for (int i = 0; i < nr; i++) {
+        const size_t h = (size_t)i * MS;
+        A.out[h] = z2;
+        A.out[n + h].y = 0.0;
+        A.out[2 * n + h].y = 0.0;
+        qa += A.rdr[i] * A.out[2 * n + h].x;
+      }
+      const double corr = (g.qdes - qa) / A.qc[0];
+      for (int i = 0; i < nr; i++) {
+        const size_t h = (size_t)i * MS;
+        A.out[2 * n + h].x += corr * A.uc[i];
+        if (!PRED) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            const double2 u1 = A.u[c * n + h], o = A.out[c * n + h];
+            const double dx = u1.x - o.x, dy = u1.y - o.y;
+            const double a1 = dx * dx + dy * dy, a2 = u1.x * u1.x + u1.y * u1.y;
+            num = (a1 > num || a1 != a1) ? a1 : num;
+            den = (a2 > den || a2 != a2) ? a2 : den;
+          }
+        }
+      }
+    }
+    if (!PRED) {
+      atomicMax(&A.norm[0], (unsigned long long)__double_as_longlong(num));
+      atomicMax(&A.norm[1], (unsigned long long)__double_as_longlong(den));
+    }
+  }
+}
+
+void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, hipStream_t s) {
+  dim3 grid((g.modes + 255) / 256, 4);
+  factor_kernel<<<grid, 256, 0, s>>>(g, r, Lu, Lp, F);
+}
+
+void launch_sweep2(const Lin2Args& a, hipStream_t s) {
+  const int blocks = (a.g.modes + 255) / 256;
+  if (a.predictor) sweep_kernel<1><<<blocks, 256, 0, s>>>(a);
+  else sweep_kernel<0><<<blocks, 256, 0, s>>>(a);
+}
+
+}  // namespace nsp

@@ -81,6 +81,32 @@ struct LinArgs {
   int predictor;
 };
 
+// Streaming linear sweep (linear2.hip) argument block.
+struct Lin2Args {
+  Geom g;
+  const double* r;      // [NR]
+  const double* rdr;    // [NR]
+  const double* uc;     // [NR] flux profile
+  const double* qc;     // [1]
+  const double* Dc;     // [2][NR][7] D weights of columns i-3..i+3 (0 outside the grid)
+  const double* Luc;    // [2][NR][7] same for Lu
+  const double* F;      // [4][NR][7][MODES] banded LU factors
+  const double* PH;     // [NR][2][MODES] dr p_h, p_h of the homogeneous pressure solution
+  const double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
+  const double2* N;     // [3] nonlinear term (r/theta)
+  double2* E;           // [3] explicit part (+/-): written by the predictor
+  const double2* u;     // [3] predictor: u^n; corrector: u^k (r/theta)
+  double2* out;         // [3] new iterate (r/theta)
+  double2* SR;          // [3] scratch
+  double2* SP;          // [1] scratch
+  unsigned long long* norm;
+  int predictor;
+};
+void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, hipStream_t s);
+void launch_sweep2(const Lin2Args& a, hipStream_t s);
+// centred stencil table: w-th weight of row i multiplies column i-3+w
+std::vector<double> centred_table(int nr, const std::vector<double>& T);
+
 // Nonlinear-phase buffers and FFT data.
 struct NlBufs {
   FftPlan pt, ph;      // theta (length NTP) and half-length z (length NZF/2) plans
@@ -171,7 +197,7 @@ void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const
 void launch_linear_sweep(const LinArgs& a, hipStream_t s);
 void launch_norm_reset(unsigned long long* norm, hipStream_t s);
 void launch_setup_influence(const Geom& g, const DevTables& t, const DevInfluence& b,
-                            double2* scratch_p, double* scratch_U, hipStream_t s);
+                            double2* scratch_p, double* scratch_U, double* PH, hipStream_t s);
 void launch_diagnostics(const Geom& g, const DevTables& t, const double2* u, double* partial,
                         hipStream_t s);
 

@@ -204,6 +204,20 @@ bool make_fft_plan(int L, FftPlan& p, std::vector<int>& perm) {
   return true;
 }
 
+std::vector<double> centred_table(int nr, const std::vector<double>& T) {
+  std::vector<double> C(2 * STEN * nr, 0.0);
+  for (int p = 0; p < 2; p++)
+    for (int i = 0; i < nr; i++) {
+      const int left = std::max(0, i - IW), right = std::min(i + IW, nr - 1);
+      const int i0 = i + IW - right;
+      for (int s = 0; s < right + 1 - left; s++) {
+        const int w = left + s - (i - IW);
+        C[p * STEN * nr + STEN * i + w] = T[p * STEN * nr + STEN * i + s + i0];
+      }
+    }
+  return C;
+}
+
 std::vector<double2> pass_twiddles(const std::vector<int>& radices) {
   std::vector<double2> tw;
   int S = 1;

@@ -148,6 +148,15 @@ int nsp_profile_count(void);
 const char* nsp_profile_name(int i);
 nsp_status nsp_profile_read(nsp_handle* h, double* total_ms, long long* launches);
 
+/* Autotuner (no reference counterpart): times every compiled kernel variant of
+ * the nonlinear stages (radial derivative fused or separate, azimuthal tile
+ * width, threads per axial row) on this handle's sizes, device and current
+ * state, `reps` launches each, and keeps the fastest per stage.
+ * nsp_tuning returns the timings and the choice as JSON text (valid until
+ * the next call on the handle). */
+nsp_status nsp_autotune(nsp_handle* h, int reps);
+const char* nsp_tuning(nsp_handle* h);
+
 /* HIP stream of the handle (a hipStream_t), for callers that order their own
  * work against it. */
 void* nsp_stream(nsp_handle* h);

@@ -51,6 +51,7 @@ struct nsp_handle {
   double* part = nullptr;   // diagnostics partials [MODES][4]
   std::vector<void*> allocs;
   Profiler prof;
+  std::string tune_report = "{\"tuned\": false}";
 };
 
 namespace {
@@ -82,7 +83,6 @@ nsp_status check_params(const nsp_params* p) {
   if (!(p->lz > 0) || !(p->re > 0) || !(p->dt > 0)) return fail(NSP_ERR_INVALID, "lz, re, dt must be > 0");
   if (p->maxit < 1) return fail(NSP_ERR_INVALID, "maxit must be >= 1");
   if (p->precision != NSP_FP64 && p->precision != NSP_MIXED) return fail(NSP_ERR_INVALID, "bad precision");
-  if (p->precision == NSP_MIXED) return fail(NSP_ERR_INVALID, "the mixed-precision form is not built yet");
   const int nzf = 3 * (p->nz - 1);
   if (nzf > 36 * 256) return fail(NSP_ERR_INVALID, "3(nz-1) must be <= 9216");
   return NSP_OK;
@@ -150,6 +150,12 @@ Lin2Args lin_args(nsp_handle* h, const double2* N, const double2* u, double2* ou
   a.norm = h->norm;
   a.predictor = predictor ? 1 : 0;
   return a;
+}
+
+std::vector<float2> to_float2(const std::vector<double2>& v) {
+  std::vector<float2> o(v.size());
+  for (size_t q = 0; q < v.size(); q++) o[q] = make_float2(float(v[q].x), float(v[q].y));
+  return o;
 }
 
 double bits_to_double(unsigned long long b) {
@@ -249,9 +255,22 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   TRY(upload(h, &h->nb.tw_z, twiddles(g.nzf, g.nh + 1)));
   h->nb.wstride = size_t(g.nr) * g.ntp * g.nz;
   h->nb.use_ct = false;
-  if (ct_has_theta(g.ntp) && ct_has_zhalf(g.nh) && !getenv("NSP_GENERIC_FFT")) {
-    TRY(upload(h, &h->nb.ct_tw_t, pass_twiddles(ct_radices(g.ntp, true))));
-    TRY(upload(h, &h->nb.ct_tw_h, pass_twiddles(ct_radices(g.nh, false))));
+  h->nb.mixed = p->precision == NSP_MIXED;
+  h->nb.var = NlVariant{0, 4, 4, 256};
+  const bool have_ct = ct_has_theta(g.ntp) && ct_has_zhalf(g.nh);
+  if (h->nb.mixed && !have_ct)
+    return bail(fail(NSP_ERR_INVALID, "the mixed form needs compiled transform plans for 3NT/2 and 3(NZ-1)/2"));
+  if (have_ct && (h->nb.mixed || !getenv("NSP_GENERIC_FFT"))) {
+    const auto twt = pass_twiddles(ct_radices(g.ntp, true));
+    const auto twh = pass_twiddles(ct_radices(g.nh, false));
+    if (h->nb.mixed) {
+      TRY(upload(h, &h->nb.ct_tw_t32, to_float2(twt)));
+      TRY(upload(h, &h->nb.ct_tw_h32, to_float2(twh)));
+      TRY(upload(h, &h->nb.tw_z32, to_float2(twiddles(g.nzf, g.nh + 1))));
+    } else {
+      TRY(upload(h, &h->nb.ct_tw_t, twt));
+      TRY(upload(h, &h->nb.ct_tw_h, twh));
+    }
     h->nb.use_ct = true;
   }
 
@@ -268,10 +287,16 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   TRY(dalloc(h, &h->PH, 2 * n));
   TRY(upload(h, &h->Dc, centred_table(g.nr, h->host.D)));
   TRY(upload(h, &h->Luc, centred_table(g.nr, h->host.Lu)));
+  h->nb.Dc = h->Dc;
   TRY(dalloc(h, &h->b.invc, 4 * size_t(g.modes)));
   TRY(dalloc(h, &h->nb.dR, 3 * n));
-  TRY(dalloc(h, &h->nb.W, 9 * h->nb.wstride));
-  TRY(dalloc(h, &h->nb.Y, 3 * h->nb.wstride));
+  if (h->nb.mixed) {
+    TRY(dalloc(h, &h->nb.W32, 9 * h->nb.wstride));
+    TRY(dalloc(h, &h->nb.Y32, 3 * h->nb.wstride));
+  } else {
+    TRY(dalloc(h, &h->nb.W, 9 * h->nb.wstride));
+    TRY(dalloc(h, &h->nb.Y, 3 * h->nb.wstride));
+  }
   TRY(dalloc(h, &h->norm, 2));
   TRY(dalloc(h, &h->part, 4 * size_t(g.modes)));
   HTRY(hipHostMalloc((void**)&h->norm_h, 2 * sizeof(unsigned long long), hipHostMallocDefault));
@@ -368,6 +393,102 @@ nsp_status nsp_profile_read(nsp_handle* h, double* total_ms, long long* launches
   }
   return NSP_OK;
 }
+
+// Times each compiled kernel variant of the nonlinear stages on the handle's
+// own sizes and current state, and keeps the fastest per stage.
+nsp_status nsp_autotune(nsp_handle* h, int reps) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  if (!h->nb.use_ct) {
+    h->tune_report = "{\"tuned\": false}";
+    return NSP_OK;
+  }
+  if (reps < 1) reps = 3;
+  const Geom& g = h->g;
+  hipStream_t s = h->stream;
+  NlBufs& nb = h->nb;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  auto stage = [&](int which, const NlVariant& v) -> bool {
+    if (which == 0) {
+      if (!v.fused) launch_dr3(g, h->t.D, h->u, nb.dR, s);
+      return nb.mixed ? ct_theta_inv<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, h->t.r, nb.Dc, h->u, nb.dR,
+                                            (cpx<float>*)nb.W32, nb.wstride, s)
+                      : ct_theta_inv<double>(v, g, (const cpx<double>*)nb.ct_tw_t, h->t.r, nb.Dc, h->u, nb.dR,
+                                             (cpx<double>*)nb.W, nb.wstride, s);
+    }
+    if (which == 1)
+      return nb.mixed ? ct_zpass<float>(v, g, (const cpx<float>*)nb.ct_tw_h32, (const cpx<float>*)nb.tw_z32, h->t.r,
+                                        (const cpx<float>*)nb.W32, nb.wstride, (cpx<float>*)nb.Y32, s)
+                      : ct_zpass<double>(v, g, (const cpx<double>*)nb.ct_tw_h, (const cpx<double>*)nb.tw_z, h->t.r,
+                                         (const cpx<double>*)nb.W, nb.wstride, (cpx<double>*)nb.Y, s);
+    return nb.mixed ? ct_theta_fwd<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, (const cpx<float>*)nb.Y32,
+                                          nb.wstride, h->t.D, h->u, h->Nl, s)
+                    : ct_theta_fwd<double>(v, g, (const cpx<double>*)nb.ct_tw_t, (const cpx<double>*)nb.Y,
+                                           nb.wstride, h->t.D, h->u, h->Nl, s);
+  };
+  std::string rep = "{\"tuned\": true, \"candidates_ms\": {";
+  bool first = true;
+  auto time_it = [&](int which, const NlVariant& v, const char* label) -> double {
+    if (!stage(which, v)) return -1.0;  // not compiled for this size
+    if (hipStreamSynchronize(s) != hipSuccess) return -1.0;
+    (void)hipEventRecord(e0, s);
+    for (int q = 0; q < reps; q++) stage(which, v);
+    (void)hipEventRecord(e1, s);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    const double t = double(ms) / reps;
+    char buf[160];
+    std::snprintf(buf, sizeof buf, "%s\"%s\": %.4f", first ? "" : ", ", label, t);
+    rep += buf;
+    first = false;
+    return t;
+  };
+  NlVariant best = nb.var;
+  double bt = 1e30;
+  for (int fused = 0; fused < 2; fused++)
+    for (int tl : {4, 8}) {
+      NlVariant v = best;
+      v.fused = fused;
+      v.tl_inv = tl;
+      char lab[64];
+      std::snprintf(lab, sizeof lab, "theta_inv fused=%d tl=%d", fused, tl);
+      const double t = time_it(0, v, lab);
+      if (t >= 0 && t < bt) { bt = t; best.fused = fused; best.tl_inv = tl; }
+    }
+  bt = 1e30;
+  for (int nthr : {256, 512}) {
+    NlVariant v = best;
+    v.z_nthr = nthr;
+    char lab[64];
+    std::snprintf(lab, sizeof lab, "zpass nthr=%d", nthr);
+    const double t = time_it(1, v, lab);
+    if (t >= 0 && t < bt) { bt = t; best.z_nthr = nthr; }
+  }
+  bt = 1e30;
+  for (int tl : {4, 8}) {
+    NlVariant v = best;
+    v.tl_fwd = tl;
+    char lab[64];
+    std::snprintf(lab, sizeof lab, "theta_fwd tl=%d", tl);
+    const double t = time_it(2, v, lab);
+    if (t >= 0 && t < bt) { bt = t; best.tl_fwd = tl; }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  HIPCHK(hipGetLastError());
+  nb.var = best;
+  char buf[200];
+  std::snprintf(buf, sizeof buf, "}, \"chosen\": {\"fused\": %d, \"tl_inv\": %d, \"z_nthr\": %d, \"tl_fwd\": %d}}",
+                best.fused, best.tl_inv, best.z_nthr, best.tl_fwd);
+  rep += buf;
+  h->tune_report = rep;
+  return NSP_OK;
+}
+
+const char* nsp_tuning(nsp_handle* h) { return h ? h->tune_report.c_str() : ""; }
 
 // integrate() loop body (mod_integrate.cu:112-131)
 nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {

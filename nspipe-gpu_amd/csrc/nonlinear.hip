@@ -303,19 +303,51 @@ const char* kernel_name(int id) {
   return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 
+void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, hipStream_t s) {
+  dim3 gd((g.modes + 255) / 256, g.nr);
+  dr3_kernel<<<gd, 256, 0, s>>>(g, D, u, dR);
+}
+
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u, double2* N,
                       hipStream_t s, Profiler* pf) {
-  dim3 gd((g.modes + 255) / 256, g.nr);
-  if (pf) pf->begin(K_DR3, s);
-  dr3_kernel<<<gd, 256, 0, s>>>(g, t.D, u, nb.dR);
-  if (pf) pf->end(s);
   if (nb.use_ct) {
-    const CtTables<double> tb{(const cpx<double>*)nb.ct_tw_t, (const cpx<double>*)nb.ct_tw_h,
-                              (const cpx<double>*)nb.tw_z};
-    if (launch_nonlinear_ct<double>(g, t, u, nb.dR, (cpx<double>*)nb.W, (cpx<double>*)nb.Y, nb.wstride, tb, N, s,
-                                    pf))
-      return;
+    const NlVariant& v = nb.var;
+    bool ok = true;
+    if (!v.fused) {
+      if (pf) pf->begin(K_DR3, s);
+      launch_dr3(g, t.D, u, nb.dR, s);
+      if (pf) pf->end(s);
+    }
+    if (pf) pf->begin(K_THETA_INV, s);
+    if (nb.mixed)
+      ok &= ct_theta_inv<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, t.r, nb.Dc, u, nb.dR, (cpx<float>*)nb.W32,
+                                nb.wstride, s);
+    else
+      ok &= ct_theta_inv<double>(v, g, (const cpx<double>*)nb.ct_tw_t, t.r, nb.Dc, u, nb.dR, (cpx<double>*)nb.W,
+                                 nb.wstride, s);
+    if (pf) pf->end(s);
+    if (pf) pf->begin(K_ZPASS, s);
+    if (nb.mixed)
+      ok &= ct_zpass<float>(v, g, (const cpx<float>*)nb.ct_tw_h32, (const cpx<float>*)nb.tw_z32, t.r,
+                            (const cpx<float>*)nb.W32, nb.wstride, (cpx<float>*)nb.Y32, s);
+    else
+      ok &= ct_zpass<double>(v, g, (const cpx<double>*)nb.ct_tw_h, (const cpx<double>*)nb.tw_z, t.r,
+                             (const cpx<double>*)nb.W, nb.wstride, (cpx<double>*)nb.Y, s);
+    if (pf) pf->end(s);
+    if (pf) pf->begin(K_THETA_FWD, s);
+    if (nb.mixed)
+      ok &= ct_theta_fwd<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, (const cpx<float>*)nb.Y32, nb.wstride, t.D, u,
+                                N, s);
+    else
+      ok &= ct_theta_fwd<double>(v, g, (const cpx<double>*)nb.ct_tw_t, (const cpx<double>*)nb.Y, nb.wstride, t.D, u,
+                                 N, s);
+    if (pf) pf->end(s);
+    (void)ok;  // variants are validated when chosen
+    return;
   }
+  if (pf) pf->begin(K_DR3, s);
+  launch_dr3(g, t.D, u, nb.dR, s);
+  if (pf) pf->end(s);
   if (pf) pf->begin(K_THETA_INV, s);
   theta_dispatch(g, t, nb, u, N, s, true);
   if (pf) pf->end(s);

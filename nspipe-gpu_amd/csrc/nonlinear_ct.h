@@ -3,57 +3,107 @@
 // theta forward), with the transform precision T as a template parameter:
 // T = double for the FP64 form, float for the FP32 transforms of the mixed
 // form.  Lengths without a compiled plan use the generic kernels.
+#pragma once
 #include "fft_ct.h"
 
 namespace nsp {
 
 __device__ __forceinline__ int m_of_ct(int it, int nt) { return it < nt / 2 ? it : it - nt; }
 
+// ------------------------------------------------------------------ block order
+// Workgroups are dealt to the 8 XCDs round robin; remap so that each XCD
+// walks one contiguous range of logical blocks and neighbouring tiles and
+// radial levels meet in the same L2.
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  constexpr int NX = 8;
+  const int q = G / NX, rmd = G % NX;
+  const int x = b % NX, j = b / NX;
+  return (x < rmd ? x * (q + 1) : rmd * (q + 1) + (x - rmd) * q) + j;
+}
+
 // ------------------------------------------------------------------ theta inverse
-// u, d_r u: FP64 state; W: 9 intermediates [f][NR][NTP][NZ] in precision T.
-template <typename T, int TL, int NTHR, typename PL>
+// W: 9 intermediates [f][NR][NTP][NZ] in precision T: u, d_r u, d_theta u / r.
+// FUSED: the radial derivative (derivr_k, mod_radialFD.cu:10-30, flags 1,1,0)
+// is formed here from levels i-3..i+3 read through L2; otherwise it is read
+// from dR (dr3_kernel).
+template <typename T, int TL, int NTHR, int FUSED, typename PL>
 __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                      const double* __restrict__ r,
+                                                     const double* __restrict__ Dc,
                                                      const double2* __restrict__ u,
                                                      const double2* __restrict__ dR,
                                                      cpx<T>* __restrict__ W, size_t ws) {
   constexpr int NTP = PL::L;
+  constexpr int NT = 2 * NTP / 3;
+  constexpr int EPT = (NT * TL + NTHR - 1) / NTHR;
   __shared__ cpx<T> buf[NTP * TL];
-  const int l0 = blockIdx.x * TL, i = blockIdx.y;
+  const int nr = g.nr, nz = g.nz;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = lb % nr, l0 = (lb / nr) * TL;
   const int tid = threadIdx.x;
   const double ri = r[i];
-  const int nt = g.nt, half = nt / 2, nz = g.nz;
-  for (int f = 0; f < 9; f++) {
-    const int c = f % 3, kind = f / 3;  // 0: u, 1: d_r u, 2: d_theta u / r
-    const double2* src = (kind == 1 ? dR : u) + c * g.n + (size_t)i * g.modes;
-    // padding rows (NT/2 <= j < NT) are zero; every other slot is written below
-    for (int e = tid; e < (NTP - nt) * TL; e += NTHR) {
-      const int j = half + e / TL, tl = e % TL;
-      buf[PL::digitrev(j) * TL + tl] = cpx<T>{T(0), T(0)};
-    }
-    for (int e = tid; e < nt * TL; e += NTHR) {
+  constexpr int half = NT / 2;
+  for (int c = 0; c < 3; c++) {
+    const double2* src = u + c * g.n;
+    double2 U[EPT], DR[EPT];
+#pragma unroll
+    for (int q = 0; q < EPT; q++) {
+      const int e = tid + q * NTHR;
       const int it = e / TL, tl = e - it * TL, l = l0 + tl;
-      cpx<T> w{T(0), T(0)};
-      if (l < nz) {
-        const double2 v = src[(size_t)it * nz + l];
-        if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
-          const double kt = double(m_of_ct(it, nt));
-          w = cpx<T>{T(-kt * v.y / ri), T(kt * v.x / ri)};
+      U[q] = DR[q] = make_double2(0.0, 0.0);
+      if (e < NT * TL && l < nz) {
+        const size_t off = (size_t)it * nz + l;
+        if (FUSED) {
+          const int mm = it < half ? it : it - NT;
+          const int par = abs((abs(mm) % 2) - (c < 2 ? 1 : 0));
+          const double* Dr = Dc + (par * nr + i) * STEN;
+          double px = 0.0, py = 0.0;
+#pragma unroll
+          for (int w = 0; w < STEN; w++) {
+            const int lev = i - IW + w;
+            if (lev >= 0 && lev < nr) {
+              const double2 a = src[(size_t)lev * g.modes + off];
+              px += Dr[w] * a.x;
+              py += Dr[w] * a.y;
+              if (w == IW) U[q] = a;
+            }
+          }
+          DR[q] = make_double2(px, py);
         } else {
-          w = cpx<T>{T(v.x), T(v.y)};
+          U[q] = src[(size_t)i * g.modes + off];
+          DR[q] = dR[c * g.n + (size_t)i * g.modes + off];
         }
       }
-      const int j = it < half ? it : it + half;  // padForward_k row map
-      buf[PL::digitrev(j) * TL + tl] = w;
     }
-    __syncthreads();
-    fft_run<T, TL, NTHR, +1>(PL{}, buf, tw, tid);
-    cpx<T>* dst = W + f * ws + (size_t)i * NTP * nz;
-    for (int e = tid; e < NTP * TL; e += NTHR) {
-      const int j = e / TL, tl = e - j * TL, l = l0 + tl;
-      if (l < nz) dst[(size_t)j * nz + l] = buf[e];
+    for (int kind = 0; kind < 3; kind++) {  // 0: u, 1: d_r u, 2: d_theta u / r
+      for (int e = tid; e < (NTP - NT) * TL; e += NTHR) {  // padding rows
+        const int j = half + e / TL, tl = e % TL;
+        buf[PL::digitrev(j) * TL + tl] = cpx<T>{T(0), T(0)};
+      }
+#pragma unroll
+      for (int q = 0; q < EPT; q++) {
+        const int e = tid + q * NTHR;
+        if (e < NT * TL) {
+          const int it = e / TL, tl = e - it * TL;
+          const double2 v = (kind == 1) ? DR[q] : U[q];
+          cpx<T> w{T(v.x), T(v.y)};
+          if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
+            const double kt = double(it < half ? it : it - NT);
+            w = cpx<T>{T(-kt * v.y / ri), T(kt * v.x / ri)};
+          }
+          const int j = it < half ? it : it + half;  // padForward_k row map
+          buf[PL::digitrev(j) * TL + tl] = w;
+        }
+      }
+      __syncthreads();
+      fft_run<T, TL, NTHR, +1>(PL{}, buf, tw, tid);
+      cpx<T>* dst = W + (kind * 3 + c) * ws + (size_t)i * NTP * nz;
+      for (int e = tid; e < NTP * TL; e += NTHR) {
+        const int j = e / TL, tl = e - j * TL, l = l0 + tl;
+        if (l < nz) dst[(size_t)j * nz + l] = buf[e];
+      }
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
@@ -166,7 +216,8 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
                                                      double2* __restrict__ N) {
   constexpr int NTP = PL::L;
   __shared__ cpx<T> buf[NTP * TL];
-  const int l0 = blockIdx.x * TL, i = blockIdx.y;
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int i = lb % g.nr, l0 = (lb / g.nr) * TL;
   const int tid = threadIdx.x;
   const int nt = g.nt, half = nt / 2, nz = g.nz;
   const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
@@ -200,16 +251,15 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
   }
 }
 
-// ------------------------------------------------------------------ dispatch
+// ------------------------------------------------------------------ plans
 // Compiled plans: theta lengths NTP and half z lengths NZF/2 of the
-// supported grids.  TL columns per theta block keep LDS at <= 64 KiB.
+// supported grids.
 #define NSP_THETA_PLANS(X) \
   X(96, Plan<6, 16>)       \
   X(144, Plan<9, 16>)      \
   X(192, Plan<12, 16>)     \
   X(288, Plan<2, 9, 16>)   \
-  X(576, Plan<4, 9, 16>)   \
-  X(1152, Plan<8, 9, 16>)
+  X(576, Plan<4, 9, 16>)
 
 #define NSP_ZHALF_PLANS(X)  \
   X(96, Plan<6, 16>)        \
@@ -218,78 +268,63 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
   X(1536, Plan<6, 16, 16>)  \
   X(2304, Plan<9, 16, 16>)
 
-bool ct_has_theta(int L) {
-#define X(LL, ...) if (L == LL) return true;
+// ------------------------------------------------------------------ stage launchers
+// Return false when the requested variant is not compiled for this size.
+template <typename T>
+bool ct_theta_inv(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double* Dc,
+                  const double2* u, const double2* dR, cpx<T>* W, size_t ws, hipStream_t s) {
+  bool ok = false;
+#define X(LL, ...)                                                                                      \
+  if (g.ntp == LL) {                                                                                    \
+    const int tl = v.tl_inv;                                                                            \
+    if (LL * tl * (int)sizeof(cpx<T>) <= 163840) {                                                      \
+      const int nb = ((g.nz + tl - 1) / tl) * g.nr;                                                     \
+      if (tl == 4 && v.fused) { theta_inv_ct<T, 4, 256, 1, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, r, Dc, u, dR, W, ws); ok = true; } \
+      if (tl == 4 && !v.fused) { theta_inv_ct<T, 4, 256, 0, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, r, Dc, u, dR, W, ws); ok = true; } \
+      if (tl == 8 && v.fused) { theta_inv_ct<T, 8, 256, 1, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, r, Dc, u, dR, W, ws); ok = true; } \
+      if (tl == 8 && !v.fused) { theta_inv_ct<T, 8, 256, 0, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, r, Dc, u, dR, W, ws); ok = true; } \
+    }                                                                                                   \
+  }
   NSP_THETA_PLANS(X)
 #undef X
-  return false;
-}
-bool ct_has_zhalf(int L) {
-#define X(LL, ...) if (L == LL) return true;
-  NSP_ZHALF_PLANS(X)
-#undef X
-  return false;
-}
-// radix list of a compiled plan (host side, for twiddle tables)
-std::vector<int> ct_radices(int L, bool theta) {
-#define X(LL, ...)                                                                 \
-  if (L == LL) {                                                                   \
-    using PP = __VA_ARGS__;                                                        \
-    return std::vector<int>(PP::rad, PP::rad + PP::P);                             \
-  }
-  if (theta) {
-    NSP_THETA_PLANS(X)
-  } else {
-    NSP_ZHALF_PLANS(X)
-  }
-#undef X
-  return {};
-}
-
-template <int NTP>
-constexpr int theta_tl(int bytes) {
-  // largest power of two TL <= 16 with NTP*TL*bytes <= 64 KiB
-  return (NTP * 16 * bytes <= 65536) ? 16 : (NTP * 8 * bytes <= 65536) ? 8 : (NTP * 4 * bytes <= 65536) ? 4 : 2;
+  return ok;
 }
 
 template <typename T>
-bool launch_nonlinear_ct(const Geom& g, const DevTables& t, const double2* u, const double2* dR, cpx<T>* W,
-                         cpx<T>* Y, size_t ws, const CtTables<T>& tb, double2* N, hipStream_t s, Profiler* pf) {
-  if (!ct_has_theta(g.ntp) || !ct_has_zhalf(g.nh)) return false;
-  if (pf) pf->begin(K_THETA_INV, s);
-#define X(LL, ...)                                                                                       \
-  if (g.ntp == LL) {                                                                                     \
-    constexpr int TL = theta_tl<LL>(sizeof(cpx<T>));                                                     \
-    dim3 grid((g.nz + TL - 1) / TL, g.nr);                                                               \
-    theta_inv_ct<T, TL, 256, __VA_ARGS__><<<grid, 256, 0, s>>>(g, tb.tw_t, t.r, u, dR, W, ws);                   \
+bool ct_theta_fwd(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* Y, size_t ws, const double* D,
+                  const double2* u, double2* N, hipStream_t s) {
+  bool ok = false;
+#define X(LL, ...)                                                                                      \
+  if (g.ntp == LL) {                                                                                    \
+    const int tl = v.tl_fwd;                                                                            \
+    if (LL * tl * (int)sizeof(cpx<T>) <= 163840) {                                                      \
+      const int nb = ((g.nz + tl - 1) / tl) * g.nr;                                                     \
+      if (tl == 4) { theta_fwd_ct<T, 4, 256, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N); ok = true; } \
+      if (tl == 8) { theta_fwd_ct<T, 8, 256, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N); ok = true; } \
+    }                                                                                                   \
   }
   NSP_THETA_PLANS(X)
 #undef X
-  if (pf) pf->end(s);
-  if (pf) pf->begin(K_ZPASS, s);
-#define X(LL, ...)                                                                                       \
-  if (g.nh == LL) {                                                                                      \
-    constexpr int MPT = (2 * LL + 255) / 256;                                                            \
-    zpass_ct<T, 256, MPT, __VA_ARGS__><<<g.nr * g.ntp, 256, 0, s>>>(g, tb.tw_h, tb.tw_z, t.r, W, ws, Y);          \
+  return ok;
+}
+
+template <typename T>
+bool ct_zpass(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* twz, const double* r,
+              const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s) {
+  bool ok = false;
+#define X(LL, ...)                                                                                      \
+  if (g.nh == LL) {                                                                                     \
+    if (v.z_nthr == 256) {                                                                              \
+      zpass_ct<T, 256, (2 * LL + 255) / 256, __VA_ARGS__><<<g.nr * g.ntp, 256, 0, s>>>(g, tw, twz, r, W, ws, Y); \
+      ok = true;                                                                                        \
+    } else if (v.z_nthr == 512) {                                                                       \
+      zpass_ct<T, 512, (2 * LL + 511) / 512, __VA_ARGS__><<<g.nr * g.ntp, 512, 0, s>>>(g, tw, twz, r, W, ws, Y); \
+      ok = true;                                                                                        \
+    }                                                                                                   \
   }
   NSP_ZHALF_PLANS(X)
 #undef X
-  if (pf) pf->end(s);
-  if (pf) pf->begin(K_THETA_FWD, s);
-#define X(LL, ...)                                                                                       \
-  if (g.ntp == LL) {                                                                                     \
-    constexpr int TL = theta_tl<LL>(sizeof(cpx<T>));                                                     \
-    dim3 grid((g.nz + TL - 1) / TL, g.nr);                                                               \
-    theta_fwd_ct<T, TL, 256, __VA_ARGS__><<<grid, 256, 0, s>>>(g, tb.tw_t, Y, ws, t.D, u, N);                     \
-  }
-  NSP_THETA_PLANS(X)
-#undef X
-  if (pf) pf->end(s);
-  return true;
+  return ok;
 }
-
-template bool launch_nonlinear_ct<double>(const Geom&, const DevTables&, const double2*, const double2*,
-                                          cpx<double>*, cpx<double>*, size_t, const CtTables<double>&, double2*,
-                                          hipStream_t, Profiler*);
 
 }  // namespace nsp

@@ -1,0 +1,40 @@
+// FP64 instantiation of the compiled-plan nonlinear-term kernels
+// (nonlinear_ct.h); the transform precision of the FP64 form.
+#include "nonlinear_ct.h"
+
+namespace nsp {
+template bool ct_theta_inv<double>(const NlVariant&, const Geom&, const cpx<double>*, const double*, const double*,
+                                  const double2*, const double2*, cpx<double>*, size_t, hipStream_t);
+template bool ct_theta_fwd<double>(const NlVariant&, const Geom&, const cpx<double>*, const cpx<double>*, size_t,
+                                  const double*, const double2*, double2*, hipStream_t);
+template bool ct_zpass<double>(const NlVariant&, const Geom&, const cpx<double>*, const cpx<double>*, const double*,
+                              const cpx<double>*, size_t, cpx<double>*, hipStream_t);
+
+bool ct_has_theta(int L) {
+#define X(LL, ...) if (L == LL) return true;
+  NSP_THETA_PLANS(X)
+#undef X
+  return false;
+}
+bool ct_has_zhalf(int L) {
+#define X(LL, ...) if (L == LL) return true;
+  NSP_ZHALF_PLANS(X)
+#undef X
+  return false;
+}
+// radix list of a compiled plan (host side, for twiddle tables)
+std::vector<int> ct_radices(int L, bool theta) {
+#define X(LL, ...)                                                                 \
+  if (L == LL) {                                                                   \
+    using PP = __VA_ARGS__;                                                        \
+    return std::vector<int>(PP::rad, PP::rad + PP::P);                             \
+  }
+  if (theta) {
+    NSP_THETA_PLANS(X)
+  } else {
+    NSP_ZHALF_PLANS(X)
+  }
+#undef X
+  return {};
+}
+}  // namespace nsp

@@ -107,6 +107,13 @@ void launch_sweep2(const Lin2Args& a, hipStream_t s);
 // centred stencil table: w-th weight of row i multiplies column i-3+w
 std::vector<double> centred_table(int nr, const std::vector<double>& T);
 
+// Kernel variant of the compiled-plan path (chosen by the autotuner).
+struct NlVariant {
+  int fused;   // 1: radial derivative inside the theta-inverse kernel
+  int tl_inv;  // l-columns per theta-inverse block (4 or 8)
+  int tl_fwd;  // l-columns per theta-forward block (4 or 8)
+  int z_nthr;  // threads per z-pass row (256 or 512)
+};
 // Nonlinear-phase buffers and FFT data.
 struct NlBufs {
   FftPlan pt, ph;      // theta (length NTP) and half-length z (length NZF/2) plans
@@ -123,6 +130,15 @@ struct NlBufs {
   bool use_ct;
   double2* ct_tw_t;    // [NTP-1]
   double2* ct_tw_h;    // [NH-1]
+  double* Dc;          // centred D table (fused radial derivative)
+  // mixed form: FP32 transforms and products
+  bool mixed;
+  float2* W32;         // [9][NR*NTP*NZ]
+  float2* Y32;         // [3][NR*NTP*NZ]
+  float2* ct_tw_t32;
+  float2* ct_tw_h32;
+  float2* tw_z32;
+  NlVariant var;
 };
 
 // ----------------------------------------------------------------- profiler
@@ -174,13 +190,7 @@ struct Profiler {
   }
 };
 
-// Compiled-plan support (nonlinear_ct.hip).
-template <typename T>
-struct CtTables {
-  const cpx<T>* tw_t;  // pass twiddles of the theta plan
-  const cpx<T>* tw_h;  // pass twiddles of the half-z plan
-  const cpx<T>* tw_z;  // exp(-2 pi i k / NZF), k <= NH
-};
+// Compiled-plan support (nonlinear_ct64.hip).
 bool ct_has_theta(int L);
 bool ct_has_zhalf(int L);
 std::vector<int> ct_radices(int L, bool theta);
@@ -188,8 +198,15 @@ std::vector<int> ct_radices(int L, bool theta);
 // entries exp(-2 pi i q k / (S R)) at [k*(R-1) + q-1]
 std::vector<double2> pass_twiddles(const std::vector<int>& radices);
 template <typename T>
-bool launch_nonlinear_ct(const Geom& g, const DevTables& t, const double2* u, const double2* dR, cpx<T>* W,
-                         cpx<T>* Y, size_t ws, const CtTables<T>& tb, double2* N, hipStream_t s, Profiler* pf);
+bool ct_theta_inv(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double* Dc,
+                  const double2* u, const double2* dR, cpx<T>* W, size_t ws, hipStream_t s);
+template <typename T>
+bool ct_theta_fwd(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* Y, size_t ws, const double* D,
+                  const double2* u, double2* N, hipStream_t s);
+template <typename T>
+bool ct_zpass(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* twz, const double* r,
+              const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s);
+void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, hipStream_t s);
 
 // Launchers (HIP sources).
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u,

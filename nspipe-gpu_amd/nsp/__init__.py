@@ -39,7 +39,7 @@ EXPORTS = ["nsp_default_params", "nsp_create", "nsp_destroy", "nsp_set_state", "
            "nsp_reference_ic", "nsp_step", "nsp_nonlinear", "nsp_diagnostics", "nsp_tables",
            "nsp_radial_tables", "nsp_read_state_files", "nsp_write_state_files", "nsp_seeded_ic",
            "nsp_profile_enable", "nsp_profile_count", "nsp_profile_name", "nsp_profile_read",
-           "nsp_stream", "nsp_synchronize", "nsp_last_error"]
+           "nsp_autotune", "nsp_tuning", "nsp_stream", "nsp_synchronize", "nsp_last_error"]
 
 
 class NspError(RuntimeError):
@@ -79,8 +79,11 @@ def lib():
     L.nsp_stream.restype = vp
     L.nsp_stream.argtypes = [vp]
     L.nsp_synchronize.argtypes = [vp]
+    L.nsp_autotune.argtypes = [vp, ip]
+    L.nsp_tuning.restype = ctypes.c_char_p
+    L.nsp_tuning.argtypes = [vp]
     for name in EXPORTS:
-        if name not in ("nsp_last_error", "nsp_stream", "nsp_profile_name"):
+        if name not in ("nsp_last_error", "nsp_stream", "nsp_profile_name", "nsp_tuning"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -204,6 +207,12 @@ class Pipe:
         cnt = (ctypes.c_longlong * n)()
         _check(lib().nsp_profile_read(self.h, _dp(ms), cnt))
         return {lib().nsp_profile_name(i).decode(): (float(ms[i]), int(cnt[i])) for i in range(n)}
+
+    def autotune(self, reps=3):
+        """Time the compiled kernel variants on this pipe and keep the fastest."""
+        import json
+        _check(lib().nsp_autotune(self.h, reps))
+        return json.loads(lib().nsp_tuning(self.h).decode())
 
     def synchronize(self):
         """hipDeviceSynchronize on the pipe's device."""

@@ -164,6 +164,53 @@ def test_one_step_parity_headline_transforms():
     run_pair(COMPILED[3], 1, 0.075, 1e-12)
 
 
+def test_autotuned_variants_keep_parity():
+    shape = COMPILED[3]
+    p = nsp.Pipe(**shape)
+    o = Oracle(**shape)
+    p.seeded_ic(SEED1, 4, 4, 0, 0.075)
+    rep = p.autotune(reps=1)
+    assert rep["tuned"] and "chosen" in rep
+    u = p.get_state()
+    fields_close(p.nonlinear(*u), o.nonlinear(*u), 1e-12)
+    p.close()
+
+
+def test_mixed_nonlinear_accuracy():
+    """FP32 transforms and products: relative L2 error of N(u) at FP32 level."""
+    shape = COMPILED[3]
+    p = nsp.Pipe(**shape, precision=nsp.MIXED)
+    o = Oracle(**shape)
+    p.seeded_ic(SEED1, 4, 4, 0, 0.075)
+    u = p.get_state()
+    a, b = p.nonlinear(*u), o.nonlinear(*u)
+    errs = [rel_l2(x, y) for x, y in zip(a, b)]
+    print("[gpu] mixed nonlinear rel L2", errs, flush=True)
+    assert max(errs) < 1e-5
+    p.close()
+
+
+def test_mixed_hundred_steps_cfg1_error_budget():
+    """Mixed form vs FP64 oracle after 100 steps: kinetic energy, wall shear
+    and centre-line velocity within 1e-5 relative, divergence and wall
+    residuals at the FP64 form's level."""
+    p = nsp.Pipe(**CFG1, precision=nsp.MIXED)
+    o = Oracle(**CFG1)
+    p.seeded_ic(SEED1, 4, 4, 0, 0.05)
+    o.set_state(*p.get_state())
+    n_p, _, f_p = p.step(100)
+    n_o, _ = o.step(100)
+    dp, do = p.diagnostics(), o.diagnostics()
+    for key in ("ke", "u_tau", "u_cl"):
+        e = abs(dp[key] - do[key]) / abs(do[key])
+        print(f"[gpu] mixed 100-step {key} rel err {e:.3e}", flush=True)
+        assert e <= 1e-5, key
+    assert np.all(f_p == 0)
+    assert dp["wall_max"] < 1e-12
+    assert dp["div_l2"] <= 2 * do["div_l2"] + 1e-12
+    p.close()
+
+
 def test_poiseuille_fixed_point_gpu():
     shape = dict(nr=16, nt=8, nz=9, lz=2 * np.pi, re=800.0, dt=0.01)
     p = nsp.Pipe(**shape)

@@ -256,7 +256,7 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   h->nb.wstride = size_t(g.nr) * g.ntp * g.nz;
   h->nb.use_ct = false;
   h->nb.mixed = p->precision == NSP_MIXED;
-  h->nb.var = NlVariant{0, 4, 4, 256};
+  h->nb.var = NlVariant{ThetaVar{0, 8, 256, 0, 1}, ThetaVar{0, 8, 256, 0, 1}, 1, 256, 0, 0};
   const bool have_ct = ct_has_theta(g.ntp) && ct_has_zhalf(g.nh);
   if (h->nb.mixed && !have_ct)
     return bail(fail(NSP_ERR_INVALID, "the mixed form needs compiled transform plans for 3NT/2 and 3(NZ-1)/2"));
@@ -287,7 +287,6 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   TRY(dalloc(h, &h->PH, 2 * n));
   TRY(upload(h, &h->Dc, centred_table(g.nr, h->host.D)));
   TRY(upload(h, &h->Luc, centred_table(g.nr, h->host.Lu)));
-  h->nb.Dc = h->Dc;
   TRY(dalloc(h, &h->b.invc, 4 * size_t(g.modes)));
   TRY(dalloc(h, &h->nb.dR, 3 * n));
   if (h->nb.mixed) {
@@ -412,10 +411,10 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
   HIPCHK(hipEventCreate(&e1));
   auto stage = [&](int which, const NlVariant& v) -> bool {
     if (which == 0) {
-      if (!v.fused) launch_dr3(g, h->t.D, h->u, nb.dR, s);
-      return nb.mixed ? ct_theta_inv<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, h->t.r, nb.Dc, h->u, nb.dR,
+      launch_dr3(g, h->t.D, h->u, nb.dR, s);
+      return nb.mixed ? ct_theta_inv<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, h->t.r, h->u, nb.dR,
                                             (cpx<float>*)nb.W32, nb.wstride, s)
-                      : ct_theta_inv<double>(v, g, (const cpx<double>*)nb.ct_tw_t, h->t.r, nb.Dc, h->u, nb.dR,
+                      : ct_theta_inv<double>(v, g, (const cpx<double>*)nb.ct_tw_t, h->t.r, h->u, nb.dR,
                                              (cpx<double>*)nb.W, nb.wstride, s);
     }
     if (which == 1)
@@ -448,41 +447,63 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
   };
   NlVariant best = nb.var;
   double bt = 1e30;
-  for (int fused = 0; fused < 2; fused++)
-    for (int tl : {4, 8}) {
-      NlVariant v = best;
-      v.fused = fused;
-      v.tl_inv = tl;
-      char lab[64];
-      std::snprintf(lab, sizeof lab, "theta_inv fused=%d tl=%d", fused, tl);
-      const double t = time_it(0, v, lab);
-      if (t >= 0 && t < bt) { bt = t; best.fused = fused; best.tl_inv = tl; }
+  // one stage at a time over all its compiled forms
+  const int treg[3][2] = {{4, 192}, {8, 256}, {8, 384}};
+  for (int stage : {0, 2}) {
+    ThetaVar& tb = stage == 0 ? best.inv : best.fwd;
+    const char* nm = stage == 0 ? "theta_inv" : "theta_fwd";
+    bt = 1e30;
+    std::vector<ThetaVar> cands;
+    for (int ord = 0; ord < 2; ord++) {
+      for (int pad = 0; pad < 2; pad++)
+        for (int tl : {4, 8}) cands.push_back(ThetaVar{0, tl, 256, pad, ord});
+      for (const auto& tr : treg) cands.push_back(ThetaVar{1, tr[0], tr[1], 0, ord});
     }
-  bt = 1e30;
-  for (int nthr : {256, 512}) {
-    NlVariant v = best;
-    v.z_nthr = nthr;
-    char lab[64];
-    std::snprintf(lab, sizeof lab, "zpass nthr=%d", nthr);
-    const double t = time_it(1, v, lab);
-    if (t >= 0 && t < bt) { bt = t; best.z_nthr = nthr; }
+    for (const ThetaVar& tv : cands) {
+      NlVariant v = best;
+      (stage == 0 ? v.inv : v.fwd) = tv;
+      char lab[80];
+      std::snprintf(lab, sizeof lab, "%s %s tl=%d nthr=%d pad=%d ord=%d", nm, tv.alg ? "register" : "scatter", tv.tl,
+                    tv.nthr, tv.pad, tv.ord);
+      const double t = time_it(stage, v, lab);
+      if (t >= 0 && t < bt) { bt = t; tb = tv; }
+    }
   }
   bt = 1e30;
-  for (int tl : {4, 8}) {
+  for (int pad = 0; pad < 2; pad++)
+    for (int nthr : {256, 512}) {
+      NlVariant v = best;
+      v.z_alg = 0;
+      v.z_nthr = nthr;
+      v.z_pad = pad;
+      char lab[64];
+      std::snprintf(lab, sizeof lab, "zpass scatter nthr=%d pad=%d", nthr, pad);
+      const double t = time_it(1, v, lab);
+      if (t >= 0 && t < bt) { bt = t; best.z_alg = 0; best.z_nthr = nthr; best.z_pad = pad; }
+    }
+  for (int pf = 0; pf < 2; pf++) {
     NlVariant v = best;
-    v.tl_fwd = tl;
+    v.z_alg = 1;
+    v.z_pf = pf;
     char lab[64];
-    std::snprintf(lab, sizeof lab, "theta_fwd tl=%d", tl);
-    const double t = time_it(2, v, lab);
-    if (t >= 0 && t < bt) { bt = t; best.tl_fwd = tl; }
+    std::snprintf(lab, sizeof lab, "zpass register pf=%d", pf);
+    const double t = time_it(1, v, lab);
+    if (t >= 0 && t < bt) { bt = t; best.z_alg = 1; best.z_pf = pf; }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIPCHK(hipGetLastError());
   nb.var = best;
   char buf[200];
-  std::snprintf(buf, sizeof buf, "}, \"chosen\": {\"fused\": %d, \"tl_inv\": %d, \"z_nthr\": %d, \"tl_fwd\": %d}}",
-                best.fused, best.tl_inv, best.z_nthr, best.tl_fwd);
+  auto tvj = [](const ThetaVar& t) {
+    char o[112];
+    std::snprintf(o, sizeof o, "{\"alg\": %d, \"tl\": %d, \"nthr\": %d, \"pad\": %d, \"ord\": %d}", t.alg, t.tl,
+                  t.nthr, t.pad, t.ord);
+    return std::string(o);
+  };
+  std::snprintf(buf, sizeof buf, ", \"z_alg\": %d, \"z_nthr\": %d, \"z_pad\": %d, \"z_pf\": %d}}", best.z_alg,
+                best.z_nthr, best.z_pad, best.z_pf);
+  rep += "}, \"chosen\": {\"theta_inv\": " + tvj(best.inv) + ", \"theta_fwd\": " + tvj(best.fwd);
   rep += buf;
   h->tune_report = rep;
   return NSP_OK;

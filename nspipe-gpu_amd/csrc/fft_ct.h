@@ -197,44 +197,133 @@ struct Plan {
   }
 };
 
+// LDS layout: logical element e at e + (e / W) * PAD, i.e. rows of W
+// elements each followed by PAD unused slots (PAD = 0: dense).  Row padding
+// turns the power-of-two strides of the passes into distinct banks.
+template <int W, int PAD>
+struct Lay {
+  __device__ __forceinline__ static int pos(int e) {
+    if constexpr (PAD == 0) return e;
+    else return e + (e / W) * PAD;
+  }
+  static constexpr int size(int n) { return PAD == 0 ? n : n + ((n + W - 1) / W) * PAD; }
+};
+using Dense = Lay<1, 0>;
+
+// Butterfly b of one DIT pass (radix R, sub-length S, twiddles at TWO):
+// loads its R elements of column b % NCOL, applies the inter-pass twiddles
+// and the radix-R DFT.  Returns the column and the first logical position;
+// element q belongs at position base + q*S.
+template <typename T, int L, int NCOL, int SIGN, typename LY, int S, int TWO, int R>
+__device__ __forceinline__ void dit_butterfly(const cpx<T>* buf, const cpx<T>* __restrict__ tw, int b, cpx<T>* v,
+                                              int& col, int& base) {
+  col = b % NCOL;
+  const int bb = b / NCOL;
+  const int k = bb % S;
+  base = (bb - k) * R + k;
+#pragma unroll
+  for (int q = 0; q < R; ++q) v[q] = buf[LY::pos(col + (base + q * S) * NCOL)];
+  if constexpr (S > 1) {
+#pragma unroll
+    for (int q = 1; q < R; ++q) {
+      const cpx<T> w = tw[TWO + k * (R - 1) + (q - 1)];
+      v[q] = (SIGN < 0) ? cx_mul(v[q], w) : cx_mulc(v[q], w);
+    }
+  }
+  DFT<T, R, SIGN>::run(v);
+}
+
 // One DIT pass and the remaining ones.  NCOL interleaved columns (element n of
-// column c at buf[c + n*NCOL]); NTHR threads cooperate; ends with a barrier.
-template <typename T, int L, int NCOL, int NTHR, int SIGN, int S, int TWO, int R, int... Rest>
+// column c at logical index c + n*NCOL, stored at LY::pos of it); NTHR
+// threads cooperate; ends with a barrier.
+template <typename T, int L, int NCOL, int NTHR, int SIGN, typename LY, int S, int TWO, int R, int... Rest>
 __device__ __forceinline__ void dit_passes(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
-  constexpr int NB = L / R;
-  constexpr int TOT = NB * NCOL;
+  constexpr int TOT = (L / R) * NCOL;
   constexpr int ITER = (TOT + NTHR - 1) / NTHR;
 #pragma unroll
   for (int it = 0; it < ITER; ++it) {
     const int b = tid + it * NTHR;
     if ((TOT % NTHR) == 0 || b < TOT) {
-      const int col = b % NCOL;
-      const int bb = b / NCOL;
-      const int k = bb % S;
-      const int base = (bb - k) * R + k;
       cpx<T> v[R];
+      int col, base;
+      dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, b, v, col, base);
 #pragma unroll
-      for (int q = 0; q < R; ++q) v[q] = buf[col + (base + q * S) * NCOL];
-      if constexpr (S > 1) {
-#pragma unroll
-        for (int q = 1; q < R; ++q) {
-          const cpx<T> w = tw[TWO + k * (R - 1) + (q - 1)];
-          v[q] = (SIGN < 0) ? cx_mul(v[q], w) : cx_mulc(v[q], w);
-        }
-      }
-      DFT<T, R, SIGN>::run(v);
-#pragma unroll
-      for (int q = 0; q < R; ++q) buf[col + (base + q * S) * NCOL] = v[q];
+      for (int q = 0; q < R; ++q) buf[LY::pos(col + (base + q * S) * NCOL)] = v[q];
     }
   }
   __syncthreads();
   if constexpr (sizeof...(Rest) > 0)
-    dit_passes<T, L, NCOL, NTHR, SIGN, S * R, TWO + S * (R - 1), Rest...>(buf, tw, tid);
+    dit_passes<T, L, NCOL, NTHR, SIGN, LY, S * R, TWO + S * (R - 1), Rest...>(buf, tw, tid);
 }
 
-template <typename T, int NCOL, int NTHR, int SIGN, int... R>
+// Like dit_passes, but the last pass hands its outputs to emit(col, k, value)
+// (k = natural output index) instead of storing them; no barrier after it.
+template <typename T, int L, int NCOL, int NTHR, int SIGN, typename LY, int S, int TWO, int R, int... Rest,
+          typename F>
+__device__ __forceinline__ void dit_passes_emit(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid, F&& emit) {
+  if constexpr (sizeof...(Rest) > 0) {
+    constexpr int TOT = (L / R) * NCOL;
+    constexpr int ITER = (TOT + NTHR - 1) / NTHR;
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+      const int b = tid + it * NTHR;
+      if ((TOT % NTHR) == 0 || b < TOT) {
+        cpx<T> v[R];
+        int col, base;
+        dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, b, v, col, base);
+#pragma unroll
+        for (int q = 0; q < R; ++q) buf[LY::pos(col + (base + q * S) * NCOL)] = v[q];
+      }
+    }
+    __syncthreads();
+    dit_passes_emit<T, L, NCOL, NTHR, SIGN, LY, S * R, TWO + S * (R - 1), Rest...>(buf, tw, tid, emit);
+  } else {
+    static_assert(S * R == L, "last pass");
+    constexpr int TOT = (L / R) * NCOL;
+    constexpr int ITER = (TOT + NTHR - 1) / NTHR;
+#pragma unroll
+    for (int it = 0; it < ITER; ++it) {
+      const int b = tid + it * NTHR;
+      if ((TOT % NTHR) == 0 || b < TOT) {
+        cpx<T> v[R];
+        int col, base;
+        dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, b, v, col, base);
+#pragma unroll
+        for (int q = 0; q < R; ++q) emit(col, base + q * S, v[q]);
+      }
+    }
+  }
+}
+
+// Whole transform: samples placed at digit-reversed positions, result in
+// natural order.
+template <typename T, int NCOL, int NTHR, int SIGN, typename LY, int... R>
 __device__ __forceinline__ void fft_run(Plan<R...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
-  dit_passes<T, Plan<R...>::L, NCOL, NTHR, SIGN, 1, 0, R...>(buf, tw, tid);
+  dit_passes<T, Plan<R...>::L, NCOL, NTHR, SIGN, LY, 1, 0, R...>(buf, tw, tid);
+}
+
+// First pass in registers: the thread with butterfly index n0 < L/R0 of
+// column col holds the samples x[n0 + (L/R0) q], q < R0.  Their
+// digit-reversed positions are Plan::digitrev(n0) + q, so the radix-R0
+// butterfly (no twiddles at sub-length 1) runs on them directly and stores
+// its outputs there; no scatter through LDS is needed.
+template <typename T, int SIGN, typename LY, int NCOL, int R0, int... Rest>
+__device__ __forceinline__ void first_pass_store(Plan<R0, Rest...>, cpx<T>* v, cpx<T>* buf, int n0, int col = 0) {
+  DFT<T, R0, SIGN>::run(v);
+  const int b = Plan<R0, Rest...>::digitrev(n0);
+#pragma unroll
+  for (int q = 0; q < R0; ++q) buf[LY::pos(col + (b + q) * NCOL)] = v[q];
+}
+// Remaining passes after first_pass_store (and a barrier).
+template <typename T, int NTHR, int SIGN, typename LY, int R0, int... Rest>
+__device__ __forceinline__ void fft_rest(Plan<R0, Rest...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
+  dit_passes<T, Plan<R0, Rest...>::L, 1, NTHR, SIGN, LY, R0, R0 - 1, Rest...>(buf, tw, tid);
+}
+// Remaining passes after first_pass_store, the last one emitting.
+template <typename T, int NCOL, int NTHR, int SIGN, typename LY, int R0, int... Rest, typename F>
+__device__ __forceinline__ void fft_rest_emit(Plan<R0, Rest...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid,
+                                              F&& emit) {
+  dit_passes_emit<T, Plan<R0, Rest...>::L, NCOL, NTHR, SIGN, LY, R0, R0 - 1, Rest...>(buf, tw, tid, emit);
 }
 
 }  // namespace nsp

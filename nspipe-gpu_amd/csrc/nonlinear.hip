@@ -26,28 +26,52 @@ using C = cpx<double>;
 __device__ __forceinline__ int m_of(int it, int nt) { return it < nt / 2 ? it : it - nt; }
 
 // ------------------------------------------------------------------ d/dr
+// One thread per (mode, component) walks r from the axis to the wall with the
+// levels i-3..i+3 in a register ring, so every input value is read from HBM
+// once (a 2-D grid over (mode, level) re-reads each level seven times, and a
+// level of the three components does not fit in L2).  Levels up to PF ahead
+// are loaded early so each thread keeps several loads in flight.  The sum
+// runs over the same terms in the same order as derivr_k.
 __global__ __launch_bounds__(256) void dr3_kernel(Geom g, const double* __restrict__ D,
                                                   const double2* __restrict__ u,
                                                   double2* __restrict__ dR) {
+  constexpr int NB = 12, PF = 5;  // ring slots; levels i-3 .. i+3+PF live at step i
+  static_assert(STEN + PF <= NB, "ring too small");
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int i = blockIdx.y;
+  const int c = blockIdx.y;
   if (k >= g.modes) return;
+  const int nr = g.nr;
   const int m = m_of(k / g.nz, g.nt);
-  const int left = max(0, i - IW), right = min(i + IW, g.nr - 1);
-  const int len = right + 1 - left, i0 = i + IW - right;
+  const double* Dp = D + abs((abs(m) % 2) - (c < 2 ? 1 : 0)) * STEN * nr;
+  const double2* src = u + (size_t)c * g.n + k;
+  double2* dst = dR + (size_t)c * g.n + k;
+  const size_t ld = g.modes;
+  double2 w[NB];  // level L in slot L % NB
 #pragma unroll
-  for (int c = 0; c < 3; c++) {
-    const int flag = c < 2 ? 1 : 0;
-    const int ip0 = abs((abs(m) % 2) - flag) * STEN * g.nr;
-    const double2* src = u + c * g.n;
-    double px = 0.0, py = 0.0;
-    for (int j = 0; j < len; j++) {
-      const double de = D[ip0 + STEN * i + j + i0];
-      const double2 a = src[(size_t)(left + j) * g.modes + k];
-      px += de * a.x;
-      py += de * a.y;
+  for (int L = 0; L < IW + PF; L++) w[L] = L < nr ? src[L * ld] : make_double2(0.0, 0.0);
+  for (int ib = 0; ib < nr; ib += NB) {
+#pragma unroll
+    for (int s = 0; s < NB; s++) {
+      const int i = ib + s;
+      if (i < nr) {
+        const int L = i + IW + PF;
+        if (L < nr) w[(s + IW + PF) % NB] = src[L * ld];
+        const int left = max(0, i - IW), right = min(i + IW, nr - 1);
+        const double* Di = Dp + STEN * i + (i + IW - right) - left;  // Di[lev] = D[ip0 + 7i + j + i0]
+        double px = 0.0, py = 0.0;
+#pragma unroll
+        for (int q = 0; q < STEN; q++) {
+          const int lev = i - IW + q;
+          if (lev >= 0 && lev < nr) {
+            const double de = Di[lev];
+            const double2 a = w[(s - IW + q + NB) % NB];
+            px += de * a.x;
+            py += de * a.y;
+          }
+        }
+        dst[i * ld] = make_double2(px, py);
+      }
     }
-    dR[c * g.n + (size_t)i * g.modes + k] = make_double2(px, py);
   }
 }
 
@@ -304,7 +328,7 @@ const char* kernel_name(int id) {
 }
 
 void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, hipStream_t s) {
-  dim3 gd((g.modes + 255) / 256, g.nr);
+  dim3 gd((g.modes + 255) / 256, 3);
   dr3_kernel<<<gd, 256, 0, s>>>(g, D, u, dR);
 }
 
@@ -313,17 +337,15 @@ void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const
   if (nb.use_ct) {
     const NlVariant& v = nb.var;
     bool ok = true;
-    if (!v.fused) {
-      if (pf) pf->begin(K_DR3, s);
-      launch_dr3(g, t.D, u, nb.dR, s);
-      if (pf) pf->end(s);
-    }
+    if (pf) pf->begin(K_DR3, s);
+    launch_dr3(g, t.D, u, nb.dR, s);
+    if (pf) pf->end(s);
     if (pf) pf->begin(K_THETA_INV, s);
     if (nb.mixed)
-      ok &= ct_theta_inv<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, t.r, nb.Dc, u, nb.dR, (cpx<float>*)nb.W32,
+      ok &= ct_theta_inv<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, t.r, u, nb.dR, (cpx<float>*)nb.W32,
                                 nb.wstride, s);
     else
-      ok &= ct_theta_inv<double>(v, g, (const cpx<double>*)nb.ct_tw_t, t.r, nb.Dc, u, nb.dR, (cpx<double>*)nb.W,
+      ok &= ct_theta_inv<double>(v, g, (const cpx<double>*)nb.ct_tw_t, t.r, u, nb.dR, (cpx<double>*)nb.W,
                                  nb.wstride, s);
     if (pf) pf->end(s);
     if (pf) pf->begin(K_ZPASS, s);

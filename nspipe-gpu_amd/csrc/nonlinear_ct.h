@@ -21,25 +21,38 @@ __device__ __forceinline__ int xcd_block(int b, int G) {
   return (x < rmd ? x * (q + 1) : rmd * (q + 1) + (x - rmd) * q) + j;
 }
 
+// (level i, first column l0) of a theta block.  ord 0: level fastest;
+// ord 1: column tile fastest, so the blocks an XCD runs together cover whole
+// rows of one level (contiguous HBM pages, write lines filled in one L2).
+__device__ __forceinline__ void theta_block(const Geom& g, int TL, int ord, int& i, int& l0) {
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  if (ord == 0) {
+    i = lb % g.nr;
+    l0 = (lb / g.nr) * TL;
+  } else {
+    const int nt = (g.nz + TL - 1) / TL;
+    i = lb / nt;
+    l0 = (lb % nt) * TL;
+  }
+}
+
 // ------------------------------------------------------------------ theta inverse
 // W: 9 intermediates [f][NR][NTP][NZ] in precision T: u, d_r u, d_theta u / r.
-// FUSED: the radial derivative (derivr_k, mod_radialFD.cu:10-30, flags 1,1,0)
-// is formed here from levels i-3..i+3 read through L2; otherwise it is read
-// from dR (dr3_kernel).
-template <typename T, int TL, int NTHR, int FUSED, typename PL>
+// The radial derivative (derivr_k, mod_radialFD.cu:10-30, flags 1,1,0) comes
+// from dR (dr3_kernel); forming it here from seven levels was slower.
+template <typename T, int TL, int NTHR, typename LY, typename PL>
 __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                      const double* __restrict__ r,
-                                                     const double* __restrict__ Dc,
                                                      const double2* __restrict__ u,
                                                      const double2* __restrict__ dR,
-                                                     cpx<T>* __restrict__ W, size_t ws) {
+                                                     cpx<T>* __restrict__ W, size_t ws, int ord) {
   constexpr int NTP = PL::L;
   constexpr int NT = 2 * NTP / 3;
   constexpr int EPT = (NT * TL + NTHR - 1) / NTHR;
-  __shared__ cpx<T> buf[NTP * TL];
-  const int nr = g.nr, nz = g.nz;
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int i = lb % nr, l0 = (lb / nr) * TL;
+  __shared__ cpx<T> buf[LY::size(NTP * TL)];
+  const int nz = g.nz;
+  int i, l0;
+  theta_block(g, TL, ord, i, l0);
   const int tid = threadIdx.x;
   const double ri = r[i];
   constexpr int half = NT / 2;
@@ -52,33 +65,15 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
       const int it = e / TL, tl = e - it * TL, l = l0 + tl;
       U[q] = DR[q] = make_double2(0.0, 0.0);
       if (e < NT * TL && l < nz) {
-        const size_t off = (size_t)it * nz + l;
-        if (FUSED) {
-          const int mm = it < half ? it : it - NT;
-          const int par = abs((abs(mm) % 2) - (c < 2 ? 1 : 0));
-          const double* Dr = Dc + (par * nr + i) * STEN;
-          double px = 0.0, py = 0.0;
-#pragma unroll
-          for (int w = 0; w < STEN; w++) {
-            const int lev = i - IW + w;
-            if (lev >= 0 && lev < nr) {
-              const double2 a = src[(size_t)lev * g.modes + off];
-              px += Dr[w] * a.x;
-              py += Dr[w] * a.y;
-              if (w == IW) U[q] = a;
-            }
-          }
-          DR[q] = make_double2(px, py);
-        } else {
-          U[q] = src[(size_t)i * g.modes + off];
-          DR[q] = dR[c * g.n + (size_t)i * g.modes + off];
-        }
+        const size_t off = (size_t)i * g.modes + (size_t)it * nz + l;
+        U[q] = src[off];
+        DR[q] = dR[c * g.n + off];
       }
     }
     for (int kind = 0; kind < 3; kind++) {  // 0: u, 1: d_r u, 2: d_theta u / r
       for (int e = tid; e < (NTP - NT) * TL; e += NTHR) {  // padding rows
         const int j = half + e / TL, tl = e % TL;
-        buf[PL::digitrev(j) * TL + tl] = cpx<T>{T(0), T(0)};
+        buf[LY::pos(PL::digitrev(j) * TL + tl)] = cpx<T>{T(0), T(0)};
       }
 #pragma unroll
       for (int q = 0; q < EPT; q++) {
@@ -92,15 +87,15 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
             w = cpx<T>{T(-kt * v.y / ri), T(kt * v.x / ri)};
           }
           const int j = it < half ? it : it + half;  // padForward_k row map
-          buf[PL::digitrev(j) * TL + tl] = w;
+          buf[LY::pos(PL::digitrev(j) * TL + tl)] = w;
         }
       }
       __syncthreads();
-      fft_run<T, TL, NTHR, +1>(PL{}, buf, tw, tid);
+      fft_run<T, TL, NTHR, +1, LY>(PL{}, buf, tw, tid);
       cpx<T>* dst = W + (kind * 3 + c) * ws + (size_t)i * NTP * nz;
       for (int e = tid; e < NTP * TL; e += NTHR) {
         const int j = e / TL, tl = e - j * TL, l = l0 + tl;
-        if (l < nz) dst[(size_t)j * nz + l] = buf[e];
+        if (l < nz) dst[(size_t)j * nz + l] = buf[LY::pos(e)];
       }
       __syncthreads();
     }
@@ -108,7 +103,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
 }
 
 // ------------------------------------------------------------------ z pass
-template <typename T, int NTHR, int MPT, typename PL>
+template <typename T, int NTHR, int MPT, typename LY, typename PL>
 __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                  const cpx<T>* __restrict__ twz,
                                                  const double* __restrict__ r,
@@ -116,7 +111,7 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
                                                  cpx<T>* __restrict__ Y) {
   constexpr int NH = PL::L;
   constexpr int NZF = 2 * NH;
-  __shared__ cpx<T> buf[NH];
+  __shared__ cpx<T> buf[LY::size(NH)];
   const int row = blockIdx.x;
   const int i = row / g.ntp;
   const int tid = threadIdx.x;
@@ -140,13 +135,13 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
       const cpx<T> E{a.x + b.x, a.y - b.y};
       const cpx<T> dl{a.x - b.x, a.y + b.y};
       const cpx<T> O = cx_mulc(dl, twz[k]);
-      buf[PL::digitrev(k)] = cpx<T>{E.x - O.y, E.y + O.x};
+      buf[LY::pos(PL::digitrev(k))] = cpx<T>{E.x - O.y, E.y + O.x};
     }
     __syncthreads();
-    fft_run<T, 1, NTHR, +1>(PL{}, buf, tw, tid);
+    fft_run<T, 1, NTHR, +1, LY>(PL{}, buf, tw, tid);
   };
   auto sample = [&](int n) -> T {
-    const cpx<T> v = buf[n >> 1];
+    const cpx<T> v = buf[LY::pos(n >> 1)];
     return (n & 1) ? v.y : v.x;
   };
 
@@ -185,17 +180,17 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
         T v = acc[q];
         if (c == 0) v = v - ut[q] * (ut[q] / rad);
         else if (c == 1) v = v + ur[q] * (ut[q] / rad);
-        T* slot = reinterpret_cast<T*>(&buf[PL::digitrev(n >> 1)]) + (n & 1);
+        T* slot = reinterpret_cast<T*>(&buf[LY::pos(PL::digitrev(n >> 1))]) + (n & 1);
         *slot = -v;
       }
     }
     __syncthreads();
-    fft_run<T, 1, NTHR, -1>(PL{}, buf, tw, tid);
+    fft_run<T, 1, NTHR, -1, LY>(PL{}, buf, tw, tid);
     cpx<T>* dst = Y + c * ws + roff;
     for (int k = tid; k < nz; k += NTHR) {
       cpx<T> X{T(0), T(0)};
       if (k < nz - 1) {
-        const cpx<T> zk = buf[k], zc = buf[k == 0 ? 0 : NH - k];
+        const cpx<T> zk = buf[LY::pos(k)], zc = buf[LY::pos(k == 0 ? 0 : NH - k)];
         const cpx<T> E{T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y - zc.y)};
         const cpx<T> O{T(0.5) * (zk.y + zc.y), T(-0.5) * (zk.x - zc.x)};
         const cpx<T> wo = cx_mul(twz[k], O);
@@ -207,17 +202,178 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
   }
 }
 
+// ------------------------------------------------------------------ z pass, register form
+// Same transforms, products and arithmetic as zpass_ct, arranged so that no
+// data passes through LDS in digit-reversed order:
+//  * thread n0 < NH/R0 owns the complex samples k = n0 + (NH/R0) q, q < R0
+//    (real samples 2k, 2k+1) of every transform of the row;
+//  * the half-length C2R pre-processing of those k reads X[k] and X[NH-k]
+//    straight from global memory (coalesced) and feeds the first radix-R0
+//    pass in registers (first_pass_store); later passes run through LDS;
+//  * the transform output is read back at the owned k, so the convective
+//    products stay in registers, and they are exactly the first-pass inputs
+//    of the forward transform of the row;
+//  * with PF the 2 R0 inputs of the next transform are loaded before the
+//    current one's LDS passes start, overlapping HBM latency with them.
+// LDS rows of NH/R_last elements padded by PAD slots (bank spreading).
+template <typename T, int NTHR, int PF, int PAD, typename PL>
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void zpass2_ct(Geom g, const cpx<T>* __restrict__ tw,
+                                                  const cpx<T>* __restrict__ twz,
+                                                  const double* __restrict__ r,
+                                                  const cpx<T>* __restrict__ W, size_t ws,
+                                                  cpx<T>* __restrict__ Y) {
+  constexpr int NH = PL::L;
+  constexpr int R0 = PL::rad[0];
+  constexpr int NB1 = NH / R0;
+  static_assert(NB1 <= NTHR, "one first-pass butterfly per thread");
+  using LY = Lay<NH / PL::rad[PL::P - 1], PAD>;
+  __shared__ cpx<T> buf[LY::size(NH)];
+  const int row = blockIdx.x;
+  const int i = row / g.ntp;
+  const int tid = threadIdx.x;
+  const bool own = tid < NB1;
+  const int nz = g.nz;
+  const T rad = T(r[i]);
+  const size_t roff = (size_t)row * nz;
+  const T kz0 = T(g.kz0);
+  const cpx<T> zero{T(0), T(0)};
+
+  // source field of inverse transform t: u_r, u_t, u_z, then for each
+  // component c: d_r u_c, d_theta u_c / r, u_c (differentiated in z)
+  auto src = [&](int t) -> const cpx<T>* {
+    const int f = t < 3 ? t : ((t - 3) % 3 == 0 ? 3 + (t - 3) / 3 : ((t - 3) % 3 == 1 ? 6 + (t - 3) / 3 : (t - 3) / 3));
+    return W + f * ws + roff;
+  };
+  auto load = [&](const cpx<T>* X, cpx<T>* A, cpx<T>* B) {
+#pragma unroll
+    for (int q = 0; q < R0; q++) {
+      const int k = tid + NB1 * q, kc = NH - k;
+      // branches, not "c ? X[k] : zero": a conditional of two lvalues
+      // becomes a load through a selected generic pointer
+      A[q] = zero;
+      B[q] = zero;
+      if (own && k < nz) A[q] = X[k];
+      if (own && kc < nz) B[q] = X[kc];
+    }
+  };
+
+  cpx<T> ur[R0], ut[R0], uz[R0], acc[R0];
+  cpx<T> A[R0], B[R0];
+  load(src(0), A, B);
+#pragma unroll 1
+  for (int t = 0; t < 12; t++) {
+    // an opaque copy of the thread index: keeps the compiler from hoisting
+    // every pass's LDS index arithmetic out of the loop into live registers
+    int lt = tid;
+    asm volatile("" : "+v"(lt));
+    cpx<T> nA[R0], nB[R0];
+    if (PF && t + 1 < 12) load(src(t + 1), nA, nB);
+    const bool dz = t >= 3 && (t - 3) % 3 == 2;
+    if (own) {
+      cpx<T> v[R0];
+#pragma unroll
+      for (int q = 0; q < R0; q++) {  // half-length C2R pre-processing
+        const int k = lt + NB1 * q, kc = NH - k;
+        cpx<T> a = A[q], b = B[q];
+        if (k == 0) { a.y = T(0); b.y = T(0); }
+        if (dz) {  // derivz_k (mod_deriv.cu:24-35)
+          const T ka = kz0 * T(k), kb = kz0 * T(kc);
+          a = cpx<T>{-ka * a.y, ka * a.x};
+          b = cpx<T>{-kb * b.y, kb * b.x};
+        }
+        const cpx<T> E{a.x + b.x, a.y - b.y};
+        const cpx<T> dl{a.x - b.x, a.y + b.y};
+        const cpx<T> O = cx_mulc(dl, twz[k]);
+        v[q] = cpx<T>{E.x - O.y, E.y + O.x};
+      }
+      first_pass_store<T, +1, LY, 1>(PL{}, v, buf, lt);
+    }
+    __syncthreads();
+    fft_rest<T, NTHR, +1, LY>(PL{}, buf, tw, lt);
+    cpx<T> sm[R0];
+#pragma unroll
+    for (int q = 0; q < R0; q++) {
+      sm[q] = zero;
+      if (own) sm[q] = buf[LY::pos(lt + NB1 * q)];
+    }
+    __syncthreads();
+    if (t == 0) {
+#pragma unroll
+      for (int q = 0; q < R0; q++) ur[q] = sm[q];
+    } else if (t == 1) {
+#pragma unroll
+      for (int q = 0; q < R0; q++) ut[q] = sm[q];
+    } else if (t == 2) {
+#pragma unroll
+      for (int q = 0; q < R0; q++) uz[q] = sm[q];
+    } else {
+      // calcNonlinear_k (mod_io_nonlinear.cu:142-170), summed in its order
+      const int kk = (t - 3) % 3, c = (t - 3) / 3;
+      if (kk == 0) {
+#pragma unroll
+        for (int q = 0; q < R0; q++) acc[q] = cpx<T>{ur[q].x * sm[q].x, ur[q].y * sm[q].y};
+      } else if (kk == 1) {
+#pragma unroll
+        for (int q = 0; q < R0; q++) {
+          acc[q].x += ut[q].x * sm[q].x;
+          acc[q].y += ut[q].y * sm[q].y;
+        }
+      } else {
+        if (own) {
+          cpx<T> v[R0];
+#pragma unroll
+          for (int q = 0; q < R0; q++) {
+            T vx = acc[q].x + uz[q].x * sm[q].x, vy = acc[q].y + uz[q].y * sm[q].y;
+            if (c == 0) {
+              vx = vx - ut[q].x * (ut[q].x / rad);
+              vy = vy - ut[q].y * (ut[q].y / rad);
+            } else if (c == 1) {
+              vx = vx + ur[q].x * (ut[q].x / rad);
+              vy = vy + ur[q].y * (ut[q].y / rad);
+            }
+            v[q] = cpx<T>{-vx, -vy};
+          }
+          first_pass_store<T, -1, LY, 1>(PL{}, v, buf, lt);
+        }
+        __syncthreads();
+        fft_rest<T, NTHR, -1, LY>(PL{}, buf, tw, lt);
+        cpx<T>* dst = Y + c * ws + roff;
+        for (int k = lt; k < nz; k += NTHR) {  // half-length R2C post-processing
+          cpx<T> X = zero;
+          if (k < nz - 1) {
+            const cpx<T> zk = buf[LY::pos(k)], zc = buf[LY::pos(k == 0 ? 0 : NH - k)];
+            const cpx<T> E{T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y - zc.y)};
+            const cpx<T> O{T(0.5) * (zk.y + zc.y), T(-0.5) * (zk.x - zc.x)};
+            const cpx<T> wo = cx_mul(twz[k], O);
+            X = cpx<T>{E.x + wo.x, E.y + wo.y};
+          }
+          dst[k] = X;
+        }
+        __syncthreads();
+      }
+    }
+    if (t + 1 < 12) {
+      if (PF) {
+#pragma unroll
+        for (int q = 0; q < R0; q++) { A[q] = nA[q]; B[q] = nB[q]; }
+      } else {
+        load(src(t + 1), A, B);
+      }
+    }
+  }
+}
+
 // ------------------------------------------------------------------ theta forward
-template <typename T, int TL, int NTHR, typename PL>
+template <typename T, int TL, int NTHR, typename LY, typename PL>
 __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                      const cpx<T>* __restrict__ Y, size_t ws,
                                                      const double* __restrict__ D,
                                                      const double2* __restrict__ u,
-                                                     double2* __restrict__ N) {
+                                                     double2* __restrict__ N, int ord) {
   constexpr int NTP = PL::L;
-  __shared__ cpx<T> buf[NTP * TL];
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
-  const int i = lb % g.nr, l0 = (lb / g.nr) * TL;
+  __shared__ cpx<T> buf[LY::size(NTP * TL)];
+  int i, l0;
+  theta_block(g, TL, ord, i, l0);
   const int tid = threadIdx.x;
   const int nt = g.nt, half = nt / 2, nz = g.nz;
   const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
@@ -227,15 +383,15 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
       const int j = e / TL, tl = e - j * TL, l = l0 + tl;
       cpx<T> v{T(0), T(0)};
       if (l < nz - 1) v = src[(size_t)j * nz + l];
-      buf[PL::digitrev(j) * TL + tl] = v;
+      buf[LY::pos(PL::digitrev(j) * TL + tl)] = v;
     }
     __syncthreads();
-    fft_run<T, TL, NTHR, -1>(PL{}, buf, tw, tid);
+    fft_run<T, TL, NTHR, -1, LY>(PL{}, buf, tw, tid);
     for (int e = tid; e < nt * TL; e += NTHR) {
       const int it = e / TL, tl = e - it * TL, l = l0 + tl;
       if (l < nz) {
         const int j = it < half ? it : it + half;  // padBackward_k row map
-        const cpx<T> w = buf[j * TL + tl];
+        const cpx<T> w = buf[LY::pos(j * TL + tl)];
         double vx = double(w.x) / norm, vy = double(w.y) / norm;
         if (it == half || l == nz - 1) vx = vy = 0.0;
         if (c == 2 && it == 0 && l == 0) {  // pressGrad_k (mod_radialFD.cu:100-111)
@@ -247,6 +403,140 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
         N[c * g.n + ((size_t)i * nt + it) * nz + l] = make_double2(vx, vy);
       }
     }
+    __syncthreads();
+  }
+}
+
+// ------------------------------------------------------------------ theta, register forms
+// Same transforms and arithmetic as theta_inv_ct / theta_fwd_ct with the
+// first pass fed from global memory and the last pass written to global
+// memory from registers: per transform two LDS round trips instead of four,
+// and no digit-reversed scatter.  Work item p = (butterfly n0, column):
+// column = p % TL (lanes of a wave cover consecutive l), n0 = p / TL; it
+// owns the rows j = n0 + (NTP/R0) q, q < R0.
+//
+// padForward_k row map: row j < NT/2 holds it = j, rows NT/2 <= j < NT are
+// zero, row j >= NT holds it = j - NT/2 (NTP - NT = NT/2).
+template <typename T, int TL, int NTHR, typename LY, typename PL>
+__global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __restrict__ tw,
+                                                      const double* __restrict__ r,
+                                                      const double2* __restrict__ u,
+                                                      const double2* __restrict__ dR,
+                                                      cpx<T>* __restrict__ W, size_t ws, int ord) {
+  constexpr int NTP = PL::L;
+  constexpr int NT = 2 * NTP / 3;
+  constexpr int half = NT / 2;
+  constexpr int R0 = PL::rad[0];
+  constexpr int NB1 = NTP / R0;
+  constexpr int PAIRS = NB1 * TL;
+  constexpr int PPT = (PAIRS + NTHR - 1) / NTHR;
+  __shared__ cpx<T> buf[LY::size(NTP * TL)];
+  const int nz = g.nz;
+  int i, l0;
+  theta_block(g, TL, ord, i, l0);
+  const int tid = threadIdx.x;
+  const double ri = r[i];
+#pragma unroll 1
+  for (int f = 0; f < 9; f++) {
+    const int c = f % 3, kind = f / 3;  // kind 0: u, 1: d_r u, 2: d_theta u / r
+    // opaque copy of the thread index: stops the index arithmetic of all nine
+    // transforms from being hoisted into live registers
+    int lt = tid;
+    asm volatile("" : "+v"(lt));
+    {
+      const double2* src = (kind == 1 ? dR : u) + c * g.n + (size_t)i * g.modes;
+#pragma unroll
+      for (int s = 0; s < PPT; s++) {
+        const int p = lt + s * NTHR;
+        if ((PAIRS % NTHR) == 0 || p < PAIRS) {
+          const int col = p % TL, n0 = p / TL, l = l0 + col;
+          cpx<T> v[R0];
+#pragma unroll
+          for (int q = 0; q < R0; q++) {
+            const int j = n0 + NB1 * q;
+            const int it = j < half ? j : (j >= NT ? j - half : -1);
+            v[q] = cpx<T>{T(0), T(0)};
+            if (it >= 0 && l < nz) {
+              const double2 a = src[(size_t)it * nz + l];
+              if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
+                const double kt = double(it < half ? it : it - NT);
+                v[q] = cpx<T>{T(-kt * a.y / ri), T(kt * a.x / ri)};
+              } else {
+                v[q] = cpx<T>{T(a.x), T(a.y)};
+              }
+            }
+          }
+          first_pass_store<T, +1, LY, TL>(PL{}, v, buf, n0, col);
+        }
+      }
+      __syncthreads();
+      cpx<T>* dst = W + (kind * 3 + c) * ws + (size_t)i * NTP * nz + l0;
+      fft_rest_emit<T, TL, NTHR, +1, LY>(PL{}, buf, tw, lt, [&](int col, int j, cpx<T> w) {
+        if (l0 + col < nz) dst[(size_t)j * nz + col] = w;
+      });
+      __syncthreads();
+    }
+  }
+}
+
+// padBackward_k: output row j < NT/2 is it = j, row j >= NT is it = j - NT/2,
+// rows NT/2 <= j < NT are dropped; it = NT/2 (m = -NT/2) and l = NZ-1 are
+// zeroed.
+template <typename T, int TL, int NTHR, typename LY, typename PL>
+__global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __restrict__ tw,
+                                                      const cpx<T>* __restrict__ Y, size_t ws,
+                                                      const double* __restrict__ D,
+                                                      const double2* __restrict__ u,
+                                                      double2* __restrict__ N, int ord) {
+  constexpr int NTP = PL::L;
+  constexpr int NT = 2 * NTP / 3;
+  constexpr int half = NT / 2;
+  constexpr int R0 = PL::rad[0];
+  constexpr int NB1 = NTP / R0;
+  constexpr int PAIRS = NB1 * TL;
+  constexpr int PPT = (PAIRS + NTHR - 1) / NTHR;
+  __shared__ cpx<T> buf[LY::size(NTP * TL)];
+  int i, l0;
+  theta_block(g, TL, ord, i, l0);
+  const int tid = threadIdx.x;
+  const int nz = g.nz;
+  const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
+#pragma unroll 1
+  for (int c = 0; c < 3; c++) {
+    int lt = tid;
+    asm volatile("" : "+v"(lt));
+    const cpx<T>* src = Y + c * ws + (size_t)i * NTP * nz;
+#pragma unroll
+    for (int s = 0; s < PPT; s++) {
+      const int p = lt + s * NTHR;
+      if ((PAIRS % NTHR) == 0 || p < PAIRS) {
+        const int col = p % TL, n0 = p / TL, l = l0 + col;
+        cpx<T> v[R0];
+#pragma unroll
+        for (int q = 0; q < R0; q++) {
+          v[q] = cpx<T>{T(0), T(0)};
+          if (l < nz - 1) v[q] = src[(size_t)(n0 + NB1 * q) * nz + l];
+        }
+        first_pass_store<T, -1, LY, TL>(PL{}, v, buf, n0, col);
+      }
+    }
+    __syncthreads();
+    double2* dst = N + c * g.n + (size_t)i * g.modes;
+    fft_rest_emit<T, TL, NTHR, -1, LY>(PL{}, buf, tw, lt, [&](int col, int j, cpx<T> w) {
+      const int l = l0 + col;
+      if (l < nz && (j < half || j >= NT)) {
+        const int it = j < half ? j : j - half;
+        double vx = double(w.x) / norm, vy = double(w.y) / norm;
+        if (it == half || l == nz - 1) vx = vy = 0.0;
+        if (c == 2 && it == 0 && l == 0) {  // pressGrad_k (mod_radialFD.cu:100-111)
+          double du = 0.0;
+          for (int j2 = 0; j2 < IW + 1; j2++)
+            du += D[STEN * (g.nr - 1) + j2 + IW] * u[2 * g.n + (size_t)(g.nr - 1 - IW + j2) * g.modes].x;
+          vx -= (2.0 / g.re) * du;
+        }
+        dst[(size_t)it * nz + l] = make_double2(vx, vy);
+      }
+    });
     __syncthreads();
   }
 }
@@ -269,62 +559,115 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
   X(2304, Plan<9, 16, 16>)
 
 // ------------------------------------------------------------------ stage launchers
+// Variant dispatch for one compiled plan PL.  The LDS pad shift is 4 (one
+// slot per 16 elements) for tile width 4 and 3 (one per 8) for width 8.
+template <typename T, typename PL>
+bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
+                    const double2* dR, cpx<T>* W, size_t ws, hipStream_t s) {
+  const int tl = v.tl;
+  if (PL::L * tl * (int)sizeof(cpx<T>) > 163840) return false;
+  const int nb = ((g.nz + tl - 1) / tl) * g.nr;
+  if (v.alg == 1) {
+    if (tl == 4 && v.nthr == 192) theta_inv2_ct<T, 4, 192, Dense, PL><<<nb, 192, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+    else if (tl == 8 && v.nthr == 256) theta_inv2_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+    else if (tl == 8 && v.nthr == 384) theta_inv2_ct<T, 8, 384, Dense, PL><<<nb, 384, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+    else return false;
+    return true;
+  }
+  if (v.alg != 0) return false;
+  if (tl == 4 && !v.pad) theta_inv_ct<T, 4, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  else if (tl == 4) theta_inv_ct<T, 4, 256, Lay<16, 1>, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  else if (tl == 8 && !v.pad) theta_inv_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  else if (tl == 8) theta_inv_ct<T, 8, 256, Lay<8, 1>, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  else return false;
+  return true;
+}
+
+template <typename T, typename PL>
+bool theta_fwd_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const cpx<T>* Y, size_t ws, const double* D,
+                    const double2* u, double2* N, hipStream_t s) {
+  const int tl = v.tl;
+  if (PL::L * tl * (int)sizeof(cpx<T>) > 163840) return false;
+  const int nb = ((g.nz + tl - 1) / tl) * g.nr;
+  if (v.alg == 1) {
+    if (tl == 4 && v.nthr == 192) theta_fwd2_ct<T, 4, 192, Dense, PL><<<nb, 192, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+    else if (tl == 8 && v.nthr == 256) theta_fwd2_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+    else if (tl == 8 && v.nthr == 384) theta_fwd2_ct<T, 8, 384, Dense, PL><<<nb, 384, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+    else return false;
+    return true;
+  }
+  if (v.alg != 0) return false;
+  if (tl == 4 && !v.pad) theta_fwd_ct<T, 4, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  else if (tl == 4) theta_fwd_ct<T, 4, 256, Lay<16, 1>, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  else if (tl == 8 && !v.pad) theta_fwd_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  else if (tl == 8) theta_fwd_ct<T, 8, 256, Lay<8, 1>, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  else return false;
+  return true;
+}
+
+// z-pass row padding of the register form: R0 slots per row for the
+// <R0, 16, 16> plans (bank model of the pass patterns), none otherwise.
+template <typename PL>
+constexpr int zpass2_pad() {
+  return (PL::P == 3 && PL::rad[1] == 16 && PL::rad[2] == 16) ? PL::rad[0] : 0;
+}
+
+template <typename T, typename PL>
+bool zpass_plan(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* twz, const double* r,
+                const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s) {
+  constexpr int LL = PL::L;
+  const int nb = g.nr * g.ntp;
+  if (v.z_alg == 1) {  // register form, 256 threads
+    constexpr int PAD = zpass2_pad<PL>();
+    if (v.z_pf)
+      zpass2_ct<T, 256, 1, PAD, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
+    else
+      zpass2_ct<T, 256, 0, PAD, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
+    return true;
+  }
+  if (v.z_alg != 0) return false;
+  if (v.z_nthr == 256 && !v.z_pad)
+    zpass_ct<T, 256, (2 * LL + 255) / 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
+  else if (v.z_nthr == 256)
+    zpass_ct<T, 256, (2 * LL + 255) / 256, Lay<16, 1>, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
+  else if (v.z_nthr == 512 && !v.z_pad)
+    zpass_ct<T, 512, (2 * LL + 511) / 512, Dense, PL><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y);
+  else if (v.z_nthr == 512)
+    zpass_ct<T, 512, (2 * LL + 511) / 512, Lay<16, 1>, PL><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y);
+  else
+    return false;
+  return true;
+}
+
 // Return false when the requested variant is not compiled for this size.
 template <typename T>
-bool ct_theta_inv(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double* Dc,
-                  const double2* u, const double2* dR, cpx<T>* W, size_t ws, hipStream_t s) {
-  bool ok = false;
-#define X(LL, ...)                                                                                      \
-  if (g.ntp == LL) {                                                                                    \
-    const int tl = v.tl_inv;                                                                            \
-    if (LL * tl * (int)sizeof(cpx<T>) <= 163840) {                                                      \
-      const int nb = ((g.nz + tl - 1) / tl) * g.nr;                                                     \
-      if (tl == 4 && v.fused) { theta_inv_ct<T, 4, 256, 1, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, r, Dc, u, dR, W, ws); ok = true; } \
-      if (tl == 4 && !v.fused) { theta_inv_ct<T, 4, 256, 0, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, r, Dc, u, dR, W, ws); ok = true; } \
-      if (tl == 8 && v.fused) { theta_inv_ct<T, 8, 256, 1, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, r, Dc, u, dR, W, ws); ok = true; } \
-      if (tl == 8 && !v.fused) { theta_inv_ct<T, 8, 256, 0, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, r, Dc, u, dR, W, ws); ok = true; } \
-    }                                                                                                   \
-  }
+bool ct_theta_inv(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
+                  const double2* dR, cpx<T>* W, size_t ws, hipStream_t s) {
+#define X(LL, ...) \
+  if (g.ntp == LL) return theta_inv_plan<T, __VA_ARGS__>(v.inv, g, tw, r, u, dR, W, ws, s);
   NSP_THETA_PLANS(X)
 #undef X
-  return ok;
+  return false;
 }
 
 template <typename T>
 bool ct_theta_fwd(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* Y, size_t ws, const double* D,
                   const double2* u, double2* N, hipStream_t s) {
-  bool ok = false;
-#define X(LL, ...)                                                                                      \
-  if (g.ntp == LL) {                                                                                    \
-    const int tl = v.tl_fwd;                                                                            \
-    if (LL * tl * (int)sizeof(cpx<T>) <= 163840) {                                                      \
-      const int nb = ((g.nz + tl - 1) / tl) * g.nr;                                                     \
-      if (tl == 4) { theta_fwd_ct<T, 4, 256, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N); ok = true; } \
-      if (tl == 8) { theta_fwd_ct<T, 8, 256, __VA_ARGS__><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N); ok = true; } \
-    }                                                                                                   \
-  }
+#define X(LL, ...) \
+  if (g.ntp == LL) return theta_fwd_plan<T, __VA_ARGS__>(v.fwd, g, tw, Y, ws, D, u, N, s);
   NSP_THETA_PLANS(X)
 #undef X
-  return ok;
+  return false;
 }
 
 template <typename T>
 bool ct_zpass(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* twz, const double* r,
               const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s) {
-  bool ok = false;
-#define X(LL, ...)                                                                                      \
-  if (g.nh == LL) {                                                                                     \
-    if (v.z_nthr == 256) {                                                                              \
-      zpass_ct<T, 256, (2 * LL + 255) / 256, __VA_ARGS__><<<g.nr * g.ntp, 256, 0, s>>>(g, tw, twz, r, W, ws, Y); \
-      ok = true;                                                                                        \
-    } else if (v.z_nthr == 512) {                                                                       \
-      zpass_ct<T, 512, (2 * LL + 511) / 512, __VA_ARGS__><<<g.nr * g.ntp, 512, 0, s>>>(g, tw, twz, r, W, ws, Y); \
-      ok = true;                                                                                        \
-    }                                                                                                   \
-  }
+#define X(LL, ...) \
+  if (g.nh == LL) return zpass_plan<T, __VA_ARGS__>(v, g, tw, twz, r, W, ws, Y, s);
   NSP_ZHALF_PLANS(X)
 #undef X
-  return ok;
+  return false;
 }
 
 }  // namespace nsp

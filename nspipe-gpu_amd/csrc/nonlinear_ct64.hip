@@ -3,8 +3,8 @@
 #include "nonlinear_ct.h"
 
 namespace nsp {
-template bool ct_theta_inv<double>(const NlVariant&, const Geom&, const cpx<double>*, const double*, const double*,
-                                  const double2*, const double2*, cpx<double>*, size_t, hipStream_t);
+template bool ct_theta_inv<double>(const NlVariant&, const Geom&, const cpx<double>*, const double*, const double2*,
+                                  const double2*, cpx<double>*, size_t, hipStream_t);
 template bool ct_theta_fwd<double>(const NlVariant&, const Geom&, const cpx<double>*, const cpx<double>*, size_t,
                                   const double*, const double2*, double2*, hipStream_t);
 template bool ct_zpass<double>(const NlVariant&, const Geom&, const cpx<double>*, const cpx<double>*, const double*,

@@ -33,7 +33,8 @@ struct FftPlan {
 };
 bool make_fft_plan(int L, FftPlan& p, std::vector<int>& perm);
 
-template <typename T> struct cpx { T x, y; };
+// complex pair aligned to its size, so loads and stores are single vector ops
+template <typename T> struct alignas(2 * sizeof(T)) cpx { T x, y; };
 
 // Geometry and constants shared by all kernels of one pipe.
 struct Geom {
@@ -108,11 +109,20 @@ void launch_sweep2(const Lin2Args& a, hipStream_t s);
 std::vector<double> centred_table(int nr, const std::vector<double>& T);
 
 // Kernel variant of the compiled-plan path (chosen by the autotuner).
+// Kernel variant of one theta stage: alg 0 = LDS-scatter form (tl 4 or 8
+// columns, 256 threads, pad = one LDS slot per 16 or 8 elements), alg 1 =
+// register form ((tl, nthr) = (4, 192), (8, 256) or (8, 384)); ord = block
+// order (0: radial level fastest, 1: column tile fastest).
+struct ThetaVar {
+  int alg, tl, nthr, pad, ord;
+};
+// Kernel variant of the compiled-plan path (chosen by the autotuner).
 struct NlVariant {
-  int fused;   // 1: radial derivative inside the theta-inverse kernel
-  int tl_inv;  // l-columns per theta-inverse block (4 or 8)
-  int tl_fwd;  // l-columns per theta-forward block (4 or 8)
-  int z_nthr;  // threads per z-pass row (256 or 512)
+  ThetaVar inv, fwd;
+  int z_alg;   // z pass: 0 LDS-scatter form, 1 register form (256 threads)
+  int z_nthr;  // scatter form: threads per row (256 or 512)
+  int z_pad;   // scatter form: one LDS slot per 16 elements
+  int z_pf;    // register form: prefetch the next transform's inputs
 };
 // Nonlinear-phase buffers and FFT data.
 struct NlBufs {
@@ -130,7 +140,6 @@ struct NlBufs {
   bool use_ct;
   double2* ct_tw_t;    // [NTP-1]
   double2* ct_tw_h;    // [NH-1]
-  double* Dc;          // centred D table (fused radial derivative)
   // mixed form: FP32 transforms and products
   bool mixed;
   float2* W32;         // [9][NR*NTP*NZ]
@@ -198,8 +207,8 @@ std::vector<int> ct_radices(int L, bool theta);
 // entries exp(-2 pi i q k / (S R)) at [k*(R-1) + q-1]
 std::vector<double2> pass_twiddles(const std::vector<int>& radices);
 template <typename T>
-bool ct_theta_inv(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double* Dc,
-                  const double2* u, const double2* dR, cpx<T>* W, size_t ws, hipStream_t s);
+bool ct_theta_inv(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
+                  const double2* dR, cpx<T>* W, size_t ws, hipStream_t s);
 template <typename T>
 bool ct_theta_fwd(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* Y, size_t ws, const double* D,
                   const double2* u, double2* N, hipStream_t s);

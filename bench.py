@@ -57,12 +57,15 @@ def step_bytes(cfg, fp64, n_it):
     return (1 + n_it) * b_nl + (1 + n_it) * b_sweep
 
 
-def run_gpu(nsp, precision, steps, warmup, cfg, tune=True):
+def run_gpu(nsp, precision, steps, warmup, cfg, chosen=None):
     p = nsp.Pipe(**cfg, precision=precision)
     p.seeded_ic(SEED3, 4, 4, 0, RMS3)
-    tuning = p.autotune(reps=2) if tune else None
-    if tuning:
-        print("autotune %s: %s" % (precision, json.dumps(tuning)), file=sys.stderr, flush=True)
+    if chosen is not None:
+        p.set_tuning(chosen)
+        tuning = dict(tuned="set", chosen=chosen)
+    else:
+        tuning = p.autotune(reps=2)
+    print("autotune %s: %s" % (precision, json.dumps(tuning)), file=sys.stderr, flush=True)
     if warmup:
         p.step(warmup)
     p.profile_enable(True)
@@ -89,11 +92,13 @@ def roofline(res, cfg, fp64, traffic):
     alg = algorithmic_bytes(name, cfg, fp64)
     achieved = alg / avg_s / 1e9
     tr = None
-    if traffic and name in traffic:
-        tr = traffic[name]
+    form = "fp64" if fp64 else "mixed"
+    if traffic and form in traffic and name in traffic[form].get("kernels", {}):
+        tr = round(traffic[form]["kernels"][name]["traffic"])
     return dict(bound="hbm", kernel=name, achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=tr, algorithmic_bytes=alg,
-                avg_launch_ms=round(avg_s * 1e3, 4), launches=cnt)
+                avg_launch_ms=round(avg_s * 1e3, 4), launches=cnt,
+                traffic_gbs=None if tr is None else round(tr / avg_s / 1e9, 1))
 
 
 def load_traffic():
@@ -133,6 +138,27 @@ def cpu_baseline(cfg, n_it_gpu):
                 sample_seconds=round(t, 2))
 
 
+def copy_calibration(nbytes, copies=3):
+    """Plain device-to-device copies of known size (hipMemcpy's blit kernel),
+    the reference point for scaling the FETCH_SIZE / WRITE_SIZE counters of a
+    profiled run.  Runs before the library launches anything."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    for ptr in (src, dst):
+        if hip.hipMalloc(ctypes.byref(ptr), ctypes.c_size_t(nbytes)) != 0:
+            raise RuntimeError("hipMalloc failed for the copy calibration")
+    hip.hipMemset(src, 1, ctypes.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    for _ in range(copies):
+        if hip.hipMemcpy(dst, src, ctypes.c_size_t(nbytes), 3) != 0:  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed in the copy calibration")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    print(f"copy calibration: {copies} x {nbytes} bytes", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,22 +166,36 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="first run 3 device-to-device copies of BYTES (counter calibration)")
+    ap.add_argument("--tuning", metavar="FILE", help="apply saved kernel variants instead of autotuning")
+    ap.add_argument("--save-tuning", metavar="FILE", help="write the chosen kernel variants")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1",
                           "n_gpus": args.gpus}))
         sys.exit(2)
     import nsp
+    nsp.lib()
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
     cfg = CONFIG3
     traffic = load_traffic()
     forms = {"fp64": nsp.FP64, "mixed": nsp.MIXED}
     want = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
+    saved = {}
+    if args.tuning:
+        with open(args.tuning) as fh:
+            saved = json.load(fh)
     results = {}
     for f in want:
         try:
-            results[f] = run_gpu(nsp, forms[f], args.steps, args.warmup, cfg)
+            results[f] = run_gpu(nsp, forms[f], args.steps, args.warmup, cfg, saved.get(f))
         except nsp.NspError as e:
             results[f] = dict(error=str(e))
+    if args.save_tuning:
+        with open(args.save_tuning, "w") as fh:
+            json.dump({f: results[f]["tuning"]["chosen"] for f in want if "tuning" in results[f]}, fh, indent=1)
     head = "mixed" if "steps_per_s" in results.get("mixed", {}) else "fp64"
     r = results[head]
     fp64 = head == "fp64"

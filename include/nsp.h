@@ -157,6 +157,13 @@ nsp_status nsp_profile_read(nsp_handle* h, double* total_ms, long long* launches
 nsp_status nsp_autotune(nsp_handle* h, int reps);
 const char* nsp_tuning(nsp_handle* h);
 
+/* Restores a choice made by nsp_autotune (no reference counterpart): json is
+ * nsp_tuning's text, or just its "chosen" object.  Fails with
+ * NSP_ERR_INVALID, keeping the current variants, if a named variant is not
+ * compiled for this handle's sizes.  Runs one nonlinear evaluation of the
+ * current state as the check. */
+nsp_status nsp_set_tuning(nsp_handle* h, const char* json);
+
 /* HIP stream of the handle (a hipStream_t), for callers that order their own
  * work against it. */
 void* nsp_stream(nsp_handle* h);

@@ -166,6 +166,42 @@ double bits_to_double(unsigned long long b) {
 
 }  // namespace
 
+namespace {
+// The "chosen" object of the tuning report.
+std::string variant_json(const NlVariant& v) {
+  auto tvj = [](const ThetaVar& t) {
+    char o[112];
+    std::snprintf(o, sizeof o, "{\"alg\": %d, \"tl\": %d, \"nthr\": %d, \"pad\": %d, \"ord\": %d}", t.alg, t.tl,
+                  t.nthr, t.pad, t.ord);
+    return std::string(o);
+  };
+  char z[112];
+  std::snprintf(z, sizeof z, ", \"z_alg\": %d, \"z_nthr\": %d, \"z_pad\": %d, \"z_pf\": %d}", v.z_alg, v.z_nthr,
+                v.z_pad, v.z_pf);
+  return "{\"theta_inv\": " + tvj(v.inv) + ", \"theta_fwd\": " + tvj(v.fwd) + z;
+}
+
+// Integer value of "key": n in text, searched from `from` (-1 if absent).
+long json_int(const std::string& t, size_t from, const char* key, size_t* at = nullptr) {
+  const std::string k = std::string("\"") + key + "\"";
+  const size_t p = t.find(k, from);
+  if (p == std::string::npos) return -1;
+  size_t c = t.find(':', p + k.size());
+  if (c == std::string::npos) return -1;
+  c++;
+  while (c < t.size() && (t[c] == ' ' || t[c] == '\t')) c++;
+  if (c < t.size() && t[c] == '{') {  // object: caller reads its fields
+    if (at) *at = c;
+    return 0;
+  }
+  char* end = nullptr;
+  const long v = std::strtol(t.c_str() + c, &end, 10);
+  if (end == t.c_str() + c) return -1;
+  return v;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* nsp_last_error(void) { return g_err.c_str(); }
@@ -494,22 +530,68 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
   (void)hipEventDestroy(e1);
   HIPCHK(hipGetLastError());
   nb.var = best;
-  char buf[200];
-  auto tvj = [](const ThetaVar& t) {
-    char o[112];
-    std::snprintf(o, sizeof o, "{\"alg\": %d, \"tl\": %d, \"nthr\": %d, \"pad\": %d, \"ord\": %d}", t.alg, t.tl,
-                  t.nthr, t.pad, t.ord);
-    return std::string(o);
-  };
-  std::snprintf(buf, sizeof buf, ", \"z_alg\": %d, \"z_nthr\": %d, \"z_pad\": %d, \"z_pf\": %d}}", best.z_alg,
-                best.z_nthr, best.z_pad, best.z_pf);
-  rep += "}, \"chosen\": {\"theta_inv\": " + tvj(best.inv) + ", \"theta_fwd\": " + tvj(best.fwd);
-  rep += buf;
+  rep += "}, \"chosen\": " + variant_json(best) + "}";
   h->tune_report = rep;
   return NSP_OK;
 }
 
 const char* nsp_tuning(nsp_handle* h) { return h ? h->tune_report.c_str() : ""; }
+
+nsp_status nsp_set_tuning(nsp_handle* h, const char* json) {
+  if (!h || !json) return fail(NSP_ERR_INVALID, "null argument");
+  if (!h->nb.use_ct) return NSP_OK;  // generic-plan sizes have no variants
+  const std::string t(json);
+  size_t base = t.find("\"chosen\"");
+  if (base == std::string::npos) base = 0;
+  NlVariant v = h->nb.var;
+  for (int st = 0; st < 2; st++) {
+    size_t at = std::string::npos;
+    if (json_int(t, base, st == 0 ? "theta_inv" : "theta_fwd", &at) < 0 || at == std::string::npos)
+      return fail(NSP_ERR_INVALID, "tuning text lacks theta_inv / theta_fwd objects");
+    ThetaVar& tv = st == 0 ? v.inv : v.fwd;
+    const long a = json_int(t, at, "alg"), tl = json_int(t, at, "tl"), n = json_int(t, at, "nthr"),
+               pd = json_int(t, at, "pad"), o = json_int(t, at, "ord");
+    if (a < 0 || tl < 0 || n < 0 || pd < 0 || o < 0) return fail(NSP_ERR_INVALID, "incomplete theta variant");
+    tv = ThetaVar{int(a), int(tl), int(n), int(pd), int(o)};
+  }
+  const long za = json_int(t, base, "z_alg"), zn = json_int(t, base, "z_nthr"), zp = json_int(t, base, "z_pad"),
+             zf = json_int(t, base, "z_pf");
+  if (za < 0 || zn < 0 || zp < 0 || zf < 0) return fail(NSP_ERR_INVALID, "incomplete z-pass variant");
+  v.z_alg = int(za);
+  v.z_nthr = int(zn);
+  v.z_pad = int(zp);
+  v.z_pf = int(zf);
+  // accept only variants that are compiled for this size: one nonlinear
+  // evaluation of the current state into the scratch of the nonlinear term
+  const NlVariant old = h->nb.var;
+  h->nb.var = v;
+  HIPCHK(hipSetDevice(h->device));
+  bool ok = true;
+  {
+    NlBufs& nb = h->nb;
+    const Geom& g = h->g;
+    hipStream_t s = h->stream;
+    ok = nb.mixed ? ct_theta_inv<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, h->t.r, h->u, nb.dR,
+                                        (cpx<float>*)nb.W32, nb.wstride, s)
+                  : ct_theta_inv<double>(v, g, (const cpx<double>*)nb.ct_tw_t, h->t.r, h->u, nb.dR,
+                                         (cpx<double>*)nb.W, nb.wstride, s);
+    ok = ok && (nb.mixed ? ct_zpass<float>(v, g, (const cpx<float>*)nb.ct_tw_h32, (const cpx<float>*)nb.tw_z32,
+                                           h->t.r, (const cpx<float>*)nb.W32, nb.wstride, (cpx<float>*)nb.Y32, s)
+                         : ct_zpass<double>(v, g, (const cpx<double>*)nb.ct_tw_h, (const cpx<double>*)nb.tw_z,
+                                            h->t.r, (const cpx<double>*)nb.W, nb.wstride, (cpx<double>*)nb.Y, s));
+    ok = ok && (nb.mixed ? ct_theta_fwd<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, (const cpx<float>*)nb.Y32,
+                                               nb.wstride, h->t.D, h->u, h->Nl, s)
+                         : ct_theta_fwd<double>(v, g, (const cpx<double>*)nb.ct_tw_t, (const cpx<double>*)nb.Y,
+                                                nb.wstride, h->t.D, h->u, h->Nl, s));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (!ok) {
+    h->nb.var = old;
+    return fail(NSP_ERR_INVALID, "tuning names a kernel variant that is not compiled for this size");
+  }
+  h->tune_report = "{\"tuned\": \"set\", \"chosen\": " + variant_json(v) + "}";
+  return NSP_OK;
+}
 
 // integrate() loop body (mod_integrate.cu:112-131)
 nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {

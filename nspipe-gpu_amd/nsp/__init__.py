@@ -11,6 +11,7 @@ There is no fallback: if libnsp.so is missing or no HIP device is present the
 constructor raises.
 """
 import ctypes
+import json
 import os
 
 import numpy as np
@@ -39,7 +40,8 @@ EXPORTS = ["nsp_default_params", "nsp_create", "nsp_destroy", "nsp_set_state", "
            "nsp_reference_ic", "nsp_step", "nsp_nonlinear", "nsp_diagnostics", "nsp_tables",
            "nsp_radial_tables", "nsp_read_state_files", "nsp_write_state_files", "nsp_seeded_ic",
            "nsp_profile_enable", "nsp_profile_count", "nsp_profile_name", "nsp_profile_read",
-           "nsp_autotune", "nsp_tuning", "nsp_stream", "nsp_synchronize", "nsp_last_error"]
+           "nsp_autotune", "nsp_tuning", "nsp_set_tuning", "nsp_stream", "nsp_synchronize",
+           "nsp_last_error"]
 
 
 class NspError(RuntimeError):
@@ -82,6 +84,7 @@ def lib():
     L.nsp_autotune.argtypes = [vp, ip]
     L.nsp_tuning.restype = ctypes.c_char_p
     L.nsp_tuning.argtypes = [vp]
+    L.nsp_set_tuning.argtypes = [vp, ctypes.c_char_p]
     for name in EXPORTS:
         if name not in ("nsp_last_error", "nsp_stream", "nsp_profile_name", "nsp_tuning"):
             getattr(L, name).restype = ctypes.c_int
@@ -210,9 +213,12 @@ class Pipe:
 
     def autotune(self, reps=3):
         """Time the compiled kernel variants on this pipe and keep the fastest."""
-        import json
         _check(lib().nsp_autotune(self.h, reps))
         return json.loads(lib().nsp_tuning(self.h).decode())
+
+    def set_tuning(self, chosen):
+        """Restore an autotuner choice (the "chosen" dict of autotune())."""
+        _check(lib().nsp_set_tuning(self.h, json.dumps({"chosen": chosen}).encode()))
 
     def synchronize(self):
         """hipDeviceSynchronize on the pipe's device."""

@@ -176,6 +176,60 @@ def test_autotuned_variants_keep_parity():
     p.close()
 
 
+def _variants():
+    """Every compiled kernel form of each nonlinear stage, one stage at a time
+    (the others at their defaults)."""
+    th = []
+    for ord_ in (0, 1):
+        for pad in (0, 1):
+            for tl in (4, 8):
+                th.append(dict(alg=0, tl=tl, nthr=256, pad=pad, ord=ord_))
+        for tl, nthr in ((4, 192), (8, 256), (8, 384)):
+            th.append(dict(alg=1, tl=tl, nthr=nthr, pad=0, ord=ord_))
+    base = dict(theta_inv=dict(alg=0, tl=8, nthr=256, pad=0, ord=1),
+                theta_fwd=dict(alg=0, tl=8, nthr=256, pad=0, ord=1),
+                z_alg=1, z_nthr=256, z_pad=0, z_pf=0)
+    out = []
+    for t in th:
+        out.append(dict(base, theta_inv=t))
+        out.append(dict(base, theta_fwd=t))
+    for nthr in (256, 512):
+        for pad in (0, 1):
+            out.append(dict(base, z_alg=0, z_nthr=nthr, z_pad=pad))
+    for pf in (0, 1):
+        out.append(dict(base, z_alg=1, z_pf=pf))
+    return out
+
+
+@pytest.mark.parametrize("shape", COMPILED, ids=lambda s: f"{s['nr']}x{s['nt']}x{s['nz']}")
+def test_every_kernel_variant_keeps_parity(shape):
+    print(f"[gpu] all kernel variants {shape}", flush=True)
+    p = nsp.Pipe(**shape)
+    o = Oracle(**shape)
+    p.seeded_ic(SEED1, 6, 6, 1, 0.1)
+    u = p.get_state()
+    b = o.nonlinear(*u)
+    for v in _variants():
+        p.set_tuning(v)
+        a = p.nonlinear(*u)
+        for c in range(3):
+            e = rel_l2(a[c], b[c])
+            assert e <= 1e-12, (v, c, e)
+    p.close()
+
+
+def test_set_tuning_rejects_missing_variant():
+    p = nsp.Pipe(**COMPILED[0])
+    bad = dict(theta_inv=dict(alg=1, tl=16, nthr=256, pad=0, ord=0),
+               theta_fwd=dict(alg=0, tl=8, nthr=256, pad=0, ord=1), z_alg=1, z_nthr=256, z_pad=0, z_pf=0)
+    with pytest.raises(nsp.NspError):
+        p.set_tuning(bad)
+    p.seeded_ic(SEED1, 4, 4, 0, 0.01)
+    n_it, _, flags = p.step(1)  # previous variants still in place
+    assert flags[0] == 0
+    p.close()
+
+
 def test_mixed_nonlinear_accuracy():
     """FP32 transforms and products: relative L2 error of N(u) at FP32 level."""
     shape = COMPILED[3]

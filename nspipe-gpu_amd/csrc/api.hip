@@ -138,6 +138,8 @@ Lin2Args lin_args(nsp_handle* h, const double2* N, const double2* u, double2* ou
   a.qc = h->t.qc;
   a.Dc = h->Dc;
   a.Luc = h->Luc;
+  a.Lu = h->t.Lu;
+  a.Lp = h->t.Lp;
   a.F = h->F;
   a.PH = h->PH;
   a.invc = h->b.invc;
@@ -484,15 +486,16 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
   NlVariant best = nb.var;
   double bt = 1e30;
   // one stage at a time over all its compiled forms
-  const int treg[3][2] = {{4, 192}, {8, 256}, {8, 384}};
+  const int treg[4][2] = {{4, 192}, {8, 256}, {8, 384}, {16, 256}};
   for (int stage : {0, 2}) {
     ThetaVar& tb = stage == 0 ? best.inv : best.fwd;
     const char* nm = stage == 0 ? "theta_inv" : "theta_fwd";
     bt = 1e30;
     std::vector<ThetaVar> cands;
     for (int ord = 0; ord < 2; ord++) {
-      for (int pad = 0; pad < 2; pad++)
+      for (int pad = 0; pad < 3; pad++)
         for (int tl : {4, 8}) cands.push_back(ThetaVar{0, tl, 256, pad, ord});
+      cands.push_back(ThetaVar{0, 16, 256, 0, ord});
       for (const auto& tr : treg) cands.push_back(ThetaVar{1, tr[0], tr[1], 0, ord});
     }
     for (const ThetaVar& tv : cands) {
@@ -506,7 +509,7 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
     }
   }
   bt = 1e30;
-  for (int pad = 0; pad < 2; pad++)
+  for (int pad = 0; pad < 3; pad++)
     for (int nthr : {256, 512}) {
       NlVariant v = best;
       v.z_alg = 0;

@@ -207,6 +207,28 @@ struct Lay {
     else return e + (e / W) * PAD;
   }
   static constexpr int size(int n) { return PAD == 0 ? n : n + ((n + W - 1) / W) * PAD; }
+  // Positions of the elements first + q*SE, q < R, of a group that starts at
+  // a multiple of SPAN: p0 + q*step.  Exact when the group stays in one row
+  // (W % SPAN == 0) or steps by whole rows (SE % W == 0), with one division
+  // instead of one per element; otherwise p0 = -1 (compute each position).
+  template <int SE, int SPAN>
+  __device__ __forceinline__ static void affine(int first, int& p0, int& step) {
+    if constexpr (PAD == 0) {
+      p0 = first;
+      step = SE;
+    } else if constexpr (W % SPAN == 0) {
+      p0 = pos(first);
+      step = SE;
+    } else if constexpr (SE % W == 0) {
+      p0 = pos(first);
+      step = SE + (SE / W) * PAD;
+    } else {
+      p0 = -1;
+      step = 0;
+    }
+  }
+  template <int SE, int SPAN>
+  static constexpr bool is_affine() { return PAD == 0 || W % SPAN == 0 || SE % W == 0; }
 };
 using Dense = Lay<1, 0>;
 
@@ -221,8 +243,15 @@ __device__ __forceinline__ void dit_butterfly(const cpx<T>* buf, const cpx<T>* _
   const int bb = b / NCOL;
   const int k = bb % S;
   base = (bb - k) * R + k;
+  if constexpr (LY::template is_affine<S * NCOL, S * R * NCOL>()) {
+    int p0, st;
+    LY::template affine<S * NCOL, S * R * NCOL>(col + base * NCOL, p0, st);
 #pragma unroll
-  for (int q = 0; q < R; ++q) v[q] = buf[LY::pos(col + (base + q * S) * NCOL)];
+    for (int q = 0; q < R; ++q) v[q] = buf[p0 + q * st];
+  } else {
+#pragma unroll
+    for (int q = 0; q < R; ++q) v[q] = buf[LY::pos(col + (base + q * S) * NCOL)];
+  }
   if constexpr (S > 1) {
 #pragma unroll
     for (int q = 1; q < R; ++q) {
@@ -231,6 +260,20 @@ __device__ __forceinline__ void dit_butterfly(const cpx<T>* buf, const cpx<T>* _
     }
   }
   DFT<T, R, SIGN>::run(v);
+}
+
+// Stores the R outputs of a butterfly back to its positions.
+template <typename T, int NCOL, typename LY, int S, int R>
+__device__ __forceinline__ void dit_store(cpx<T>* buf, const cpx<T>* v, int col, int base) {
+  if constexpr (LY::template is_affine<S * NCOL, S * R * NCOL>()) {
+    int p0, st;
+    LY::template affine<S * NCOL, S * R * NCOL>(col + base * NCOL, p0, st);
+#pragma unroll
+    for (int q = 0; q < R; ++q) buf[p0 + q * st] = v[q];
+  } else {
+#pragma unroll
+    for (int q = 0; q < R; ++q) buf[LY::pos(col + (base + q * S) * NCOL)] = v[q];
+  }
 }
 
 // One DIT pass and the remaining ones.  NCOL interleaved columns (element n of
@@ -247,8 +290,7 @@ __device__ __forceinline__ void dit_passes(cpx<T>* buf, const cpx<T>* __restrict
       cpx<T> v[R];
       int col, base;
       dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, b, v, col, base);
-#pragma unroll
-      for (int q = 0; q < R; ++q) buf[LY::pos(col + (base + q * S) * NCOL)] = v[q];
+      dit_store<T, NCOL, LY, S, R>(buf, v, col, base);
     }
   }
   __syncthreads();
@@ -271,8 +313,7 @@ __device__ __forceinline__ void dit_passes_emit(cpx<T>* buf, const cpx<T>* __res
         cpx<T> v[R];
         int col, base;
         dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, b, v, col, base);
-#pragma unroll
-        for (int q = 0; q < R; ++q) buf[LY::pos(col + (base + q * S) * NCOL)] = v[q];
+        dit_store<T, NCOL, LY, S, R>(buf, v, col, base);
       }
     }
     __syncthreads();
@@ -311,8 +352,7 @@ template <typename T, int SIGN, typename LY, int NCOL, int R0, int... Rest>
 __device__ __forceinline__ void first_pass_store(Plan<R0, Rest...>, cpx<T>* v, cpx<T>* buf, int n0, int col = 0) {
   DFT<T, R0, SIGN>::run(v);
   const int b = Plan<R0, Rest...>::digitrev(n0);
-#pragma unroll
-  for (int q = 0; q < R0; ++q) buf[LY::pos(col + (b + q) * NCOL)] = v[q];
+  dit_store<T, NCOL, LY, 1, R0>(buf, v, col, b);
 }
 // Remaining passes after first_pass_store (and a barrier).
 template <typename T, int NTHR, int SIGN, typename LY, int R0, int... Rest>

@@ -4,9 +4,9 @@
 //   A (up)   rhs = E + d_im N (corrector) or N + CN part of u^n (predictor,
 //            which also stores the explicit part E); to r/theta; divergence
 //            (div_k, zero wall row); forward elimination of the pressure system
-//   B (down) pressure back substitution; rhs -= grad p three rows behind
-//            (addGrad_k); to +/-; zero wall rows (write_bcon_k)
-//   C (up)   forward elimination of the u+, u-, uz systems
+//   B (down) pressure back substitution (p overwrites its eliminated rhs)
+//   C (up)   rhs -= grad p three rows behind (addGrad_k); to +/-; zero wall
+//            rows (write_bcon_k); forward elimination of the u+, u-, uz systems
 //   D (down) back substitution of u+, u-, uz together with the homogeneous
 //            wall-unit solutions; after the four wall rows the wall
 //            divergence fixes the influence coefficients (corBCon), which
@@ -16,9 +16,16 @@
 //
 // The banded LU factors of the four operators depend only on (m, kz), dt and
 // Re, so they are computed once at setup with the reference's recurrence
-// (LUsolve_*_k, mod_linear.cu:55-646) and stored mode-fastest; every access of
-// a wavefront is one contiguous line.  Scratch SR (3 complex per row) and SP
-// (1 complex per row) carry the column between passes.
+// (LUsolve_*_k, mod_linear.cu:55-646) and stored mode-fastest, 7 per row:
+// every access of a wavefront is one contiguous line.  (Recomputing the L
+// multipliers in the upward passes saves 3 of the 7 but needs the U window of
+// three operators in registers, which halves occupancy: slower.)
+// Workgroups pair the modes m and -m of one axial chunk: the u- factors at m
+// are the u+ factors at -m (bit for bit: the operators differ only in the sign
+// of the 2m/r^2 term) and the uz and pressure factors and the pressure
+// homogeneous solution depend on |m|, so both waves read the same lines.
+// Scratch SR (3 complex per row) and SP (1 complex per row) carry the column
+// between passes.
 #include "nsp_internal.h"
 
 namespace nsp {
@@ -50,14 +57,119 @@ __device__ __forceinline__ void rt_of(double2& a, double2& b) {  // decoupleBack
   b = make_double2(0.5 * (up.y - um.y), -0.5 * (up.x - um.x));
 }
 
-// factor slot c of operator `kind` at row i: c 0..2 = L multipliers of columns
-// i-3..i-1, 3 = 1/U_ii, 4..6 = U_{i,i+1..i+3}
+// factor slot c of operator `kind` at row i: c 0..2 = L multipliers of
+// columns i-3..i-1, 3 = 1/U_ii, 4..6 = U_{i,i+1..i+3}
 __device__ __forceinline__ size_t fidx(const Geom& g, int kind, int i, int c, int k) {
   return ((size_t)(kind * g.nr + i) * 7 + c) * g.modes + k;
 }
 
+// The operator rows of one mode (LUsolve_*_k): kind 0 u+, 1 u-, 2 uz
+// (velocity Helmholtz, (1/dt) I - (d_im/Re)(Lu + diag)), 3 pressure (Lp + diag;
+// mode (0,0) has the wall row p = 0).
+struct OpMode {
+  const double* LHS;
+  int ip0, kind;
+  bool vel, p00;
+  double a, idt, kt, kz;
+};
+__device__ __forceinline__ OpMode op_mode(const Geom& g, const double* Lu, const double* Lp, int kind, int m, int l) {
+  OpMode o;
+  o.kind = kind;
+  o.LHS = (kind == 3) ? Lp : Lu;
+  const int fp = (kind <= 1) ? 1 : 0;
+  o.ip0 = abs((abs(m) % 2) - fp) * STEN * g.nr;
+  o.vel = kind < 3;
+  o.p00 = (kind == 3) && (l == 0 && m == 0);
+  o.a = g.d_im / g.re;
+  o.idt = 1.0 / g.dt;
+  o.kt = double(m);
+  o.kz = g.kz0 * double(l);
+  return o;
+}
+
+// Row i of the reference recurrence without the right-hand side.  Uw[s] is
+// the U row i-3+s (s = 2: row i-1), diagonal first.  Out: Lc[c] = multiplier
+// of column i-3+c (0 where the column is outside the band), Ui = U row i.
+// NL = min(i, 3) is the number of columns left of the diagonal; it is a
+// template parameter so that every register index is a constant.
+// Contraction is off so every caller gets the same bits.
+template <int NL>
+__device__ __forceinline__ void factor_row_nl(const OpMode& o, int nr, int i, double ri, const double (&Uw)[IW][IW + 1],
+                                              double (&Lc)[IW], double (&Ui)[IW + 1]) {
+#pragma clang fp contract(off)
+  const int right = min(i + IW, nr - 1);
+  const int i0 = i + IW - right;
+  const double r2 = ri * ri;
+  const bool last = (i == nr - 1);
+  const double* row = o.LHS + o.ip0 + STEN * i + i0;  // row[j] = entry of column left + j
+  double L[IW];
+#pragma unroll
+  for (int j = 0; j < IW; j++) {
+    L[j] = 0.0;
+    if (j < NL) {
+      double XM = row[j];
+      if (!last) {
+        if (o.vel) XM = -o.a * XM;
+      } else if (o.p00) {
+        XM = 0.0;
+      }
+      double Lj = XM;
+      // U row of column left+h lives at Uw[IW - NL + h]
+#pragma unroll
+      for (int h = 0; h < IW; h++)
+        if (h < j) Lj -= L[h] * Uw[IW - NL + h][j - h];
+      Lj = Lj / Uw[IW - NL + j][0];
+      L[j] = Lj;
+    }
+  }
+#pragma unroll
+  for (int j = 0; j <= IW; j++) {
+    Ui[j] = 0.0;
+    if (j < right - i + 1) {
+      double XM = row[NL + j];
+      if (!last) {
+        if (j == 0) {
+          const double kt = o.kt, kz = o.kz;
+          if (o.kind == 0) XM += -(1.0 / r2) - (2.0 * kt / r2) - kz * kz - kt * kt / r2;
+          else if (o.kind == 1) XM += -(1.0 / r2) + (2.0 * kt / r2) - kz * kz - kt * kt / r2;
+          else XM += -kz * kz - kt * kt / r2;
+          if (o.vel) XM = o.idt - o.a * XM;
+        } else if (o.vel) {
+          XM = -o.a * XM;
+        }
+      } else if (o.p00) {
+        XM = (j == 0) ? 1.0 : 0.0;
+      }
+      const int hleft = (NL + j - IW) > 0 ? (NL + j - IW) : 0;
+      double Uij = XM;
+#pragma unroll
+      for (int h = 0; h < IW; h++)
+        if (h >= hleft && h < NL) Uij -= L[h] * Uw[IW - NL + h][j + NL - h];
+      Ui[j] = Uij;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < IW; c++) Lc[c] = (c - (IW - NL) >= 0) ? L[(c - (IW - NL)) >= 0 ? c - (IW - NL) : 0] : 0.0;
+}
+
+__device__ __forceinline__ void factor_row(const OpMode& o, int nr, int i, double ri, const double (&Uw)[IW][IW + 1],
+                                           double (&Lc)[IW], double (&Ui)[IW + 1]) {
+  if (i >= IW) factor_row_nl<IW>(o, nr, i, ri, Uw, Lc, Ui);
+  else if (i == 2) factor_row_nl<2>(o, nr, i, ri, Uw, Lc, Ui);
+  else if (i == 1) factor_row_nl<1>(o, nr, i, ri, Uw, Lc, Ui);
+  else factor_row_nl<0>(o, nr, i, ri, Uw, Lc, Ui);
+}
+
+__device__ __forceinline__ void shift_u(double (&Uw)[IW][IW + 1], const double (&Ui)[IW + 1]) {
+#pragma unroll
+  for (int s = 0; s < IW - 1; s++)
+#pragma unroll
+    for (int j = 0; j <= IW; j++) Uw[s][j] = Uw[s + 1][j];
+#pragma unroll
+  for (int j = 0; j <= IW; j++) Uw[IW - 1][j] = Ui[j];
+}
+
 // ------------------------------------------------------------------ factors
-// Reference recurrence of LUsolve_*_k without the right-hand side.
 __global__ __launch_bounds__(256) void factor_kernel(Geom g, const double* __restrict__ r,
                                                      const double* __restrict__ Lu,
                                                      const double* __restrict__ Lp, double* __restrict__ F) {
@@ -65,98 +177,45 @@ __global__ __launch_bounds__(256) void factor_kernel(Geom g, const double* __res
   const int kind = blockIdx.y;
   if (k >= g.modes) return;
   const ModeQ q = modeq(g, k);
-  const double* LHS = (kind == 3) ? Lp : Lu;
-  const int fp = (kind <= 1) ? 1 : 0;
-  const int ip0 = abs((abs(q.m) % 2) - fp) * STEN * g.nr;
-  const bool vel = kind < 3;
-  const bool p00 = (kind == 3) && (q.l == 0 && q.m == 0);
-  const double a = g.d_im / g.re, idt = 1.0 / g.dt;
-  const double kt = q.kt, kz = q.kz;
-  const int nr = g.nr;
-  double Uw[IW][IW + 1];  // U rows i-3..i-1 (Uw[2] = row i-1)
+  const OpMode o = op_mode(g, Lu, Lp, kind, q.m, q.l);
+  double Uw[IW][IW + 1];
 #pragma unroll
   for (int s = 0; s < IW; s++)
 #pragma unroll
     for (int j = 0; j <= IW; j++) Uw[s][j] = 0.0;
-  for (int i = 0; i < nr; i++) {
-    const int left = max(0, i - IW), right = min(i + IW, nr - 1);
-    const int i0 = i + IW - right;
-    const int nl = i - left;
-    const double r2 = r[i] * r[i];
-    const bool last = (i == nr - 1);
-    double L[IW];
+  for (int i = 0; i < g.nr; i++) {
+    double Lc[IW], Ui[IW + 1];
+    factor_row(o, g.nr, i, r[i], Uw, Lc, Ui);
 #pragma unroll
-    for (int j = 0; j < IW; j++) {
-      L[j] = 0.0;
-      if (j < nl) {
-        double XM = LHS[ip0 + j + i0 + STEN * i];
-        if (!last) {
-          if (vel) XM = -a * XM;
-        } else if (p00) {
-          XM = 0.0;
-        }
-        double Lj = XM;
-        // U row of column left+h lives at Uw[IW - nl + h]
-#pragma unroll
-        for (int h = 0; h < IW; h++)
-          if (h < j) Lj -= L[h] * Uw[IW - nl + h][j - h];
-        Lj = Lj / Uw[IW - nl + j][0];
-        L[j] = Lj;
-      }
-    }
-    double Ui[IW + 1] = {0.0, 0.0, 0.0, 0.0};
-    int coun = nl;
-#pragma unroll
-    for (int j = 0; j <= IW; j++) {
-      if (j < right - i + 1) {
-        double XM = LHS[ip0 + STEN * i + i0 + coun];
-        if (!last) {
-          if (j == 0) {
-            if (kind == 0) XM += -(1.0 / r2) - (2.0 * kt / r2) - kz * kz - kt * kt / r2;
-            else if (kind == 1) XM += -(1.0 / r2) + (2.0 * kt / r2) - kz * kz - kt * kt / r2;
-            else XM += -kz * kz - kt * kt / r2;
-            if (vel) XM = idt - a * XM;
-          } else if (vel) {
-            XM = -a * XM;
-          }
-        } else if (p00) {
-          XM = (j == 0) ? 1.0 : 0.0;
-        }
-        const int hleft = max(0, coun - IW);
-        coun += 1;
-        double Uij = XM;
-#pragma unroll
-        for (int h = 0; h < IW; h++)
-          if (h >= hleft && h < nl) Uij -= L[h] * Uw[IW - nl + h][j + nl - h];
-        Ui[j] = Uij;
-      }
-    }
-    // store: L aligned to columns i-3..i-1
-#pragma unroll
-    for (int c = 0; c < IW; c++) {
-      const int j = c - (IW - nl);  // multiplier index of column i-3+c
-      F[fidx(g, kind, i, c, k)] = (j >= 0) ? L[j] : 0.0;
-    }
+    for (int c = 0; c < IW; c++) F[fidx(g, kind, i, c, k)] = Lc[c];
     F[fidx(g, kind, i, 3, k)] = 1.0 / Ui[0];
 #pragma unroll
     for (int j = 1; j <= IW; j++) F[fidx(g, kind, i, 3 + j, k)] = Ui[j];
-    // shift the U window
-#pragma unroll
-    for (int s = 0; s < IW - 1; s++)
-#pragma unroll
-      for (int j = 0; j <= IW; j++) Uw[s][j] = Uw[s + 1][j];
-#pragma unroll
-    for (int j = 0; j <= IW; j++) Uw[IW - 1][j] = Ui[j];
+    shift_u(Uw, Ui);
   }
 }
 
 // ------------------------------------------------------------------ sweep
-template <int PRED>
-__global__ __launch_bounds__(256) void sweep_kernel(Lin2Args A) {
+// Workgroup = 2 waves: azimuthal rows it and NT - it (0 with NT/2) over one
+// chunk of 64 axial modes.  One kernel per pass (PASS 0..3 = A..D): the
+// column moves between passes through SR/SP in any case, and each pass then
+// gets its own register allocation (and occupancy).
+template <int PASS, int PRED>
+__global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A) {
   const Geom& g = A.g;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.modes) return;
+  const int nch = (g.nz + 63) / 64;
+  const int pair = blockIdx.x / nch, ch = blockIdx.x - pair * nch;
+  const int nth = g.nt / 2;
+  const int it = (threadIdx.x < 64) ? pair : (pair == 0 ? nth : g.nt - pair);
+  const int l = ch * 64 + (threadIdx.x & 63);
+  if (l >= g.nz) return;
+  const int k = it * g.nz + l;
   const ModeQ q = modeq(g, k);
+  // stored factor columns: u+ at this mode; u- = u+ at -m (Nyquist row: its
+  // own u- factors); uz, p and the pressure homogeneous solution at |m|
+  const int k_abs = (it <= nth ? it : g.nt - it) * g.nz + l;
+  const int k_um = (it == nth) ? k : (it == 0 ? k : (g.nt - it) * g.nz + l);
+  const int kind_um = (it == nth) ? 1 : 0;
   const size_t n = g.n;
   const int nr = g.nr;
   const int MS = g.modes;
@@ -168,7 +227,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(Lin2Args A) {
   const double2 z2 = make_double2(0.0, 0.0);
 
   // ============================== pass A (up)
-  {
+  if constexpr (PASS == 0) {
     constexpr int LAG = PRED ? 6 : 3;
     double2 uw[3][7];     // predictor: u^n +/- rows t-6..t
     double2 Rr[7];        // R_r rows i-3..i+3
@@ -286,7 +345,8 @@ __global__ __launch_bounds__(256) void sweep_kernel(Lin2Args A) {
           py += q.kz * uz.x;
           dv = make_double2(px, py);
         }
-        const double l0 = A.F[fidx(g, 3, i, 0, k)], l1 = A.F[fidx(g, 3, i, 1, k)], l2 = A.F[fidx(g, 3, i, 2, k)];
+        const double l0 = A.F[fidx(g, 3, i, 0, k_abs)], l1 = A.F[fidx(g, 3, i, 1, k_abs)],
+                     l2 = A.F[fidx(g, 3, i, 2, k_abs)];
         dv.x -= l0 * yp[0].x;
         dv.y -= l0 * yp[0].y;
         dv.x -= l1 * yp[1].x;
@@ -302,74 +362,79 @@ __global__ __launch_bounds__(256) void sweep_kernel(Lin2Args A) {
   }
 
   // ============================== pass B (down)
-  {
-    double2 pw[7];  // p rows t..t+6
+  if constexpr (PASS == 1) {
+    double2 pw[3];  // p rows t+1..t+3
 #pragma unroll
-    for (int w = 0; w < 7; w++) pw[w] = z2;
-    for (int t = nr - 1; t >= -IW; t--) {
-      double2 pn = z2;
-      if (t >= 0) {
-        const double ru = A.F[fidx(g, 3, t, 3, k)];
-        const double u1 = A.F[fidx(g, 3, t, 4, k)], u2 = A.F[fidx(g, 3, t, 5, k)], u3 = A.F[fidx(g, 3, t, 6, k)];
-        double2 x = A.SP[(size_t)t * MS + k];
-        x.x -= u1 * pw[0].x;
-        x.y -= u1 * pw[0].y;
-        x.x -= u2 * pw[1].x;
-        x.y -= u2 * pw[1].y;
-        x.x -= u3 * pw[2].x;
-        x.y -= u3 * pw[2].y;
-        pn = make_double2(x.x * ru, x.y * ru);
-      }
-#pragma unroll
-      for (int w = 6; w > 0; w--) pw[w] = pw[w - 1];
+    for (int w = 0; w < 3; w++) pw[w] = z2;
+    for (int t = nr - 1; t >= 0; t--) {
+      const double ru = A.F[fidx(g, 3, t, 3, k_abs)];
+      const double u1 = A.F[fidx(g, 3, t, 4, k_abs)], u2 = A.F[fidx(g, 3, t, 5, k_abs)],
+                   u3 = A.F[fidx(g, 3, t, 6, k_abs)];
+      const size_t h = (size_t)t * MS + k;
+      double2 x = A.SP[h];
+      x.x -= u1 * pw[0].x;
+      x.y -= u1 * pw[0].y;
+      x.x -= u2 * pw[1].x;
+      x.y -= u2 * pw[1].y;
+      x.x -= u3 * pw[2].x;
+      x.y -= u3 * pw[2].y;
+      const double2 pn = make_double2(x.x * ru, x.y * ru);
+      A.SP[h] = pn;
+      pw[2] = pw[1];
+      pw[1] = pw[0];
       pw[0] = pn;
-      const int j = t + IW;
-      if (j >= 0 && j < nr) {  // addGrad_k (mod_radialFD.cu:115-143)
-        const size_t h = (size_t)j * MS + k;
-        const double* Dr = Dcp + STEN * j;
-        double px = 0.0, py = 0.0;
-#pragma unroll
-        for (int w = 0; w < 7; w++) {
-          px += Dr[w] * pw[w].x;
-          py += Dr[w] * pw[w].y;
-        }
-        const double rK = A.r[j];
-        const double2 ph = pw[3];
-        double2 a = A.SR[h], b = A.SR[n + h], c = A.SR[2 * n + h];
-        a.x -= px;
-        a.y -= py;
-        b.x -= -q.kt * ph.y / rK;
-        b.y -= q.kt * ph.x / rK;
-        c.x -= -q.kz * ph.y;
-        c.y -= q.kz * ph.x;
-        pm_of(a, b);
-        if (j == nr - 1) a = b = c = z2;
-        A.SR[h] = a;
-        A.SR[n + h] = b;
-        A.SR[2 * n + h] = c;
-      }
     }
   }
 
   // ============================== pass C (up)
-  {
-    double2 yw[3][3];
+  if constexpr (PASS == 2) {
+    double2 pw[7];      // p rows j-3..j+3
+    double2 yw[3][3];   // eliminated rows j-3..j-1 of u+, u-, uz
 #pragma unroll
     for (int c = 0; c < 3; c++)
 #pragma unroll
       for (int w = 0; w < 3; w++) yw[c][w] = z2;
-    for (int i = 0; i < nr; i++) {
-      const size_t h = (size_t)i * MS + k;
 #pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const double l0 = A.F[fidx(g, c, i, 0, k)], l1 = A.F[fidx(g, c, i, 1, k)], l2 = A.F[fidx(g, c, i, 2, k)];
-        double2 y = A.SR[c * n + h];
-        y.x -= l0 * yw[c][0].x;
-        y.y -= l0 * yw[c][0].y;
-        y.x -= l1 * yw[c][1].x;
-        y.y -= l1 * yw[c][1].y;
-        y.x -= l2 * yw[c][2].x;
-        y.y -= l2 * yw[c][2].y;
+    for (int w = 0; w < 7; w++) pw[w] = z2;
+    for (int t = 0; t < nr + IW; t++) {
+#pragma unroll
+      for (int w = 0; w < 6; w++) pw[w] = pw[w + 1];
+      pw[6] = (t < nr) ? A.SP[(size_t)t * MS + k] : z2;
+      const int j = t - IW;
+      if (j < 0) continue;
+      // addGrad_k (mod_radialFD.cu:115-143)
+      const size_t h = (size_t)j * MS + k;
+      const double* Dr = Dcp + STEN * j;
+      double px = 0.0, py = 0.0;
+#pragma unroll
+      for (int w = 0; w < 7; w++) {
+        px += Dr[w] * pw[w].x;
+        py += Dr[w] * pw[w].y;
+      }
+      const double rK = A.r[j];
+      const double2 ph = pw[3];
+      double2 rh[3] = {A.SR[h], A.SR[n + h], A.SR[2 * n + h]};
+      rh[0].x -= px;
+      rh[0].y -= py;
+      rh[1].x -= -q.kt * ph.y / rK;
+      rh[1].y -= q.kt * ph.x / rK;
+      rh[2].x -= -q.kz * ph.y;
+      rh[2].y -= q.kz * ph.x;
+      pm_of(rh[0], rh[1]);
+      if (j == nr - 1) rh[0] = rh[1] = rh[2] = z2;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {  // forward elimination
+        const int fkind = c == 1 ? kind_um : c, fk = c == 0 ? k : (c == 1 ? k_um : k_abs);
+        double Lc[IW];
+#pragma unroll
+        for (int w = 0; w < IW; w++) Lc[w] = A.F[fidx(g, fkind, j, w, fk)];
+        double2 y = rh[c];
+        y.x -= Lc[0] * yw[c][0].x;
+        y.y -= Lc[0] * yw[c][0].y;
+        y.x -= Lc[1] * yw[c][1].x;
+        y.y -= Lc[1] * yw[c][1].y;
+        y.x -= Lc[2] * yw[c][2].x;
+        y.y -= Lc[2] * yw[c][2].y;
         A.SR[c * n + h] = y;
         yw[c][0] = yw[c][1];
         yw[c][1] = yw[c][2];
@@ -379,7 +444,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(Lin2Args A) {
   }
 
   // ============================== pass D (down)
-  {
+  if constexpr (PASS == 3) {
     double2 xw[3][3];   // x rows i+1..i+3
     double xh[3][3];    // homogeneous wall-unit solutions rows i+1..i+3
 #pragma unroll
@@ -391,11 +456,13 @@ __global__ __launch_bounds__(256) void sweep_kernel(Lin2Args A) {
       }
     double2 xs[4][3];   // wall rows NR-1..NR-4
     double xhs[4][3];
+    const int fk[3] = {k, k_um, k_abs}, fkind[3] = {0, kind_um, 2};
     auto backsub = [&](int i, double2* xo, double* ho) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const double ru = A.F[fidx(g, c, i, 3, k)];
-        const double u1 = A.F[fidx(g, c, i, 4, k)], u2 = A.F[fidx(g, c, i, 5, k)], u3 = A.F[fidx(g, c, i, 6, k)];
+        const double ru = A.F[fidx(g, fkind[c], i, 3, fk[c])];
+        const double u1 = A.F[fidx(g, fkind[c], i, 4, fk[c])], u2 = A.F[fidx(g, fkind[c], i, 5, fk[c])],
+                     u3 = A.F[fidx(g, fkind[c], i, 6, fk[c])];
         double2 x = A.SR[c * n + (size_t)i * MS + k];
         x.x -= u1 * xw[c][0].x;
         x.y -= u1 * xw[c][0].y;
@@ -460,7 +527,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(Lin2Args A) {
     auto emit = [&](int i, const double2* xo, const double* ho) {
       const size_t h = (size_t)i * MS + k;
       // pressure-gradient homogeneous solution: (dr p_h -+ kt p_h / r, 0), (0, kz p_h)
-      const double dph = A.PH[(size_t)(2 * i) * MS + k], ph = A.PH[(size_t)(2 * i + 1) * MS + k];
+      const double dph = A.PH[(size_t)(2 * i) * MS + k_abs], ph = A.PH[(size_t)(2 * i + 1) * MS + k_abs];
       const double tq = q.kt * ph / A.r[i];
       const double2 hp[3] = {make_double2(dph - tq, 0.0), make_double2(dph + tq, 0.0), make_double2(-0.0, q.kz * ph)};
       const double2 hv[3] = {make_double2(ho[0], 0.0), make_double2(ho[1], 0.0), make_double2(0.0, ho[2])};
@@ -534,9 +601,18 @@ void launch_factor(const Geom& g, const double* r, const double* Lu, const doubl
 }
 
 void launch_sweep2(const Lin2Args& a, hipStream_t s) {
-  const int blocks = (a.g.modes + 255) / 256;
-  if (a.predictor) sweep_kernel<1><<<blocks, 256, 0, s>>>(a);
-  else sweep_kernel<0><<<blocks, 256, 0, s>>>(a);
+  const int blocks = (a.g.nt / 2) * ((a.g.nz + 63) / 64);
+  if (a.predictor) {
+    sweep_kernel<0, 1><<<blocks, 128, 0, s>>>(a);
+    sweep_kernel<1, 1><<<blocks, 128, 0, s>>>(a);
+    sweep_kernel<2, 1><<<blocks, 128, 0, s>>>(a);
+    sweep_kernel<3, 1><<<blocks, 128, 0, s>>>(a);
+  } else {
+    sweep_kernel<0, 0><<<blocks, 128, 0, s>>>(a);
+    sweep_kernel<1, 0><<<blocks, 128, 0, s>>>(a);
+    sweep_kernel<2, 0><<<blocks, 128, 0, s>>>(a);
+    sweep_kernel<3, 0><<<blocks, 128, 0, s>>>(a);
+  }
 }
 
 }  // namespace nsp

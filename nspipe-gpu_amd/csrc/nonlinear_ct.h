@@ -10,6 +10,14 @@ namespace nsp {
 
 __device__ __forceinline__ int m_of_ct(int it, int nt) { return it < nt / 2 ? it : it - nt; }
 
+// A copy of x the compiler cannot see through.  Index arithmetic derived from
+// it is recomputed where it is used instead of being hoisted out of the
+// transform loops and kept live in registers (which cost occupancy).
+__device__ __forceinline__ int opaque(int x) {
+  asm volatile("" : "+v"(x));
+  return x;
+}
+
 // ------------------------------------------------------------------ block order
 // Workgroups are dealt to the 8 XCDs round robin; remap so that each XCD
 // walks one contiguous range of logical blocks and neighbouring tiles and
@@ -70,14 +78,16 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
         DR[q] = dR[c * g.n + off];
       }
     }
+#pragma unroll 1
     for (int kind = 0; kind < 3; kind++) {  // 0: u, 1: d_r u, 2: d_theta u / r
-      for (int e = tid; e < (NTP - NT) * TL; e += NTHR) {  // padding rows
+      const int lt = opaque(tid);
+      for (int e = lt; e < (NTP - NT) * TL; e += NTHR) {  // padding rows
         const int j = half + e / TL, tl = e % TL;
         buf[LY::pos(PL::digitrev(j) * TL + tl)] = cpx<T>{T(0), T(0)};
       }
 #pragma unroll
       for (int q = 0; q < EPT; q++) {
-        const int e = tid + q * NTHR;
+        const int e = lt + q * NTHR;
         if (e < NT * TL) {
           const int it = e / TL, tl = e - it * TL;
           const double2 v = (kind == 1) ? DR[q] : U[q];
@@ -91,9 +101,9 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
         }
       }
       __syncthreads();
-      fft_run<T, TL, NTHR, +1, LY>(PL{}, buf, tw, tid);
+      fft_run<T, TL, NTHR, +1, LY>(PL{}, buf, tw, lt);
       cpx<T>* dst = W + (kind * 3 + c) * ws + (size_t)i * NTP * nz;
-      for (int e = tid; e < NTP * TL; e += NTHR) {
+      for (int e = lt; e < NTP * TL; e += NTHR) {
         const int j = e / TL, tl = e - j * TL, l = l0 + tl;
         if (l < nz) dst[(size_t)j * nz + l] = buf[LY::pos(e)];
       }
@@ -121,7 +131,8 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
   const T kz0 = T(g.kz0);
 
   auto c2r = [&](const cpx<T>* X, bool dz) {
-    for (int k = tid; k < NH; k += NTHR) {
+    const int lt = opaque(tid);
+    for (int k = lt; k < NH; k += NTHR) {
       const int kc = NH - k;
       cpx<T> a{T(0), T(0)}, b{T(0), T(0)};
       if (k < nz) a = X[k];
@@ -138,44 +149,52 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
       buf[LY::pos(PL::digitrev(k))] = cpx<T>{E.x - O.y, E.y + O.x};
     }
     __syncthreads();
-    fft_run<T, 1, NTHR, +1, LY>(PL{}, buf, tw, tid);
+    fft_run<T, 1, NTHR, +1, LY>(PL{}, buf, tw, lt);
   };
-  auto sample = [&](int n) -> T {
+  auto sample = [&](int n) -> T {  // n from an opaque thread index
     const cpx<T> v = buf[LY::pos(n >> 1)];
     return (n & 1) ? v.y : v.x;
   };
 
   T ur[MPT], ut[MPT], uz[MPT], acc[MPT];
   c2r(W + 0 * ws + roff, false);
+  int lt = opaque(tid);
 #pragma unroll
-  for (int q = 0; q < MPT; q++) { const int n = tid + q * NTHR; ur[q] = n < NZF ? sample(n) : T(0); }
+  for (int q = 0; q < MPT; q++) { const int n = lt + q * NTHR; ur[q] = n < NZF ? sample(n) : T(0); }
   __syncthreads();
   c2r(W + 1 * ws + roff, false);
+  lt = opaque(tid);
 #pragma unroll
-  for (int q = 0; q < MPT; q++) { const int n = tid + q * NTHR; ut[q] = n < NZF ? sample(n) : T(0); }
+  for (int q = 0; q < MPT; q++) { const int n = lt + q * NTHR; ut[q] = n < NZF ? sample(n) : T(0); }
   __syncthreads();
   c2r(W + 2 * ws + roff, false);
+  lt = opaque(tid);
 #pragma unroll
-  for (int q = 0; q < MPT; q++) { const int n = tid + q * NTHR; uz[q] = n < NZF ? sample(n) : T(0); }
+  for (int q = 0; q < MPT; q++) { const int n = lt + q * NTHR; uz[q] = n < NZF ? sample(n) : T(0); }
   __syncthreads();
 
+#pragma unroll 1
   for (int c = 0; c < 3; c++) {
     // calcNonlinear_k (mod_io_nonlinear.cu:142-170), summed in its order
     c2r(W + (3 + c) * ws + roff, false);
+    lt = opaque(tid);
 #pragma unroll
-    for (int q = 0; q < MPT; q++) { const int n = tid + q * NTHR; acc[q] = n < NZF ? ur[q] * sample(n) : T(0); }
+    for (int q = 0; q < MPT; q++) { const int n = lt + q * NTHR; acc[q] = n < NZF ? ur[q] * sample(n) : T(0); }
     __syncthreads();
     c2r(W + (6 + c) * ws + roff, false);
+    lt = opaque(tid);
 #pragma unroll
-    for (int q = 0; q < MPT; q++) { const int n = tid + q * NTHR; if (n < NZF) acc[q] += ut[q] * sample(n); }
+    for (int q = 0; q < MPT; q++) { const int n = lt + q * NTHR; if (n < NZF) acc[q] += ut[q] * sample(n); }
     __syncthreads();
     c2r(W + c * ws + roff, true);
+    lt = opaque(tid);
 #pragma unroll
-    for (int q = 0; q < MPT; q++) { const int n = tid + q * NTHR; if (n < NZF) acc[q] += uz[q] * sample(n); }
+    for (int q = 0; q < MPT; q++) { const int n = lt + q * NTHR; if (n < NZF) acc[q] += uz[q] * sample(n); }
     __syncthreads();
+    lt = opaque(tid);
 #pragma unroll
     for (int q = 0; q < MPT; q++) {
-      const int n = tid + q * NTHR;
+      const int n = lt + q * NTHR;
       if (n < NZF) {
         T v = acc[q];
         if (c == 0) v = v - ut[q] * (ut[q] / rad);
@@ -185,9 +204,9 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
       }
     }
     __syncthreads();
-    fft_run<T, 1, NTHR, -1, LY>(PL{}, buf, tw, tid);
+    fft_run<T, 1, NTHR, -1, LY>(PL{}, buf, tw, lt);
     cpx<T>* dst = Y + c * ws + roff;
-    for (int k = tid; k < nz; k += NTHR) {
+    for (int k = lt; k < nz; k += NTHR) {
       cpx<T> X{T(0), T(0)};
       if (k < nz - 1) {
         const cpx<T> zk = buf[LY::pos(k)], zc = buf[LY::pos(k == 0 ? 0 : NH - k)];
@@ -244,16 +263,20 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
     const int f = t < 3 ? t : ((t - 3) % 3 == 0 ? 3 + (t - 3) / 3 : ((t - 3) % 3 == 1 ? 6 + (t - 3) / 3 : (t - 3) / 3));
     return W + f * ws + roff;
   };
+  // X[k] and X[NH-k] of the owned samples, zero beyond the stored NZ modes:
+  // clamped in-range loads and a select, no per-element branches
   auto load = [&](const cpx<T>* X, cpx<T>* A, cpx<T>* B) {
+    const int lt = opaque(tid);
+    if (lt < NB1) {
 #pragma unroll
-    for (int q = 0; q < R0; q++) {
-      const int k = tid + NB1 * q, kc = NH - k;
-      // branches, not "c ? X[k] : zero": a conditional of two lvalues
-      // becomes a load through a selected generic pointer
-      A[q] = zero;
-      B[q] = zero;
-      if (own && k < nz) A[q] = X[k];
-      if (own && kc < nz) B[q] = X[kc];
+      for (int q = 0; q < R0; q++) {
+        const int k = lt + NB1 * q, kc = NH - k;
+        const cpx<T> a = X[min(k, nz - 1)], b = X[min(kc, nz - 1)];
+        A[q].x = k < nz ? a.x : T(0);
+        A[q].y = k < nz ? a.y : T(0);
+        B[q].x = kc < nz ? b.x : T(0);
+        B[q].y = kc < nz ? b.y : T(0);
+      }
     }
   };
 
@@ -262,10 +285,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   load(src(0), A, B);
 #pragma unroll 1
   for (int t = 0; t < 12; t++) {
-    // an opaque copy of the thread index: keeps the compiler from hoisting
-    // every pass's LDS index arithmetic out of the loop into live registers
-    int lt = tid;
-    asm volatile("" : "+v"(lt));
+    const int lt = opaque(tid);
     cpx<T> nA[R0], nB[R0];
     if (PF && t + 1 < 12) load(src(t + 1), nA, nB);
     const bool dz = t >= 3 && (t - 3) % 3 == 2;
@@ -377,17 +397,19 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
   const int tid = threadIdx.x;
   const int nt = g.nt, half = nt / 2, nz = g.nz;
   const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
+#pragma unroll 1
   for (int c = 0; c < 3; c++) {
+    const int lt = opaque(tid);
     const cpx<T>* src = Y + c * ws + (size_t)i * NTP * nz;
-    for (int e = tid; e < NTP * TL; e += NTHR) {
+    for (int e = lt; e < NTP * TL; e += NTHR) {
       const int j = e / TL, tl = e - j * TL, l = l0 + tl;
       cpx<T> v{T(0), T(0)};
       if (l < nz - 1) v = src[(size_t)j * nz + l];
       buf[LY::pos(PL::digitrev(j) * TL + tl)] = v;
     }
     __syncthreads();
-    fft_run<T, TL, NTHR, -1, LY>(PL{}, buf, tw, tid);
-    for (int e = tid; e < nt * TL; e += NTHR) {
+    fft_run<T, TL, NTHR, -1, LY>(PL{}, buf, tw, lt);
+    for (int e = lt; e < nt * TL; e += NTHR) {
       const int it = e / TL, tl = e - it * TL, l = l0 + tl;
       if (l < nz) {
         const int j = it < half ? it : it + half;  // padBackward_k row map
@@ -439,10 +461,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
 #pragma unroll 1
   for (int f = 0; f < 9; f++) {
     const int c = f % 3, kind = f / 3;  // kind 0: u, 1: d_r u, 2: d_theta u / r
-    // opaque copy of the thread index: stops the index arithmetic of all nine
-    // transforms from being hoisted into live registers
-    int lt = tid;
-    asm volatile("" : "+v"(lt));
+    const int lt = opaque(tid);
     {
       const double2* src = (kind == 1 ? dR : u) + c * g.n + (size_t)i * g.modes;
 #pragma unroll
@@ -503,8 +522,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
   const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
 #pragma unroll 1
   for (int c = 0; c < 3; c++) {
-    int lt = tid;
-    asm volatile("" : "+v"(lt));
+    const int lt = opaque(tid);
     const cpx<T>* src = Y + c * ws + (size_t)i * NTP * nz;
 #pragma unroll
     for (int s = 0; s < PPT; s++) {
@@ -561,6 +579,12 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
 // ------------------------------------------------------------------ stage launchers
 // Variant dispatch for one compiled plan PL.  The LDS pad shift is 4 (one
 // slot per 16 elements) for tile width 4 and 3 (one per 8) for width 8.
+// Row padding of the theta scatter forms: rows of (NTP / R_last) positions x
+// TL columns followed by TL unused slots (bank model of the scatter and pass
+// patterns).
+template <typename PL, int TL>
+using ThetaRowPad = Lay<(PL::L / PL::rad[PL::P - 1]) * TL, TL>;
+
 template <typename T, typename PL>
 bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
                     const double2* dR, cpx<T>* W, size_t ws, hipStream_t s) {
@@ -571,14 +595,20 @@ bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const do
     if (tl == 4 && v.nthr == 192) theta_inv2_ct<T, 4, 192, Dense, PL><<<nb, 192, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
     else if (tl == 8 && v.nthr == 256) theta_inv2_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
     else if (tl == 8 && v.nthr == 384) theta_inv2_ct<T, 8, 384, Dense, PL><<<nb, 384, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+    else if (tl == 16 && v.nthr == 256) theta_inv2_ct<T, 16, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
     else return false;
     return true;
   }
   if (v.alg != 0) return false;
-  if (tl == 4 && !v.pad) theta_inv_ct<T, 4, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-  else if (tl == 4) theta_inv_ct<T, 4, 256, Lay<16, 1>, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-  else if (tl == 8 && !v.pad) theta_inv_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-  else if (tl == 8) theta_inv_ct<T, 8, 256, Lay<8, 1>, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  using R4 = ThetaRowPad<PL, 4>;
+  using R8 = ThetaRowPad<PL, 8>;
+  if (tl == 4 && v.pad == 0) theta_inv_ct<T, 4, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  else if (tl == 4 && v.pad == 1) theta_inv_ct<T, 4, 256, Lay<16, 1>, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  else if (tl == 4 && v.pad == 2) theta_inv_ct<T, 4, 256, R4, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  else if (tl == 8 && v.pad == 0) theta_inv_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  else if (tl == 8 && v.pad == 1) theta_inv_ct<T, 8, 256, Lay<8, 1>, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  else if (tl == 8 && v.pad == 2) theta_inv_ct<T, 8, 256, R8, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  else if (tl == 16 && v.pad == 0) theta_inv_ct<T, 16, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
   else return false;
   return true;
 }
@@ -593,14 +623,20 @@ bool theta_fwd_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const cp
     if (tl == 4 && v.nthr == 192) theta_fwd2_ct<T, 4, 192, Dense, PL><<<nb, 192, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
     else if (tl == 8 && v.nthr == 256) theta_fwd2_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
     else if (tl == 8 && v.nthr == 384) theta_fwd2_ct<T, 8, 384, Dense, PL><<<nb, 384, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+    else if (tl == 16 && v.nthr == 256) theta_fwd2_ct<T, 16, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
     else return false;
     return true;
   }
   if (v.alg != 0) return false;
-  if (tl == 4 && !v.pad) theta_fwd_ct<T, 4, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-  else if (tl == 4) theta_fwd_ct<T, 4, 256, Lay<16, 1>, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-  else if (tl == 8 && !v.pad) theta_fwd_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-  else if (tl == 8) theta_fwd_ct<T, 8, 256, Lay<8, 1>, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  using R4 = ThetaRowPad<PL, 4>;
+  using R8 = ThetaRowPad<PL, 8>;
+  if (tl == 4 && v.pad == 0) theta_fwd_ct<T, 4, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  else if (tl == 4 && v.pad == 1) theta_fwd_ct<T, 4, 256, Lay<16, 1>, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  else if (tl == 4 && v.pad == 2) theta_fwd_ct<T, 4, 256, R4, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  else if (tl == 8 && v.pad == 0) theta_fwd_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  else if (tl == 8 && v.pad == 1) theta_fwd_ct<T, 8, 256, Lay<8, 1>, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  else if (tl == 8 && v.pad == 2) theta_fwd_ct<T, 8, 256, R8, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  else if (tl == 16 && v.pad == 0) theta_fwd_ct<T, 16, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
   else return false;
   return true;
 }
@@ -626,6 +662,16 @@ bool zpass_plan(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T
     return true;
   }
   if (v.z_alg != 0) return false;
+  using RP = Lay<LL / PL::rad[PL::P - 1], zpass2_pad<PL>()>;
+  if (v.z_pad == 2) {
+    if (v.z_nthr == 256)
+      zpass_ct<T, 256, (2 * LL + 255) / 256, RP, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
+    else if (v.z_nthr == 512)
+      zpass_ct<T, 512, (2 * LL + 511) / 512, RP, PL><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y);
+    else
+      return false;
+    return true;
+  }
   if (v.z_nthr == 256 && !v.z_pad)
     zpass_ct<T, 256, (2 * LL + 255) / 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
   else if (v.z_nthr == 256)

@@ -91,6 +91,8 @@ struct Lin2Args {
   const double* qc;     // [1]
   const double* Dc;     // [2][NR][7] D weights of columns i-3..i+3 (0 outside the grid)
   const double* Luc;    // [2][NR][7] same for Lu
+  const double* Lu;     // [2][NR][7] operator tables (factor recurrence on the fly)
+  const double* Lp;     // [2][NR][7]
   const double* F;      // [4][NR][7][MODES] banded LU factors
   const double* PH;     // [NR][2][MODES] dr p_h, p_h of the homogeneous pressure solution
   const double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
@@ -108,10 +110,11 @@ void launch_sweep2(const Lin2Args& a, hipStream_t s);
 // centred stencil table: w-th weight of row i multiplies column i-3+w
 std::vector<double> centred_table(int nr, const std::vector<double>& T);
 
-// Kernel variant of the compiled-plan path (chosen by the autotuner).
-// Kernel variant of one theta stage: alg 0 = LDS-scatter form (tl 4 or 8
-// columns, 256 threads, pad = one LDS slot per 16 or 8 elements), alg 1 =
-// register form ((tl, nthr) = (4, 192), (8, 256) or (8, 384)); ord = block
+// Kernel variant of one theta stage: alg 0 = LDS-scatter form (tl 4, 8 or 16
+// columns, 256 threads; pad 1 = one LDS slot per 16 or 8 elements, pad 2 =
+// rows of (NTP/R_last)*tl elements padded by tl slots; tl 16 dense only),
+// alg 1 = register form ((tl, nthr) = (4, 192), (8, 256), (8, 384) or
+// (16, 256)); ord = block
 // order (0: radial level fastest, 1: column tile fastest).
 struct ThetaVar {
   int alg, tl, nthr, pad, ord;
@@ -121,7 +124,7 @@ struct NlVariant {
   ThetaVar inv, fwd;
   int z_alg;   // z pass: 0 LDS-scatter form, 1 register form (256 threads)
   int z_nthr;  // scatter form: threads per row (256 or 512)
-  int z_pad;   // scatter form: one LDS slot per 16 elements
+  int z_pad;   // scatter form: 1 = one LDS slot per 16 elements, 2 = row padding as the register form
   int z_pf;    // register form: prefetch the next transform's inputs
 };
 // Nonlinear-phase buffers and FFT data.

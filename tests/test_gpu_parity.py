@@ -181,10 +181,11 @@ def _variants():
     (the others at their defaults)."""
     th = []
     for ord_ in (0, 1):
-        for pad in (0, 1):
+        for pad in (0, 1, 2):
             for tl in (4, 8):
                 th.append(dict(alg=0, tl=tl, nthr=256, pad=pad, ord=ord_))
-        for tl, nthr in ((4, 192), (8, 256), (8, 384)):
+        th.append(dict(alg=0, tl=16, nthr=256, pad=0, ord=ord_))
+        for tl, nthr in ((4, 192), (8, 256), (8, 384), (16, 256)):
             th.append(dict(alg=1, tl=tl, nthr=nthr, pad=0, ord=ord_))
     base = dict(theta_inv=dict(alg=0, tl=8, nthr=256, pad=0, ord=1),
                 theta_fwd=dict(alg=0, tl=8, nthr=256, pad=0, ord=1),
@@ -194,7 +195,7 @@ def _variants():
         out.append(dict(base, theta_inv=t))
         out.append(dict(base, theta_fwd=t))
     for nthr in (256, 512):
-        for pad in (0, 1):
+        for pad in (0, 1, 2):
             out.append(dict(base, z_alg=0, z_nthr=nthr, z_pad=pad))
     for pf in (0, 1):
         out.append(dict(base, z_alg=1, z_pf=pf))
@@ -220,7 +221,7 @@ def test_every_kernel_variant_keeps_parity(shape):
 
 def test_set_tuning_rejects_missing_variant():
     p = nsp.Pipe(**COMPILED[0])
-    bad = dict(theta_inv=dict(alg=1, tl=16, nthr=256, pad=0, ord=0),
+    bad = dict(theta_inv=dict(alg=1, tl=32, nthr=256, pad=0, ord=0),
                theta_fwd=dict(alg=0, tl=8, nthr=256, pad=0, ord=1), z_alg=1, z_nthr=256, z_pad=0, z_pf=0)
     with pytest.raises(nsp.NspError):
         p.set_tuning(bad)

@@ -8,9 +8,11 @@ Poiseuille profile.  A "step" is one time step of the reference integrator:
 one nonlinear evaluation + predictor, then corrector sweeps (each a nonlinear
 evaluation + solve) until the convergence measure drops below tol.
 
-Prints one JSON line (rank 0).  value = time steps/s of the headline form;
-the roofline object is for the kernel with the largest share of the timed
-region, measured with HIP events on the stream the kernels run on.
+Prints one JSON line (rank 0).  value = time steps/s of the headline form,
+each step one hipGraph launch (the library default).  The same number of
+steps is then run kernel by kernel with HIP events recorded on the stream the
+kernels run on; the roofline object is for the kernel with the largest share
+of that run, from its average launch duration.
 """
 import argparse
 import json
@@ -68,20 +70,29 @@ def run_gpu(nsp, precision, steps, warmup, cfg, chosen=None):
     print("autotune %s: %s" % (precision, json.dumps(tuning)), file=sys.stderr, flush=True)
     if warmup:
         p.step(warmup)
-    p.profile_enable(True)
+    # timed region: whole steps as captured hipGraphs (the library default)
     p.synchronize()  # single process: the barrier is the device synchronisation
     t0 = time.perf_counter()
     n_it, err, flags = p.step(steps)
     p.synchronize()
     t1 = time.perf_counter()
+    # the same number of steps launched kernel by kernel with HIP events on the
+    # pipe's stream, for the per-kernel durations (events inside a captured
+    # graph yield no elapsed times)
+    p.profile_enable(True)
+    p.synchronize()
+    t2 = time.perf_counter()
+    n_it2, _, flags2 = p.step(steps)
+    p.synchronize()
+    t3 = time.perf_counter()
     prof = p.profile_read()
     p.profile_enable(False)
     diag = p.diagnostics()
     p.close()
     secs = t1 - t0
     return dict(steps_per_s=steps / secs, ms_per_step=1e3 * secs / steps, n_it=n_it.tolist(),
-                err_last=float(err[-1]), flags=int(flags.max()), kernels=prof, diag=diag,
-                timed_s=secs, tuning=tuning)
+                err_last=float(err[-1]), flags=int(max(flags.max(), flags2.max())), kernels=prof, diag=diag,
+                timed_s=secs, tuning=tuning, events_steps_per_s=steps / (t3 - t2), events_n_it=n_it2.tolist())
 
 
 def roofline(res, cfg, fp64, traffic):
@@ -225,7 +236,8 @@ def main():
         rr = results[f]
         if "steps_per_s" in rr:
             out[f] = dict(steps_per_s=round(rr["steps_per_s"], 4), ms_per_step=round(rr["ms_per_step"], 3),
-                          n_it=rr["n_it"], kernels_ms={k: round(v[0] / max(v[1], 1), 4)
+                          n_it=rr["n_it"], events_steps_per_s=round(rr["events_steps_per_s"], 4),
+                          kernels_ms={k: round(v[0] / max(v[1], 1), 4)
                                                        for k, v in rr["kernels"].items() if v[1]},
                           ke=rr["diag"]["ke"], u_tau=rr["diag"]["u_tau"],
                           tuning=(rr["tuning"] or {}).get("chosen"))

@@ -25,6 +25,13 @@ static nsp_status fail(nsp_status st, const std::string& msg) {
       return fail(NSP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
   } while (0)
 
+// Outcome of one captured step, written on the device and copied back by the
+// graph: converged flag, corrector iterations, flags; final err.
+struct StepRec {
+  int st[4];
+  double err;
+};
+
 struct nsp_handle {
   nsp_params P;
   Geom g;
@@ -49,6 +56,14 @@ struct nsp_handle {
   unsigned long long* norm = nullptr;  // device [2]
   unsigned long long* norm_h = nullptr;  // pinned host [2]
   double* part = nullptr;   // diagnostics partials [MODES][4]
+  // whole-step graphs: state buffers U[0..2] (u, ui, un point into them),
+  // U[loc] holds the state; one graph per starting buffer
+  double2* U[3] = {};
+  int loc = 0;
+  bool graphs = true;
+  StepRec* rec = nullptr;    // device
+  StepRec* rec_h = nullptr;  // pinned host
+  hipGraphExec_t gexec[3] = {};
   std::vector<void*> allocs;
   Profiler prof;
   std::string tune_report = "{\"tuned\": false}";
@@ -66,7 +81,17 @@ nsp_status dalloc(nsp_handle* h, T** ptr, size_t count) {
   return NSP_OK;
 }
 
+void drop_graphs(nsp_handle* h) {
+  for (int a = 0; a < 3; a++) {
+    if (h->gexec[a]) (void)hipGraphExecDestroy(h->gexec[a]);
+    h->gexec[a] = nullptr;
+  }
+}
+
 void free_all(nsp_handle* h) {
+  drop_graphs(h);
+  if (h->rec_h) (void)hipHostFree(h->rec_h);
+  h->rec_h = nullptr;
   for (void* p : h->allocs) (void)hipFree(p);
   h->allocs.clear();
   if (h->norm_h) (void)hipHostFree(h->norm_h);
@@ -129,9 +154,9 @@ nsp_status fetch_fields(nsp_handle* h, const double2* src, double* a, double* b,
   return NSP_OK;
 }
 
-Lin2Args lin_args(nsp_handle* h, const double2* N, const double2* u, double2* out, bool predictor) {
+Lin2Args lin_args(nsp_handle* h, const Geom& g, const double2* N, const double2* u, double2* out, bool predictor) {
   Lin2Args a;
-  a.g = h->g;
+  a.g = g;
   a.r = h->t.r;
   a.rdr = h->t.rdr;
   a.uc = h->t.uc;
@@ -251,6 +276,7 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   g.dt = p->dt;
   g.d_im = p->d_im;
   g.qdes = p->qdes;
+  g.skip = nullptr;
 
   auto bail = [&](nsp_status s) {
     free_all(h);
@@ -317,6 +343,20 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   TRY(dalloc(h, &h->u, 3 * n));
   TRY(dalloc(h, &h->ui, 3 * n));
   TRY(dalloc(h, &h->un, 3 * n));
+  h->U[0] = h->u;
+  h->U[1] = h->ui;
+  h->U[2] = h->un;
+  h->loc = 0;
+  TRY(dalloc(h, &h->rec, 1));
+  {
+    void* ph = nullptr;
+    if (hipHostMalloc(&ph, sizeof(StepRec)) != hipSuccess) return fail(NSP_ERR_NOMEM, "pinned step record");
+    h->rec_h = static_cast<StepRec*>(ph);
+  }
+  {
+    const char* ng = std::getenv("NSP_NO_GRAPH");
+    h->graphs = !(ng && ng[0] && ng[0] != '0');
+  }
   TRY(dalloc(h, &h->Nl, 3 * n));
   TRY(dalloc(h, &h->E, 3 * n));
   TRY(dalloc(h, &h->SR, 3 * n));
@@ -533,6 +573,7 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
   (void)hipEventDestroy(e1);
   HIPCHK(hipGetLastError());
   nb.var = best;
+  drop_graphs(h);  // captured steps hold the old variants
   rep += "}, \"chosen\": " + variant_json(best) + "}";
   h->tune_report = rep;
   return NSP_OK;
@@ -593,10 +634,64 @@ nsp_status nsp_set_tuning(nsp_handle* h, const char* json) {
     return fail(NSP_ERR_INVALID, "tuning names a kernel variant that is not compiled for this size");
   }
   h->tune_report = "{\"tuned\": \"set\", \"chosen\": " + variant_json(v) + "}";
+  drop_graphs(h);
   return NSP_OK;
 }
 
-// integrate() loop body (mod_integrate.cu:112-131)
+namespace {
+
+// Captures one whole time step starting from the state in U[a]: the predictor
+// writes U[a+1], correctors alternate U[a+1] -> U[a+2] -> U[a+1] ...; every
+// corrector launch carries the converged flag and returns at once after the
+// loop has ended, so the graph always holds maxit corrector iterations and
+// the host only reads the step record at the end.
+nsp_status capture_step(nsp_handle* h, int a) {
+  hipStream_t s = h->stream;
+  const Geom& g = h->g;
+  Geom gk = g;
+  gk.skip = &h->rec->st[0];
+  double2* ua = h->U[a];
+  double2* ub = h->U[(a + 1) % 3];
+  double2* uc = h->U[(a + 2) % 3];
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  launch_step_begin(&h->rec->st[0], &h->rec->err, s);
+  launch_nonlinear(g, h->t, h->nb, ua, h->Nl, s, nullptr);
+  launch_sweep2(lin_args(h, g, h->Nl, ua, ub, true), s);
+  for (int it = 1; it <= h->P.maxit; it++) {
+    double2* src = (it % 2) ? ub : uc;
+    double2* dst = (it % 2) ? uc : ub;
+    launch_nonlinear(gk, h->t, h->nb, src, h->Nl, s, nullptr);
+    launch_norm_reset(h->norm, gk.skip, s);
+    launch_sweep2(lin_args(h, gk, h->Nl, src, dst, false), s);
+    launch_converge(h->norm, &h->rec->st[0], &h->rec->err, h->P.tol, it, h->P.maxit, s);
+  }
+  (void)hipMemcpyAsync(h->rec_h, h->rec, sizeof(StepRec), hipMemcpyDeviceToHost, s);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(s, &graph);
+  if (e != hipSuccess || !graph) return fail(NSP_ERR_HIP, std::string("step capture: ") + hipGetErrorString(e));
+  const hipError_t e2 = hipGraphInstantiate(&h->gexec[a], graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e2 != hipSuccess) {
+    h->gexec[a] = nullptr;
+    return fail(NSP_ERR_HIP, std::string("step graph instantiate: ") + hipGetErrorString(e2));
+  }
+  return NSP_OK;
+}
+
+void set_state_slot(nsp_handle* h, int loc) {
+  h->loc = loc;
+  h->u = h->U[loc];
+  h->ui = h->U[(loc + 1) % 3];
+  h->un = h->U[(loc + 2) % 3];
+}
+
+}  // namespace
+
+// integrate() loop body (mod_integrate.cu:112-131).  Default: one hipGraph
+// launch per step (capture_step).  With per-kernel profiling on, or with
+// NSP_NO_GRAPH=1, the kernels are launched one by one with a host-side
+// convergence test after every corrector (HIP events recorded inside a
+// captured graph do not yield elapsed times, so timing needs eager launches).
 nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {
   if (!h) return fail(NSP_ERR_INVALID, "null handle");
   if (nsteps < 0) return fail(NSP_ERR_INVALID, "nsteps < 0");
@@ -605,33 +700,54 @@ nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {
   const Geom& g = h->g;
   Profiler* pf = h->prof.on ? &h->prof : nullptr;
   for (int step = 0; step < nsteps; step++) {
-    // nonlinear(u,rhs); predictor(u,rhs,uw,rhsw,ui)
-    launch_nonlinear(g, h->t, h->nb, h->u, h->Nl, s, pf);
-    if (pf) pf->begin(K_LIN_PRED, s);
-    launch_sweep2(lin_args(h, h->Nl, h->u, h->ui, true), s);
-    if (pf) pf->end(s);
-    HIPCHK(hipGetLastError());
+    int ite = 0, flags = 0;
     double err = 1e98;
-    int ite = 0;
-    while (ite < h->P.maxit && err > h->P.tol) {
-      ite += 1;
-      launch_nonlinear(g, h->t, h->nb, h->ui, h->Nl, s, pf);
-      launch_norm_reset(h->norm, s);
-      if (pf) pf->begin(K_LIN_CORR, s);
-      launch_sweep2(lin_args(h, h->Nl, h->ui, h->un, false), s);
+    if (h->graphs && !h->prof.on) {
+      const int a = h->loc;
+      if (!h->gexec[a]) {
+        const nsp_status st = capture_step(h, a);
+        if (st != NSP_OK) return st;
+      }
+      HIPCHK(hipGraphLaunch(h->gexec[a], s));
+      HIPCHK(hipStreamSynchronize(s));
+      ite = h->rec_h->st[1];
+      flags = h->rec_h->st[2];
+      err = h->rec_h->err;
+      // newest iterate: the predictor's output (no corrector) or the last corrector's
+      const int newest = (ite % 2 == 1) ? (a + 2) % 3 : (a + 1) % 3;
+      set_state_slot(h, newest);
+    } else {
+      // nonlinear(u,rhs); predictor(u,rhs,uw,rhsw,ui)
+      launch_nonlinear(g, h->t, h->nb, h->u, h->Nl, s, pf);
+      if (pf) pf->begin(K_LIN_PRED, s);
+      launch_sweep2(lin_args(h, g, h->Nl, h->u, h->ui, true), s);
       if (pf) pf->end(s);
       HIPCHK(hipGetLastError());
-      HIPCHK(hipMemcpyAsync(h->norm_h, h->norm, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
-      HIPCHK(hipStreamSynchronize(s));
-      err = bits_to_double(h->norm_h[0]) / bits_to_double(h->norm_h[1]);
-      std::swap(h->ui, h->un);
+      while (ite < h->P.maxit && err > h->P.tol) {
+        ite += 1;
+        launch_nonlinear(g, h->t, h->nb, h->ui, h->Nl, s, pf);
+        launch_norm_reset(h->norm, nullptr, s);
+        if (pf) pf->begin(K_LIN_CORR, s);
+        launch_sweep2(lin_args(h, g, h->Nl, h->ui, h->un, false), s);
+        if (pf) pf->end(s);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipMemcpyAsync(h->norm_h, h->norm, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+        HIPCHK(hipStreamSynchronize(s));
+        err = bits_to_double(h->norm_h[0]) / bits_to_double(h->norm_h[1]);
+        std::swap(h->ui, h->un);
+      }
+      // the last iterate becomes the state (ui); keep the U[] slots in step
+      int newest = 0;
+      for (int q = 0; q < 3; q++)
+        if (h->U[q] == h->ui) newest = q;
+      set_state_slot(h, newest);
+      flags = (std::isnan(err) ? NSP_FLAG_NAN : 0) | ((!std::isnan(err) && err > h->P.tol) ? NSP_FLAG_MAXIT : 0);
+      if (pf) pf->collect();
     }
-    std::swap(h->u, h->ui);  // the last iterate becomes the state
-    if (pf) pf->collect();
     if (info) {
       info[step].n_it = ite;
       info[step].err = err;
-      info[step].flags = (std::isnan(err) ? NSP_FLAG_NAN : 0) | ((!std::isnan(err) && err > h->P.tol) ? NSP_FLAG_MAXIT : 0);
+      info[step].flags = flags;
     }
   }
   return NSP_OK;

@@ -170,208 +170,35 @@ __device__ __forceinline__ double2 cmul2(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// ------------------------------------------------------------------ sweep
-__global__ __launch_bounds__(256) void linear_sweep_kernel(LinArgs A) {
-  const Geom& g = A.g;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.modes) return;
-  const Mode q = mode_of(g, k);
-  const size_t n = g.n;
-  const int nr = g.nr;
-  double2* R0 = A.rhs;
-  double2* R1 = A.rhs + n;
-  double2* R2 = A.rhs + 2 * n;
-  const double d_im = g.d_im;
+__global__ void norm_reset_kernel(unsigned long long* norm, const int* skip) {
+  if (skip && *skip) return;
+  if (threadIdx.x < 2) norm[threadIdx.x] = 0ull;
+}
 
-  // (1) rhs = N(u) [predictor] or d_im N(u^k) + (1-d_im) N(u^n) [corrector], +/- form
-  for (int i = 0; i < nr; i++) {
-    const size_t h = at(g, i, k);
-    double2 nr_ = A.N[h], nt_ = A.N[n + h];
-    const double2 nz_ = A.N[2 * n + h];
-    to_pm(nr_, nt_);
-    if (A.predictor) {
-      A.Nw[h] = nr_;
-      A.Nw[n + h] = nt_;
-      A.Nw[2 * n + h] = nz_;
-      double2 a = A.u[h], b = A.u[n + h];
-      to_pm(a, b);
-      A.uw[h] = a;
-      A.uw[n + h] = b;
-      A.uw[2 * n + h] = A.u[2 * n + h];
-      R0[h] = nr_;
-      R1[h] = nt_;
-      R2[h] = nz_;
-    } else {  // addRHS_k (mod_integrate.cu:43-59)
-      const double2 w0 = A.Nw[h], w1 = A.Nw[n + h], w2 = A.Nw[2 * n + h];
-      R0[h] = make_double2(d_im * nr_.x + (1.0 - d_im) * w0.x, d_im * nr_.y + (1.0 - d_im) * w0.y);
-      R1[h] = make_double2(d_im * nt_.x + (1.0 - d_im) * w1.x, d_im * nt_.y + (1.0 - d_im) * w1.y);
-      R2[h] = make_double2(d_im * nz_.x + (1.0 - d_im) * w2.x, d_im * nz_.y + (1.0 - d_im) * w2.y);
-    }
-  }
-  // (2) explicit Crank-Nicolson part, addLap_k (mod_linear.cu:12-51)
-  {
-    const double cn = (1.0 - d_im) / g.re, idt = 1.0 / g.dt;
-    for (int i = 0; i < nr; i++) {
-      const size_t h = at(g, i, k);
-      const double r2 = A.t.r[i] * A.t.r[i];
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const int fp = (c < 2) ? 1 : 0;
-        const int ip0 = abs((abs(q.m) % 2) - fp) * STEN * nr;
-        const double2* uwc = A.uw + c * n;
-        double2 prod = stencil(g, A.t.Lu, ip0, i, uwc, k);
-        const double2 aux = uwc[h];
-        double de = -q.kz * q.kz - q.kt * q.kt / r2;
-        if (c == 0) de += -(1.0 / r2) - (2.0 * q.kt / r2);
-        else if (c == 1) de += -(1.0 / r2) + (2.0 * q.kt / r2);
-        prod.x += de * aux.x;
-        prod.y += de * aux.y;
-        prod.x *= cn;
-        prod.y *= cn;
-        prod.x += idt * aux.x;
-        prod.y += idt * aux.y;
-        double2* Rc = A.rhs + c * n;
-        double2 rv = Rc[h];
-        rv.x += prod.x;
-        rv.y += prod.y;
-        Rc[h] = rv;
-      }
-    }
-  }
-  // (3) back to r/theta, divergence with zero wall row (div_k, write_bcon_k)
-  for (int i = 0; i < nr; i++) {
-    const size_t h = at(g, i, k);
-    double2 a = R0[h], b = R1[h];
-    to_rt(a, b);
-    R0[h] = a;
-    R1[h] = b;
-  }
-  {
-    const int ip0 = abs((abs(q.m) % 2) - 1) * STEN * nr;
-    for (int i = 0; i < nr - 1; i++) {
-      const size_t h = at(g, i, k);
-      const double rK = A.t.r[i];
-      double2 prod = stencil(g, A.t.D, ip0, i, R0, k);
-      const double2 ur = R0[h], ut = R1[h], uz = R2[h];
-      prod.x += ur.x / rK;
-      prod.y += ur.y / rK;
-      prod.x += -q.kt * ut.y / rK;
-      prod.y += q.kt * ut.x / rK;
-      prod.x += -q.kz * uz.y;
-      prod.y += q.kz * uz.x;
-      A.p[h] = prod;
-    }
-    A.p[at(g, nr - 1, k)] = make_double2(0.0, 0.0);
-  }
-  // (4) pressure Poisson solve, in place
-  lu_solve(g, A.t, A.U, 3, q, A.p, A.p);
-  // (5) rhs -= grad p (addGrad_k), to +/-, zero wall rows
-  {
-    const int ip0 = abs((abs(q.m) % 2) - 0) * STEN * nr;
-    for (int i = 0; i < nr; i++) {
-      const size_t h = at(g, i, k);
-      const double rK = A.t.r[i];
-      const double2 prod = stencil(g, A.t.D, ip0, i, A.p, k);
-      const double2 ph = A.p[h];
-      double2 a = R0[h], b = R1[h], c = R2[h];
-      a.x -= prod.x;
-      a.y -= prod.y;
-      b.x -= -q.kt * ph.y / rK;
-      b.y -= q.kt * ph.x / rK;
-      c.x -= -q.kz * ph.y;
-      c.y -= q.kz * ph.x;
-      to_pm(a, b);
-      if (i == nr - 1) {
-        a = b = c = make_double2(0.0, 0.0);
-      }
-      R0[h] = a;
-      R1[h] = b;
-      R2[h] = c;
-    }
-  }
-  // (6) Helmholtz solves for u+, u-, uz
-  double2* O0 = A.out;
-  double2* O1 = A.out + n;
-  double2* O2 = A.out + 2 * n;
-  lu_solve(g, A.t, A.U, 0, q, R0, O0);
-  lu_solve(g, A.t, A.U, 1, q, R1, O1);
-  lu_solve(g, A.t, A.U, 2, q, R2, O2);
-  // (7) influence-matrix correction, corBCon (mod_boundary.cu:269-278)
-  for (int i = 0; i < nr; i++) {  // decoupleBackward / decoupleForward round trip
-    const size_t h = at(g, i, k);
-    double2 a = O0[h], b = O1[h];
-    to_rt(a, b);
-    O0[h] = a;
-    O1[h] = b;
-  }
-  const double2 dv = div_wall(g, A.t, q, O0, O1, O2);
-  for (int i = 0; i < nr; i++) {
-    const size_t h = at(g, i, k);
-    double2 a = O0[h], b = O1[h];
-    to_pm(a, b);
-    O0[h] = a;
-    O1[h] = b;
-  }
-  {
-    const double2 ndv = make_double2(-dv.x, -dv.y);  // compcoef_k
-    double2 cf[4];
-#pragma unroll
-    for (int j = 0; j < 4; j++) cf[j] = cmul2(ndv, A.b.invc[(size_t)k * 4 + j]);
-    for (int i = 0; i < nr; i++) {  // addBCon_k, then decoupleBackward
-      const size_t h = at(g, i, k);
-      double2 o[3];
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const double2 hh = A.b.uh[c * n + h], hp = A.b.uhP[c * n + h];
-        double2 v = A.out[c * n + h];
-        const double ax = cf[c].x * hh.x - cf[c].y * hh.y + cf[3].x * hp.x - cf[3].y * hp.y;
-        const double ay = cf[c].x * hh.y + cf[c].y * hh.x + cf[3].x * hp.y + cf[3].y * hp.x;
-        v.x += ax;
-        v.y += ay;
-        o[c] = v;
-      }
-      to_rt(o[0], o[1]);
-      O0[h] = o[0];
-      O1[h] = o[1];
-      O2[h] = o[2];
-    }
-  }
-  // (8) mode (0,0): modeZZ_k and adj_flux (mod_integrate.cu:63-68, mod_cublasflux.cu:331-344)
-  if (k == 0) {
-    double qa = 0.0;
-    for (int i = 0; i < nr; i++) {
-      const size_t h = at(g, i, 0);
-      O0[h] = make_double2(0.0, 0.0);
-      O1[h].y = 0.0;
-      O2[h].y = 0.0;
-    }
-    for (int i = 0; i < nr; i++) qa += A.t.rdr[i] * O2[at(g, i, 0)].x;
-    const double corr = (g.qdes - qa) / A.t.qc[0];
-    for (int i = 0; i < nr; i++) O2[at(g, i, 0)].x += corr * A.t.uc[i];
-  }
-  // (9) corrector convergence measure (mcor_k, iden_max)
-  if (!A.predictor) {
-    double num = 0.0, den = 0.0;
-    for (int i = 0; i < nr; i++) {
-      const size_t h = at(g, i, k);
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        const double2 u1 = A.u[c * n + h], u2 = A.out[c * n + h];
-        const double dx = u1.x - u2.x, dy = u1.y - u2.y;
-        const double a1 = dx * dx + dy * dy, a2 = u1.x * u1.x + u1.y * u1.y;
-        // NaN-propagating max: a NaN entry makes the measure NaN
-        num = (a1 > num || a1 != a1) ? a1 : num;
-        den = (a2 > den || a2 != a2) ? a2 : den;
-      }
-    }
-    // non-negative doubles (and NaN) order like their bit patterns
-    atomicMax(&A.norm[0], (unsigned long long)__double_as_longlong(num));
-    atomicMax(&A.norm[1], (unsigned long long)__double_as_longlong(den));
+// Start of a captured step: not converged, no corrector yet.
+__global__ void step_begin_kernel(int* st, double* err) {
+  if (threadIdx.x == 0) {
+    st[0] = 0;
+    st[1] = 0;
+    st[2] = 0;
+    err[0] = 1e98;
   }
 }
 
-__global__ void norm_reset_kernel(unsigned long long* norm) {
-  if (threadIdx.x < 2) norm[threadIdx.x] = 0ull;
+// After corrector iteration `it` (measucorr + the loop test of integrate(),
+// mod_integrate.cu:116-131): err = max |ui - u|^2 / max |ui|^2; the loop
+// goes on while err > tol and it < maxit, so a NaN err also ends it.
+__global__ void converge_kernel(const unsigned long long* norm, int* st, double* err, double tol, int it,
+                                int maxit) {
+  if (threadIdx.x != 0 || st[0]) return;
+  const double e = __longlong_as_double((long long)norm[0]) / __longlong_as_double((long long)norm[1]);
+  err[0] = e;
+  st[1] = it;
+  const bool nan = e != e;
+  if (!(e > tol) || it >= maxit) {
+    st[0] = 1;
+    st[2] = (nan ? 1 : 0) | ((!nan && e > tol) ? 2 : 0);
+  }
 }
 
 // ------------------------------------------------------------------ setup
@@ -527,10 +354,14 @@ __global__ __launch_bounds__(256) void diag_kernel(Geom g, DevTables t, const do
 }
 
 // ------------------------------------------------------------------ launchers
-void launch_linear_sweep(const LinArgs& a, hipStream_t s) {
-  linear_sweep_kernel<<<(a.g.modes + 255) / 256, 256, 0, s>>>(a);
+void launch_norm_reset(unsigned long long* norm, const int* skip, hipStream_t s) {
+  norm_reset_kernel<<<1, 64, 0, s>>>(norm, skip);
 }
-void launch_norm_reset(unsigned long long* norm, hipStream_t s) { norm_reset_kernel<<<1, 64, 0, s>>>(norm); }
+void launch_step_begin(int* st, double* err, hipStream_t s) { step_begin_kernel<<<1, 64, 0, s>>>(st, err); }
+void launch_converge(const unsigned long long* norm, int* st, double* err, double tol, int it, int maxit,
+                     hipStream_t s) {
+  converge_kernel<<<1, 64, 0, s>>>(norm, st, err, tol, it, maxit);
+}
 void launch_setup_influence(const Geom& g, const DevTables& t, const DevInfluence& b, double2* P, double* U,
                             double* PH, hipStream_t s) {
   setup_influence_kernel<<<(g.modes + 255) / 256, 256, 0, s>>>(g, t, b, P, U, PH);

@@ -203,6 +203,7 @@ __global__ __launch_bounds__(256) void factor_kernel(Geom g, const double* __res
 template <int PASS, int PRED>
 __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A) {
   const Geom& g = A.g;
+  if (step_done(g)) return;
   const int nch = (g.nz + 63) / 64;
   const int pair = blockIdx.x / nch, ch = blockIdx.x - pair * nch;
   const int nth = g.nt / 2;

@@ -35,6 +35,7 @@ __device__ __forceinline__ int m_of(int it, int nt) { return it < nt / 2 ? it : 
 __global__ __launch_bounds__(256) void dr3_kernel(Geom g, const double* __restrict__ D,
                                                   const double2* __restrict__ u,
                                                   double2* __restrict__ dR) {
+  if (step_done(g)) return;
   constexpr int NB = 12, PF = 5;  // ring slots; levels i-3 .. i+3+PF live at step i
   static_assert(STEN + PF <= NB, "ring too small");
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,6 +84,7 @@ __global__ __launch_bounds__(256) void theta_inv_kernel(Geom g, FftPlan P, const
                                                         const double2* __restrict__ u,
                                                         const double2* __restrict__ dR,
                                                         double2* __restrict__ W, size_t ws) {
+  if (step_done(g)) return;
   extern __shared__ double2 smem[];
   C* buf = reinterpret_cast<C*>(smem);
   const int l0 = blockIdx.x * TL, i = blockIdx.y;
@@ -132,6 +134,7 @@ __global__ __launch_bounds__(256) void zpass_kernel(Geom g, FftPlan P, const int
                                                     const double* __restrict__ r,
                                                     const double2* __restrict__ W, size_t ws,
                                                     double2* __restrict__ Y) {
+  if (step_done(g)) return;
   extern __shared__ double2 smem[];
   C* buf = reinterpret_cast<C*>(smem);
   const int row = blockIdx.x;
@@ -235,6 +238,7 @@ __global__ __launch_bounds__(256) void theta_fwd_kernel(Geom g, FftPlan P, const
                                                         const double* __restrict__ D,
                                                         const double2* __restrict__ u,
                                                         double2* __restrict__ N) {
+  if (step_done(g)) return;
   extern __shared__ double2 smem[];
   C* buf = reinterpret_cast<C*>(smem);
   const int l0 = blockIdx.x * TL, i = blockIdx.y;

@@ -54,6 +54,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
                                                      const double2* __restrict__ u,
                                                      const double2* __restrict__ dR,
                                                      cpx<T>* __restrict__ W, size_t ws, int ord) {
+  if (step_done(g)) return;
   constexpr int NTP = PL::L;
   constexpr int NT = 2 * NTP / 3;
   constexpr int EPT = (NT * TL + NTHR - 1) / NTHR;
@@ -119,6 +120,7 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
                                                  const double* __restrict__ r,
                                                  const cpx<T>* __restrict__ W, size_t ws,
                                                  cpx<T>* __restrict__ Y) {
+  if (step_done(g)) return;
   constexpr int NH = PL::L;
   constexpr int NZF = 2 * NH;
   __shared__ cpx<T> buf[LY::size(NH)];
@@ -241,6 +243,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
                                                   const double* __restrict__ r,
                                                   const cpx<T>* __restrict__ W, size_t ws,
                                                   cpx<T>* __restrict__ Y) {
+  if (step_done(g)) return;
   constexpr int NH = PL::L;
   constexpr int R0 = PL::rad[0];
   constexpr int NB1 = NH / R0;
@@ -390,6 +393,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
                                                      const double* __restrict__ D,
                                                      const double2* __restrict__ u,
                                                      double2* __restrict__ N, int ord) {
+  if (step_done(g)) return;
   constexpr int NTP = PL::L;
   __shared__ cpx<T> buf[LY::size(NTP * TL)];
   int i, l0;
@@ -445,6 +449,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
                                                       const double2* __restrict__ u,
                                                       const double2* __restrict__ dR,
                                                       cpx<T>* __restrict__ W, size_t ws, int ord) {
+  if (step_done(g)) return;
   constexpr int NTP = PL::L;
   constexpr int NT = 2 * NTP / 3;
   constexpr int half = NT / 2;
@@ -507,6 +512,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
                                                       const double* __restrict__ D,
                                                       const double2* __restrict__ u,
                                                       double2* __restrict__ N, int ord) {
+  if (step_done(g)) return;
   constexpr int NTP = PL::L;
   constexpr int NT = 2 * NTP / 3;
   constexpr int half = NT / 2;

@@ -43,7 +43,12 @@ struct Geom {
   size_t n;                // NR*NT*NZ (field stride)
   double kz0;              // 2*pi/LZ as the reference forms it (PI2/LZ)
   double re, dt, d_im, qdes;
+  // Corrector launches of a captured time step: nonzero *skip means the step
+  // has already converged and the launch returns at once.  Null elsewhere.
+  const int* skip;
 };
+
+__device__ __forceinline__ bool step_done(const Geom& g) { return g.skip != nullptr && *g.skip != 0; }
 
 // Device tables of one pipe.
 struct DevTables {
@@ -63,23 +68,6 @@ struct DevInfluence {
   double2* uh;    // [3][N] homogeneous velocity solutions
   double2* uhP;   // [3][N] gradient of the homogeneous pressure solution
   double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
-};
-
-// Per-mode linear sweep (predictor / corrector) argument block.
-struct LinArgs {
-  Geom g;
-  DevTables t;
-  DevInfluence b;
-  const double2* N;     // [3] nonlinear term of the current iterate, r/theta form
-  double2* Nw;          // [3] nonlinear term of u^n, +/- form (rhsw)
-  const double2* u;     // [3] predictor: u^n (r/theta); corrector: u^k (norm)
-  double2* uw;          // [3] u^n in +/- form
-  double2* out;         // [3] new iterate (r/theta)
-  double2* rhs;         // [3] scratch rhs
-  double2* p;           // [1] scratch pressure / LU vector
-  double* U;            // scratch U factor [NR*4][MODES]
-  unsigned long long* norm;  // [2] max |u^k - u^{k+1}|^2, max |u^k|^2 (bits)
-  int predictor;
 };
 
 // Streaming linear sweep (linear2.hip) argument block.
@@ -159,46 +147,61 @@ enum KernelId { K_DR3 = 0, K_THETA_INV, K_ZPASS, K_THETA_FWD, K_LIN_PRED, K_LIN_
 const char* kernel_name(int id);
 
 struct Profiler {
+  struct Pair {
+    hipEvent_t b, e;
+    int id;
+  };
   bool on = false;
-  std::vector<hipEvent_t> pool;
-  std::vector<int> ids;
-  int used = 0;
+  std::vector<Pair> pending;  // recorded since the last collect
+  std::vector<hipEvent_t> spare;
   double total_ms[K_COUNT] = {};
   long long launches[K_COUNT] = {};
   void reset() {
     for (int q = 0; q < K_COUNT; q++) { total_ms[q] = 0.0; launches[q] = 0; }
-    used = 0;
-    ids.clear();
+    recycle();
+  }
+  hipEvent_t get() {
+    hipEvent_t e = nullptr;
+    if (!spare.empty()) {
+      e = spare.back();
+      spare.pop_back();
+    } else if (hipEventCreate(&e) != hipSuccess) {
+      e = nullptr;
+    }
+    return e;
   }
   void begin(int id, hipStream_t s) {
     if (!on) return;
-    while ((int)pool.size() < used + 2) {
-      hipEvent_t e;
-      if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
-      pool.push_back(e);
-    }
-    ids.push_back(id);
-    (void)hipEventRecord(pool[used], s);
+    Pair p{get(), get(), id};
+    if (!p.b || !p.e) { on = false; return; }
+    (void)hipEventRecord(p.b, s);
+    pending.push_back(p);
   }
   void end(hipStream_t s) {
-    if (!on) return;
-    (void)hipEventRecord(pool[used + 1], s);
-    used += 2;
+    if (!on || pending.empty()) return;
+    (void)hipEventRecord(pending.back().e, s);
+  }
+  void recycle() {
+    for (const Pair& p : pending) {
+      spare.push_back(p.b);
+      spare.push_back(p.e);
+    }
+    pending.clear();
   }
   // call after the stream has been synchronised
   void collect() {
-    for (int q = 0; q < used / 2; q++) {
+    for (const Pair& p : pending) {
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, pool[2 * q], pool[2 * q + 1]) == hipSuccess) {
-        total_ms[ids[q]] += ms;
-        launches[ids[q]] += 1;
+      if (hipEventElapsedTime(&ms, p.b, p.e) == hipSuccess) {
+        total_ms[p.id] += ms;
+        launches[p.id] += 1;
       }
     }
-    used = 0;
-    ids.clear();
+    recycle();
   }
   ~Profiler() {
-    for (auto e : pool) (void)hipEventDestroy(e);
+    recycle();
+    for (auto e : spare) (void)hipEventDestroy(e);
   }
 };
 
@@ -223,8 +226,10 @@ void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, h
 // Launchers (HIP sources).
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u,
                       double2* N, hipStream_t s, Profiler* pf);
-void launch_linear_sweep(const LinArgs& a, hipStream_t s);
-void launch_norm_reset(unsigned long long* norm, hipStream_t s);
+void launch_norm_reset(unsigned long long* norm, const int* skip, hipStream_t s);
+// Step bookkeeping of a captured step: st = {converged, corrector iterations, flags}.
+void launch_step_begin(int* st, double* err, hipStream_t s);
+void launch_converge(const unsigned long long* norm, int* st, double* err, double tol, int it, int maxit, hipStream_t s);
 void launch_setup_influence(const Geom& g, const DevTables& t, const DevInfluence& b,
                             double2* scratch_p, double* scratch_U, double* PH, hipStream_t s);
 void launch_diagnostics(const Geom& g, const DevTables& t, const double2* u, double* partial,

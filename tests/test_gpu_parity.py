@@ -231,6 +231,41 @@ def test_set_tuning_rejects_missing_variant():
     p.close()
 
 
+def test_graph_and_eager_steps_agree(monkeypatch):
+    """The captured step (default) and the launch-by-launch step with a host
+    convergence test give identical bits and iteration counts."""
+    p1 = nsp.Pipe(**CFG1)
+    monkeypatch.setenv("NSP_NO_GRAPH", "1")
+    p2 = nsp.Pipe(**CFG1)
+    for p in (p1, p2):
+        p.seeded_ic(SEED1, 4, 4, 0, 1e-3)
+    n1, e1, f1 = p1.step(5)
+    n2, e2, f2 = p2.step(5)
+    assert np.array_equal(n1, n2) and np.array_equal(f1, f2)
+    assert np.array_equal(e1, e2)
+    a, b = p1.get_state(), p2.get_state()
+    for c in range(3):
+        assert np.array_equal(a[c], b[c])
+    p1.close()
+    p2.close()
+
+
+def test_profile_counts_executed_launches():
+    """Corrector iterations skipped inside a captured step are not counted."""
+    p = nsp.Pipe(**CFG1)
+    p.seeded_ic(SEED1, 4, 4, 0, 1e-3)
+    p.profile_enable(True)
+    n_it, _, _ = p.step(3)
+    prof = p.profile_read()
+    assert prof["dr3_kernel"][1] == int(np.sum(1 + n_it))
+    assert prof["linear_sweep_kernel(predictor)"][1] == 3
+    assert prof["linear_sweep_kernel(corrector)"][1] == int(np.sum(n_it))
+    for name, (ms, cnt) in prof.items():
+        if cnt:
+            assert ms > 0, name
+    p.close()
+
+
 def test_mixed_nonlinear_accuracy():
     """FP32 transforms and products: relative L2 error of N(u) at FP32 level."""
     shape = COMPILED[3]

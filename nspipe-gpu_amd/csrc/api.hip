@@ -169,6 +169,7 @@ Lin2Args lin_args(nsp_handle* h, const Geom& g, const double2* N, const double2*
   a.PH = h->PH;
   a.invc = h->b.invc;
   a.N = N;
+  a.n32 = h->nb.mixed ? 1 : 0;
   a.E = h->E;
   a.u = u;
   a.out = out;
@@ -448,6 +449,11 @@ nsp_status nsp_nonlinear(nsp_handle* h, const double* ur, const double* ut, cons
   if (st) return st;
   launch_nonlinear(h->g, h->t, h->nb, h->ui, h->Nl, h->stream, nullptr);
   HIPCHK(hipGetLastError());
+  if (h->nb.mixed) {  // N(u) is stored as FP32 pairs: widen into the sweep scratch
+    launch_widen(reinterpret_cast<const float2*>(h->Nl), h->SR, 3 * h->g.n, h->stream);
+    HIPCHK(hipGetLastError());
+    return fetch_fields(h, h->SR, dr, dt, dz, where);
+  }
   return fetch_fields(h, h->Nl, dr, dt, dz, where);
 }
 

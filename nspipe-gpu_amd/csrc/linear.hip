@@ -175,6 +175,15 @@ __global__ void norm_reset_kernel(unsigned long long* norm, const int* skip) {
   if (threadIdx.x < 2) norm[threadIdx.x] = 0ull;
 }
 
+// FP32 pairs to FP64 pairs (the mixed form's stored N(u), for nsp_nonlinear).
+__global__ void widen_kernel(const float2* __restrict__ in, double2* __restrict__ out, size_t count) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < count) {
+    const float2 v = in[q];
+    out[q] = make_double2(v.x, v.y);
+  }
+}
+
 // Start of a captured step: not converged, no corrector yet.
 __global__ void step_begin_kernel(int* st, double* err) {
   if (threadIdx.x == 0) {
@@ -356,6 +365,9 @@ __global__ __launch_bounds__(256) void diag_kernel(Geom g, DevTables t, const do
 // ------------------------------------------------------------------ launchers
 void launch_norm_reset(unsigned long long* norm, const int* skip, hipStream_t s) {
   norm_reset_kernel<<<1, 64, 0, s>>>(norm, skip);
+}
+void launch_widen(const float2* in, double2* out, size_t count, hipStream_t s) {
+  widen_kernel<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(in, out, count);
 }
 void launch_step_begin(int* st, double* err, hipStream_t s) { step_begin_kernel<<<1, 64, 0, s>>>(st, err); }
 void launch_converge(const unsigned long long* norm, int* st, double* err, double tol, int it, int maxit,

@@ -270,8 +270,18 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A) {
       double2 R0 = z2, R1 = z2, R2 = z2;
       if (j >= 0 && j < nr) {
         const size_t h = (size_t)j * MS + k;
-        double2 n0 = A.N[h], n1 = A.N[n + h];
-        const double2 n2 = A.N[2 * n + h];
+        double2 n0, n1, n2;
+        if (A.n32) {
+          const float2* N32 = reinterpret_cast<const float2*>(A.N);
+          const float2 f0 = N32[h], f1 = N32[n + h], f2 = N32[2 * n + h];
+          n0 = make_double2(f0.x, f0.y);
+          n1 = make_double2(f1.x, f1.y);
+          n2 = make_double2(f2.x, f2.y);
+        } else {
+          n0 = A.N[h];
+          n1 = A.N[n + h];
+          n2 = A.N[2 * n + h];
+        }
         pm_of(n0, n1);
         double2 rh[3];
         if (PRED) {

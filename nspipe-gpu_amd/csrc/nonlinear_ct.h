@@ -426,7 +426,8 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
             du += D[STEN * (g.nr - 1) + j2 + IW] * u[2 * g.n + (size_t)(g.nr - 1 - IW + j2) * g.modes].x;
           vx -= (2.0 / g.re) * du;
         }
-        N[c * g.n + ((size_t)i * nt + it) * nz + l] = make_double2(vx, vy);
+        // N(u) is stored at the transform precision (FP32 in the mixed form)
+        reinterpret_cast<cpx<T>*>(N)[c * g.n + ((size_t)i * nt + it) * nz + l] = cpx<T>{T(vx), T(vy)};
       }
     }
     __syncthreads();
@@ -545,7 +546,8 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
       }
     }
     __syncthreads();
-    double2* dst = N + c * g.n + (size_t)i * g.modes;
+    // N(u) is stored at the transform precision (FP32 in the mixed form)
+    cpx<T>* dst = reinterpret_cast<cpx<T>*>(N) + c * g.n + (size_t)i * g.modes;
     fft_rest_emit<T, TL, NTHR, -1, LY>(PL{}, buf, tw, lt, [&](int col, int j, cpx<T> w) {
       const int l = l0 + col;
       if (l < nz && (j < half || j >= NT)) {
@@ -558,7 +560,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
             du += D[STEN * (g.nr - 1) + j2 + IW] * u[2 * g.n + (size_t)(g.nr - 1 - IW + j2) * g.modes].x;
           vx -= (2.0 / g.re) * du;
         }
-        dst[(size_t)it * nz + l] = make_double2(vx, vy);
+        dst[(size_t)it * nz + l] = cpx<T>{T(vx), T(vy)};
       }
     });
     __syncthreads();

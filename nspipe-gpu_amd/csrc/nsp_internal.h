@@ -84,7 +84,8 @@ struct Lin2Args {
   const double* F;      // [4][NR][7][MODES] banded LU factors
   const double* PH;     // [NR][2][MODES] dr p_h, p_h of the homogeneous pressure solution
   const double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
-  const double2* N;     // [3] nonlinear term (r/theta)
+  const double2* N;     // [3] nonlinear term (r/theta); float2 storage when n32
+  int n32;              // mixed form: N(u) stored as FP32 pairs
   double2* E;           // [3] explicit part (+/-): written by the predictor
   const double2* u;     // [3] predictor: u^n; corrector: u^k (r/theta)
   double2* out;         // [3] new iterate (r/theta)
@@ -227,6 +228,7 @@ void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, h
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u,
                       double2* N, hipStream_t s, Profiler* pf);
 void launch_norm_reset(unsigned long long* norm, const int* skip, hipStream_t s);
+void launch_widen(const float2* in, double2* out, size_t count, hipStream_t s);
 // Step bookkeeping of a captured step: st = {converged, corrector iterations, flags}.
 void launch_step_begin(int* st, double* err, hipStream_t s);
 void launch_converge(const unsigned long long* norm, int* st, double* err, double tol, int it, int maxit, hipStream_t s);

@@ -495,7 +495,7 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
   HIPCHK(hipEventCreate(&e1));
   auto stage = [&](int which, const NlVariant& v) -> bool {
     if (which == 0) {
-      launch_dr3(g, h->t.D, h->u, nb.dR, s);
+      launch_dr3(g, h->t.D, h->u, nb.dR, nb.mixed, s);
       return nb.mixed ? ct_theta_inv<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, h->t.r, h->u, nb.dR,
                                             (cpx<float>*)nb.W32, nb.wstride, s)
                       : ct_theta_inv<double>(v, g, (const cpx<double>*)nb.ct_tw_t, h->t.r, h->u, nb.dR,

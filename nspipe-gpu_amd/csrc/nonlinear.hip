@@ -32,9 +32,10 @@ __device__ __forceinline__ int m_of(int it, int nt) { return it < nt / 2 ? it : 
 // level of the three components does not fit in L2).  Levels up to PF ahead
 // are loaded early so each thread keeps several loads in flight.  The sum
 // runs over the same terms in the same order as derivr_k.
+template <typename OT>
 __global__ __launch_bounds__(256) void dr3_kernel(Geom g, const double* __restrict__ D,
                                                   const double2* __restrict__ u,
-                                                  double2* __restrict__ dR) {
+                                                  OT* __restrict__ dR) {
   if (step_done(g)) return;
   constexpr int NB = 12, PF = 5;  // ring slots; levels i-3 .. i+3+PF live at step i
   static_assert(STEN + PF <= NB, "ring too small");
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(256) void dr3_kernel(Geom g, const double* __restri
   const int m = m_of(k / g.nz, g.nt);
   const double* Dp = D + abs((abs(m) % 2) - (c < 2 ? 1 : 0)) * STEN * nr;
   const double2* src = u + (size_t)c * g.n + k;
-  double2* dst = dR + (size_t)c * g.n + k;
+  OT* dst = dR + (size_t)c * g.n + k;
   const size_t ld = g.modes;
   double2 w[NB];  // level L in slot L % NB
 #pragma unroll
@@ -70,7 +71,7 @@ __global__ __launch_bounds__(256) void dr3_kernel(Geom g, const double* __restri
             py += de * a.y;
           }
         }
-        dst[i * ld] = make_double2(px, py);
+        dst[i * ld] = OT{(decltype(OT::x))px, (decltype(OT::x))py};
       }
     }
   }
@@ -331,9 +332,12 @@ const char* kernel_name(int id) {
   return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 
-void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, hipStream_t s) {
+void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, bool f32, hipStream_t s) {
   dim3 gd((g.modes + 255) / 256, 3);
-  dr3_kernel<<<gd, 256, 0, s>>>(g, D, u, dR);
+  if (f32)  // mixed form: d_r u only feeds the FP32 theta transforms
+    dr3_kernel<float2><<<gd, 256, 0, s>>>(g, D, u, reinterpret_cast<float2*>(dR));
+  else
+    dr3_kernel<double2><<<gd, 256, 0, s>>>(g, D, u, dR);
 }
 
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u, double2* N,
@@ -342,7 +346,7 @@ void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const
     const NlVariant& v = nb.var;
     bool ok = true;
     if (pf) pf->begin(K_DR3, s);
-    launch_dr3(g, t.D, u, nb.dR, s);
+    launch_dr3(g, t.D, u, nb.dR, nb.mixed, s);
     if (pf) pf->end(s);
     if (pf) pf->begin(K_THETA_INV, s);
     if (nb.mixed)
@@ -372,7 +376,7 @@ void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const
     return;
   }
   if (pf) pf->begin(K_DR3, s);
-  launch_dr3(g, t.D, u, nb.dR, s);
+  launch_dr3(g, t.D, u, nb.dR, false, s);
   if (pf) pf->end(s);
   if (pf) pf->begin(K_THETA_INV, s);
   theta_dispatch(g, t, nb, u, N, s, true);

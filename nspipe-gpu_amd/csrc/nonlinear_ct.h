@@ -47,12 +47,13 @@ __device__ __forceinline__ void theta_block(const Geom& g, int TL, int ord, int&
 // ------------------------------------------------------------------ theta inverse
 // W: 9 intermediates [f][NR][NTP][NZ] in precision T: u, d_r u, d_theta u / r.
 // The radial derivative (derivr_k, mod_radialFD.cu:10-30, flags 1,1,0) comes
-// from dR (dr3_kernel); forming it here from seven levels was slower.
+// from dR (dr3_kernel, stored at precision T); forming it here from seven
+// levels was slower.
 template <typename T, int TL, int NTHR, typename LY, typename PL>
 __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                      const double* __restrict__ r,
                                                      const double2* __restrict__ u,
-                                                     const double2* __restrict__ dR,
+                                                     const cpx<T>* __restrict__ dR,
                                                      cpx<T>* __restrict__ W, size_t ws, int ord) {
   if (step_done(g)) return;
   constexpr int NTP = PL::L;
@@ -67,12 +68,14 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
   constexpr int half = NT / 2;
   for (int c = 0; c < 3; c++) {
     const double2* src = u + c * g.n;
-    double2 U[EPT], DR[EPT];
+    double2 U[EPT];
+    cpx<T> DR[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
       const int e = tid + q * NTHR;
       const int it = e / TL, tl = e - it * TL, l = l0 + tl;
-      U[q] = DR[q] = make_double2(0.0, 0.0);
+      U[q] = make_double2(0.0, 0.0);
+      DR[q] = cpx<T>{T(0), T(0)};
       if (e < NT * TL && l < nz) {
         const size_t off = (size_t)i * g.modes + (size_t)it * nz + l;
         U[q] = src[off];
@@ -91,8 +94,8 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
         const int e = lt + q * NTHR;
         if (e < NT * TL) {
           const int it = e / TL, tl = e - it * TL;
-          const double2 v = (kind == 1) ? DR[q] : U[q];
-          cpx<T> w{T(v.x), T(v.y)};
+          const double2 v = U[q];
+          cpx<T> w = (kind == 1) ? DR[q] : cpx<T>{T(v.x), T(v.y)};
           if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
             const double kt = double(it < half ? it : it - NT);
             w = cpx<T>{T(-kt * v.y / ri), T(kt * v.x / ri)};
@@ -448,7 +451,7 @@ template <typename T, int TL, int NTHR, typename LY, typename PL>
 __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                       const double* __restrict__ r,
                                                       const double2* __restrict__ u,
-                                                      const double2* __restrict__ dR,
+                                                      const cpx<T>* __restrict__ dR,
                                                       cpx<T>* __restrict__ W, size_t ws, int ord) {
   if (step_done(g)) return;
   constexpr int NTP = PL::L;
@@ -469,7 +472,8 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
     const int c = f % 3, kind = f / 3;  // kind 0: u, 1: d_r u, 2: d_theta u / r
     const int lt = opaque(tid);
     {
-      const double2* src = (kind == 1 ? dR : u) + c * g.n + (size_t)i * g.modes;
+      const double2* src = u + c * g.n + (size_t)i * g.modes;
+      const cpx<T>* srd = dR + c * g.n + (size_t)i * g.modes;
 #pragma unroll
       for (int s = 0; s < PPT; s++) {
         const int p = lt + s * NTHR;
@@ -482,12 +486,16 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
             const int it = j < half ? j : (j >= NT ? j - half : -1);
             v[q] = cpx<T>{T(0), T(0)};
             if (it >= 0 && l < nz) {
-              const double2 a = src[(size_t)it * nz + l];
-              if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
-                const double kt = double(it < half ? it : it - NT);
-                v[q] = cpx<T>{T(-kt * a.y / ri), T(kt * a.x / ri)};
+              if (kind == 1) {
+                v[q] = srd[(size_t)it * nz + l];
               } else {
-                v[q] = cpx<T>{T(a.x), T(a.y)};
+                const double2 a = src[(size_t)it * nz + l];
+                if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
+                  const double kt = double(it < half ? it : it - NT);
+                  v[q] = cpx<T>{T(-kt * a.y / ri), T(kt * a.x / ri)};
+                } else {
+                  v[q] = cpx<T>{T(a.x), T(a.y)};
+                }
               }
             }
           }
@@ -595,7 +603,8 @@ using ThetaRowPad = Lay<(PL::L / PL::rad[PL::P - 1]) * TL, TL>;
 
 template <typename T, typename PL>
 bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
-                    const double2* dR, cpx<T>* W, size_t ws, hipStream_t s) {
+                    const double2* dR_, cpx<T>* W, size_t ws, hipStream_t s) {
+  const cpx<T>* dR = reinterpret_cast<const cpx<T>*>(dR_);  // stored at precision T
   const int tl = v.tl;
   if (PL::L * tl * (int)sizeof(cpx<T>) > 163840) return false;
   const int nb = ((g.nz + tl - 1) / tl) * g.nr;

@@ -222,7 +222,8 @@ bool ct_theta_fwd(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx
 template <typename T>
 bool ct_zpass(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* twz, const double* r,
               const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s);
-void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, hipStream_t s);
+// dR is stored as FP32 pairs when f32 (the mixed form)
+void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, bool f32, hipStream_t s);
 
 // Launchers (HIP sources).
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u,

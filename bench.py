@@ -95,6 +95,40 @@ def run_gpu(nsp, precision, steps, warmup, cfg, chosen=None):
                 timed_s=secs, tuning=tuning, events_steps_per_s=steps / (t3 - t2), events_n_it=n_it2.tolist())
 
 
+CONFIG4A = dict(nr=48, nt=96, nz=513, lz=50.0, re=2000.0, dt=0.005)
+CONFIG4B = dict(nr=64, nt=128, nz=1025, lz=50.0, re=2000.0, dt=0.005)
+SEED4 = 20261202 + 4 * 1000
+
+
+def run_ensemble(nsp, precision, steps, warmup, per_class=32):
+    """Ensemble for predictability: 32 pipes at 48x96x513 and 32 at 64x128x1025
+    (Re 2000, L = 50R, dt 0.005) stepped together by one nsp_ensemble.  Every
+    member starts from the same seeded localized puff (Gaussian axial window of
+    sigma 2.5R, i.e. width 5R, with rms 0.5 U_bulk inside the window) plus a
+    member-specific seeded perturbation of rms 1e-6 U_bulk."""
+    ms = [dict(CONFIG4A, precision=precision)] * per_class + [dict(CONFIG4B, precision=precision)] * per_class
+    ens = nsp.Ensemble(ms)
+    # domain rms = window rms * sqrt(mean of the squared window) = sqrt(sqrt(pi) sigma / L)
+    puff_rms = 0.5 * 0.5 * math.sqrt(math.sqrt(math.pi) * 2.5 / CONFIG4A["lz"])
+    for j, p in enumerate(ens.members):
+        p.seeded_ic(SEED4, 4, 16, nsp.IC_PUFF, puff_rms)
+        p.seeded_ic(SEED4 + 1 + j, 4, 4, nsp.IC_FLAT | nsp.IC_ADD, 1e-6 * 0.5)
+    ens.autotune(1)
+    if warmup:
+        ens.step(warmup)
+    t0 = time.perf_counter()
+    n_it, err, flags = ens.step(steps)
+    t1 = time.perf_counter()
+    ens.close()
+    secs = t1 - t0
+    return dict(members=len(ms), member_steps_per_s=round(len(ms) * steps / secs, 2),
+                ensemble_steps_per_s=round(steps / secs, 4), steps=steps, n_it_mean=round(float(n_it.mean()), 3),
+                flags=int(flags.max()),
+                config="32 x (NR=48, NT=96, NZ=513) + 32 x (NR=64, NT=128, NZ=1025), Re 2000, L 50R, dt 0.005",
+                ic=f"puff seed {SEED4} (window rms 0.5 U_bulk, width 5R) + member j perturbation seed "
+                   f"{SEED4}+1+j (rms 1e-6 U_bulk)")
+
+
 def roofline(res, cfg, fp64, traffic):
     kern = {k: v for k, v in res["kernels"].items() if v[1] > 0}
     name = max(kern, key=lambda k: kern[k][0])
@@ -179,6 +213,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the oracle CPU baseline")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run 3 device-to-device copies of BYTES (counter calibration)")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the config-4 ensemble measurement")
+    ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--tuning", metavar="FILE", help="apply saved kernel variants instead of autotuning")
     ap.add_argument("--save-tuning", metavar="FILE", help="write the chosen kernel variants")
     args = ap.parse_args()
@@ -245,6 +281,12 @@ def main():
             out[f] = rr
     if "steps_per_s" in results.get("mixed", {}) and "steps_per_s" in results.get("fp64", {}):
         out["mixed_over_fp64"] = round(results["mixed"]["steps_per_s"] / results["fp64"]["steps_per_s"], 4)
+    if not args.no_ensemble:
+        try:
+            out["ensemble"] = run_ensemble(nsp, nsp.MIXED if head == "mixed" else nsp.FP64, args.ensemble_steps, 1)
+            out["ensemble"]["form"] = head
+        except nsp.NspError as e:
+            out["ensemble"] = dict(error=str(e))
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(cfg, n_it_mean)

@@ -132,8 +132,13 @@ nsp_status nsp_write_state_files(nsp_handle* h, const char* ur_path, const char*
  * to bulk flux qdes, plus an analytically divergence-free, no-slip
  * perturbation on modes |m| <= mmax, 0 <= l <= lmax with complex Gaussian
  * amplitudes (mt19937_64(seed), Box-Muller) times the envelope
- * (kind 0: flat, kind 1: |k|^(-5/3) with |k|^2 = kz^2 + m^2), rescaled so the
- * perturbation rms velocity is `rms`.  Written into the state. */
+ * (kind 0: flat, kind 1: |k|^(-5/3) with |k|^2 = kz^2 + m^2; kind 2: a
+ * localized puff, one amplitude per m shared by all l with the spectrum of a
+ * Gaussian axial window of width 5R centred at LZ/2), rescaled so the
+ * perturbation rms velocity is `rms`.  Written into the state; with
+ * kind | NSP_IC_ADD the perturbation alone (no base flow) is added to the
+ * current state (e.g. member-specific perturbations of an ensemble). */
+#define NSP_IC_ADD 16
 nsp_status nsp_seeded_ic(nsp_handle* h, unsigned long long seed, int mmax, int lmax, int kind,
                          double rms);
 
@@ -163,6 +168,27 @@ const char* nsp_tuning(nsp_handle* h);
  * compiled for this handle's sizes.  Runs one nonlinear evaluation of the
  * current state as the check. */
 nsp_status nsp_set_tuning(nsp_handle* h, const char* json);
+
+/* Ensembles of independent pipes (no reference counterpart: the reference runs
+ * one pipe per process).  nsp_ensemble_create builds count pipes from
+ * params[0..count-1] (sizes may differ; every member needs compiled transform
+ * plans, and all share maxit and tol).  Members with identical parameters
+ * share their operator data (LU factors, influence data).  nsp_ensemble_step
+ * advances every member nsteps time steps: each stage is one launch per class
+ * of members with the same transform plans (one for all members in the radial
+ * derivative and the solves), the whole step one hipGraph; members converge
+ * independently.  info, if not NULL, receives nsteps*count entries, member
+ * fastest.  nsp_ensemble_member returns a member's handle for the per-pipe
+ * calls (state, ICs, diagnostics, restart files); it is owned by the
+ * ensemble.  nsp_ensemble_autotune tunes one member per class and applies
+ * the choice to the class. */
+typedef struct nsp_ensemble nsp_ensemble;
+nsp_status nsp_ensemble_create(const nsp_params* params, int count, int device, nsp_ensemble** out);
+nsp_status nsp_ensemble_destroy(nsp_ensemble* e);
+int nsp_ensemble_size(nsp_ensemble* e);
+nsp_handle* nsp_ensemble_member(nsp_ensemble* e, int i);
+nsp_status nsp_ensemble_autotune(nsp_ensemble* e, int reps);
+nsp_status nsp_ensemble_step(nsp_ensemble* e, int nsteps, nsp_step_info* info);
 
 /* HIP stream of the handle (a hipStream_t), for callers that order their own
  * work against it. */

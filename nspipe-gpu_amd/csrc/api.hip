@@ -25,13 +25,6 @@ static nsp_status fail(nsp_status st, const std::string& msg) {
       return fail(NSP_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
   } while (0)
 
-// Outcome of one captured step, written on the device and copied back by the
-// graph: converged flag, corrector iterations, flags; final err.
-struct StepRec {
-  int st[4];
-  double err;
-};
-
 struct nsp_handle {
   nsp_params P;
   Geom g;
@@ -250,7 +243,10 @@ nsp_status nsp_default_params(nsp_params* p, int nr, int nt, int nz, double lz, 
   return NSP_OK;
 }
 
-nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
+// share: a handle with identical parameters whose operator data (LU factors,
+// pressure homogeneous solution, influence-matrix column, flux profile) this
+// one uses instead of computing its own; it must outlive this handle.
+static nsp_status create_handle(const nsp_params* p, int device, const nsp_handle* share, nsp_handle** out) {
   if (!out) return fail(NSP_ERR_INVALID, "null handle pointer");
   *out = nullptr;
   nsp_status st = check_params(p);
@@ -306,8 +302,13 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   TRY(upload(h, &h->t.rdr, h->host.rdr));
   TRY(upload(h, &h->t.dr0, h->host.dr0));
   TRY(upload(h, &h->t.dr1, h->host.dr1));
-  TRY(dalloc(h, &h->t.uc, g.nr));
-  TRY(dalloc(h, &h->t.qc, 1));
+  if (share) {
+    h->t.uc = share->t.uc;
+    h->t.qc = share->t.qc;
+  } else {
+    TRY(dalloc(h, &h->t.uc, g.nr));
+    TRY(dalloc(h, &h->t.qc, 1));
+  }
 
   // FFT plans
   std::vector<int> perm_t, perm_h;
@@ -362,11 +363,19 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   TRY(dalloc(h, &h->E, 3 * n));
   TRY(dalloc(h, &h->SR, 3 * n));
   TRY(dalloc(h, &h->SP, n));
-  TRY(dalloc(h, &h->F, 4 * 7 * n));
-  TRY(dalloc(h, &h->PH, 2 * n));
+  if (share) {
+    h->F = share->F;
+    h->PH = share->PH;
+  } else {
+    TRY(dalloc(h, &h->F, 4 * 7 * n));
+    TRY(dalloc(h, &h->PH, 2 * n));
+  }
   TRY(upload(h, &h->Dc, centred_table(g.nr, h->host.D)));
   TRY(upload(h, &h->Luc, centred_table(g.nr, h->host.Lu)));
-  TRY(dalloc(h, &h->b.invc, 4 * size_t(g.modes)));
+  if (share)
+    h->b.invc = share->b.invc;
+  else
+    TRY(dalloc(h, &h->b.invc, 4 * size_t(g.modes)));
   TRY(dalloc(h, &h->nb.dR, 3 * n));
   if (h->nb.mixed) {
     TRY(dalloc(h, &h->nb.W32, 9 * h->nb.wstride));
@@ -383,7 +392,9 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
   // influence matrix, homogeneous solutions and flux profile (setBCon,
   // setCublasFlux) with the reference's solver order; its full-field
   // homogeneous solutions are only needed here
-  {
+  if (share) {
+    HTRY(hipStreamSynchronize(h->stream));
+  } else {
     double2 *uh = nullptr, *uhP = nullptr, *P = nullptr;
     double* U = nullptr;
     hipError_t e1 = hipMalloc(&uh, 3 * n * sizeof(double2));
@@ -409,6 +420,10 @@ nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
 #undef HTRY
   *out = h;
   return NSP_OK;
+}
+
+nsp_status nsp_create(const nsp_params* p, int device, nsp_handle** out) {
+  return create_handle(p, device, nullptr, out);
 }
 
 nsp_status nsp_destroy(nsp_handle* h) {
@@ -928,7 +943,9 @@ nsp_status nsp_reference_ic(nsp_handle* h, double a_p) {
 nsp_status nsp_seeded_ic(nsp_handle* h, unsigned long long seed, int mmax, int lmax, int kind, double rms) {
   if (!h) return fail(NSP_ERR_INVALID, "null handle");
   const Geom& g = h->g;
-  if (mmax < 0 || lmax < 0 || kind < 0 || kind > 1) return fail(NSP_ERR_INVALID, "bad IC arguments");
+  const bool add = (kind & NSP_IC_ADD) != 0;
+  kind &= ~NSP_IC_ADD;
+  if (mmax < 0 || lmax < 0 || kind < 0 || kind > 2) return fail(NSP_ERR_INVALID, "bad IC arguments");
   mmax = std::min(mmax, g.nt / 2 - 1);
   lmax = std::min(lmax, g.nz - 2);
   const std::vector<double>& r = h->host.r;
@@ -942,20 +959,53 @@ nsp_status nsp_seeded_ic(nsp_handle* h, unsigned long long seed, int mmax, int l
   };
   std::vector<double2> f(3 * g.n, make_double2(0.0, 0.0));
   auto cm = [](double2 a, double2 b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); };
+  // kind 2 (localized puff): one random amplitude triple per |m|, shared by all
+  // l with the spectrum exp(-(kz sigma)^2 / 2) exp(-i kz LZ/2) of a Gaussian
+  // axial window of width 5R (sigma = 2.5R) centred at z = LZ/2.  The stored
+  // (-m, l > 0) coefficients are the conjugates of (m, -l), so they carry the
+  // conjugate amplitude (real for m = 0): every azimuthal mode is then the
+  // window times a fixed radial/azimuthal shape, localized in z.
+  std::vector<double2> pa(2 * mmax + 1), pb(2 * mmax + 1), pc(2 * mmax + 1);
+  if (kind == 2)
+    for (int m = 0; m <= mmax; m++) {
+      pa[m + mmax] = gauss();
+      pb[m + mmax] = gauss();
+      pc[m + mmax] = gauss();
+      if (m == 0) {
+        pa[mmax].y = pb[mmax].y = pc[mmax].y = 0.0;
+      } else {
+        pa[mmax - m] = make_double2(pa[m + mmax].x, -pa[m + mmax].y);
+        pb[mmax - m] = make_double2(pb[m + mmax].x, -pb[m + mmax].y);
+        pc[mmax - m] = make_double2(pc[m + mmax].x, -pc[m + mmax].y);
+      }
+    }
+  const double sigma = 2.5, lz = PI2 / g.kz0;
   for (int l = 0; l <= lmax; l++) {
     for (int m = -mmax; m <= mmax; m++) {
       if (l == 0 && m < 0) continue;
       const double kz = g.kz0 * double(l);
       const int q = std::abs(m);
-      double env = 1.0;
+      double2 env = make_double2(1.0, 0.0);
       if (kind == 1) {
         const double kk = std::sqrt(kz * kz + double(m * m));
-        env = kk > 0 ? std::pow(kk, -5.0 / 3.0) : 0.0;
+        env.x = kk > 0 ? std::pow(kk, -5.0 / 3.0) : 0.0;
+      } else if (kind == 2) {
+        const double amp = std::exp(-0.5 * kz * kz * sigma * sigma);
+        env = make_double2(amp * std::cos(kz * 0.5 * lz), -amp * std::sin(kz * 0.5 * lz));
       }
-      double2 a = gauss(), b = gauss(), c = gauss();
-      a = make_double2(a.x * env, a.y * env);
-      b = make_double2(b.x * env, b.y * env);
-      c = make_double2(c.x * env, c.y * env);
+      double2 a, b, c;
+      if (kind == 2) {
+        a = pa[m + mmax];
+        b = pb[m + mmax];
+        c = pc[m + mmax];
+      } else {
+        a = gauss();
+        b = gauss();
+        c = gauss();
+      }
+      a = cm(a, env);
+      b = cm(b, env);
+      c = cm(c, env);
       if (l == 0) {  // kz = 0 modes of a real field: keep a real radial structure
         if (m == 0) { a = make_double2(0, 0); b.y = 0.0; c = make_double2(0, 0); }
       }
@@ -999,7 +1049,6 @@ nsp_status nsp_seeded_ic(nsp_handle* h, unsigned long long seed, int mmax, int l
           f[2 * g.n + hc] = make_double2(uz.x, -uz.y);
         }
       }
-      (void)cm;
     }
   }
   // rms of the perturbation: sqrt(sum_modes w_l sum_r rdr |u|^2)
@@ -1014,6 +1063,14 @@ nsp_status nsp_seeded_ic(nsp_handle* h, unsigned long long seed, int mmax, int l
     }
   const double scale = e > 0 ? rms / std::sqrt(e) : 0.0;
   for (auto& v : f) v = make_double2(v.x * scale, v.y * scale);
+  if (add) {  // perturbation only, added to the current state on the device
+    HIPCHK(hipSetDevice(h->device));
+    HIPCHK(hipMemcpyAsync(h->SR, f.data(), 3 * g.n * sizeof(double2), hipMemcpyHostToDevice, h->stream));
+    launch_add(h->u, h->SR, 3 * g.n, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return NSP_OK;
+  }
   // flux-matched Poiseuille
   double q0 = 0.0;
   for (int i = 0; i < g.nr; i++) q0 += rdr[i] * (1 - r[i] * r[i]);
@@ -1023,4 +1080,320 @@ nsp_status nsp_seeded_ic(nsp_handle* h, unsigned long long seed, int mmax, int l
                        (const double*)(f.data() + 2 * g.n), NSP_HOST);
 }
 
+
+// ------------------------------------------------------------------ ensembles
+// Independent pipes stepped together: every stage is one launch per class of
+// members with the same compiled transform plans and precision (the radial
+// derivative and the four sweep passes: one launch for all members), with
+// blockIdx.y selecting the member through device tables of per-member
+// arguments.  Members converge independently: each has its own converged flag
+// in the corrector launches.  The whole ensemble step is one hipGraph; the
+// tables (which state buffer plays which role) are refreshed from pinned host
+// memory by the graph's first node, so one graph serves every step.
+struct nsp_ensemble {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<nsp_handle*> m;      // members in caller order
+  std::vector<int> owner;          // member whose operator data it uses (itself if none)
+  struct Cls {
+    std::vector<int> idx;          // members (caller indices), table order
+    int off = 0;                   // first table slot
+    int max_nr = 0, max_nz = 0, max_modes = 0;
+  };
+  std::vector<Cls> cls;
+  std::vector<int> order;          // table slot -> member
+  int max_nr = 0, max_nt = 0, max_nz = 0, max_modes = 0, maxit = 0;
+  double tol = 0.0;
+  // device: NL tables per input role [3][M], sweep tables [3][M], records [M], norm pointers [M]
+  NlMember* nl = nullptr;
+  Lin2Args* sw = nullptr;
+  StepRec* rec = nullptr;
+  unsigned long long** norms = nullptr;
+  // pinned host staging of the tables, and of the records read back
+  char* stage_h = nullptr;
+  size_t stage_bytes = 0;
+  StepRec* rec_h = nullptr;
+  hipGraphExec_t gexec = nullptr;
+};
+
+namespace {
+
+void ens_free(nsp_ensemble* e) {
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  e->gexec = nullptr;
+  if (e->nl) (void)hipFree(e->nl);
+  if (e->rec) (void)hipFree(e->rec);
+  if (e->norms) (void)hipFree(e->norms);
+  if (e->stage_h) (void)hipHostFree(e->stage_h);
+  if (e->rec_h) (void)hipHostFree(e->rec_h);
+  e->nl = nullptr;
+  e->sw = nullptr;
+  e->rec = nullptr;
+  e->norms = nullptr;
+  e->stage_h = nullptr;
+  e->rec_h = nullptr;
+  // members that use another's operator data go first
+  for (size_t j = 0; j < e->m.size(); j++)
+    if (e->m[j] && e->owner[j] != (int)j) {
+      (void)nsp_destroy(e->m[j]);
+      e->m[j] = nullptr;
+    }
+  for (size_t j = 0; j < e->m.size(); j++)
+    if (e->m[j]) (void)nsp_destroy(e->m[j]);
+  e->m.clear();
+  if (e->stream) (void)hipStreamDestroy(e->stream);
+  e->stream = nullptr;
+}
+
+bool same_pipe(const nsp_params& a, const nsp_params& b) {
+  return a.nr == b.nr && a.nt == b.nt && a.nz == b.nz && a.lz == b.lz && a.re == b.re && a.dt == b.dt &&
+         a.d_im == b.d_im && a.precision == b.precision;
+}
+
+NlMember nl_member(nsp_handle* h, const Geom& g, const double2* u) {
+  NlMember mb;
+  mb.g = g;
+  mb.r = h->t.r;
+  mb.D = h->t.D;
+  mb.u = u;
+  mb.dR = h->nb.dR;
+  mb.W = h->nb.mixed ? (void*)h->nb.W32 : (void*)h->nb.W;
+  mb.Y = h->nb.mixed ? (void*)h->nb.Y32 : (void*)h->nb.Y;
+  mb.N = h->Nl;
+  mb.ws = h->nb.wstride;
+  mb.f32 = h->nb.mixed ? 1 : 0;
+  return mb;
+}
+
+// Table contents for the members' current state buffers, into pinned staging.
+void ens_stage(nsp_ensemble* e) {
+  const int M = (int)e->m.size();
+  NlMember* nl = reinterpret_cast<NlMember*>(e->stage_h);
+  Lin2Args* sw = reinterpret_cast<Lin2Args*>(e->stage_h + 3 * M * sizeof(NlMember));
+  for (int j = 0; j < M; j++) {
+    nsp_handle* h = e->m[e->order[j]];
+    const int a = h->loc;
+    double2* ua = h->U[a];
+    double2* ub = h->U[(a + 1) % 3];
+    double2* uc = h->U[(a + 2) % 3];
+    Geom g0 = h->g, gk = h->g;
+    g0.skip = nullptr;
+    gk.skip = &e->rec[j].st[0];
+    nl[0 * M + j] = nl_member(h, g0, ua);
+    nl[1 * M + j] = nl_member(h, gk, ub);
+    nl[2 * M + j] = nl_member(h, gk, uc);
+    sw[0 * M + j] = lin_args(h, g0, h->Nl, ua, ub, true);
+    sw[1 * M + j] = lin_args(h, gk, h->Nl, ub, uc, false);
+    sw[2 * M + j] = lin_args(h, gk, h->Nl, uc, ub, false);
+  }
+}
+
+// One nonlinear evaluation of every member, input role `role`.
+bool ens_nonlinear(nsp_ensemble* e, int role, hipStream_t s) {
+  const int M = (int)e->m.size();
+  NlMember* nl = e->nl + role * M;
+  Batch all{nl, M, e->max_nr, e->max_nz, e->max_modes};
+  nsp_handle* h0 = e->m[e->order[0]];
+  launch_dr3(h0->g, nullptr, nullptr, nullptr, false, s, &all);
+  bool ok = true;
+  for (const auto& c : e->cls) {
+    nsp_handle* h = e->m[c.idx[0]];
+    const NlBufs& nb = h->nb;
+    const Geom& g = h->g;
+    Batch bt{nl + c.off, (int)c.idx.size(), c.max_nr, c.max_nz, c.max_modes};
+    if (nb.mixed) {
+      ok &= ct_theta_inv<float>(nb.var, g, (const cpx<float>*)nb.ct_tw_t32, nullptr, nullptr, nullptr, nullptr, 0, s,
+                                &bt);
+      ok &= ct_zpass<float>(nb.var, g, (const cpx<float>*)nb.ct_tw_h32, (const cpx<float>*)nb.tw_z32, nullptr,
+                            nullptr, 0, nullptr, s, &bt);
+      ok &= ct_theta_fwd<float>(nb.var, g, (const cpx<float>*)nb.ct_tw_t32, nullptr, 0, nullptr, nullptr, nullptr, s,
+                                &bt);
+    } else {
+      ok &= ct_theta_inv<double>(nb.var, g, (const cpx<double>*)nb.ct_tw_t, nullptr, nullptr, nullptr, nullptr, 0, s,
+                                 &bt);
+      ok &= ct_zpass<double>(nb.var, g, (const cpx<double>*)nb.ct_tw_h, (const cpx<double>*)nb.tw_z, nullptr,
+                             nullptr, 0, nullptr, s, &bt);
+      ok &= ct_theta_fwd<double>(nb.var, g, (const cpx<double>*)nb.ct_tw_t, nullptr, 0, nullptr, nullptr, nullptr, s,
+                                 &bt);
+    }
+  }
+  return ok;
+}
+
+nsp_status ens_capture(nsp_ensemble* e) {
+  const int M = (int)e->m.size();
+  hipStream_t s = e->stream;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  (void)hipMemcpyAsync(e->nl, e->stage_h, e->stage_bytes, hipMemcpyHostToDevice, s);
+  launch_ens_begin(e->rec, M, s);
+  bool ok = ens_nonlinear(e, 0, s);
+  launch_sweep_batch(e->sw, M, 1, e->max_nt, e->max_nz, s);
+  for (int it = 1; it <= e->maxit; it++) {
+    const int role = (it % 2) ? 1 : 2;
+    ok &= ens_nonlinear(e, role, s);
+    launch_ens_norm_reset(e->norms, e->rec, M, s);
+    launch_sweep_batch(e->sw + role * M, M, 0, e->max_nt, e->max_nz, s);
+    launch_ens_converge(e->norms, e->rec, M, e->tol, it, e->maxit, s);
+  }
+  (void)hipMemcpyAsync(e->rec_h, e->rec, M * sizeof(StepRec), hipMemcpyDeviceToHost, s);
+  hipGraph_t graph = nullptr;
+  const hipError_t err = hipStreamEndCapture(s, &graph);
+  if (err != hipSuccess || !graph) return fail(NSP_ERR_HIP, std::string("ensemble capture: ") + hipGetErrorString(err));
+  if (!ok) {
+    (void)hipGraphDestroy(graph);
+    return fail(NSP_ERR_INVALID, "an ensemble class names a kernel variant that is not compiled");
+  }
+  const hipError_t e2 = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e2 != hipSuccess) {
+    e->gexec = nullptr;
+    return fail(NSP_ERR_HIP, std::string("ensemble graph instantiate: ") + hipGetErrorString(e2));
+  }
+  return NSP_OK;
+}
+
+}  // namespace
+
+nsp_status nsp_ensemble_create(const nsp_params* params, int count, int device, nsp_ensemble** out) {
+  if (!out || !params || count < 1) return fail(NSP_ERR_INVALID, "bad ensemble arguments");
+  *out = nullptr;
+  nsp_ensemble* e = new nsp_ensemble();
+  e->device = device;
+  auto bail = [&](nsp_status st) {
+    ens_free(e);
+    delete e;
+    return st;
+  };
+  e->m.assign(count, nullptr);
+  e->owner.assign(count, 0);
+  for (int j = 0; j < count; j++) {
+    int own = j;
+    for (int q = 0; q < j; q++)
+      if (e->owner[q] == q && same_pipe(params[q], params[j])) {
+        own = q;
+        break;
+      }
+    e->owner[j] = own;
+    const nsp_status st = create_handle(&params[j], device, own == j ? nullptr : e->m[own], &e->m[j]);
+    if (st) return bail(st);
+    if (!e->m[j]->nb.use_ct) return bail(fail(NSP_ERR_INVALID, "ensemble members need compiled transform plans"));
+    if (params[j].maxit != params[0].maxit || params[j].tol != params[0].tol)
+      return bail(fail(NSP_ERR_INVALID, "ensemble members must share maxit and tol"));
+  }
+  e->maxit = params[0].maxit;
+  e->tol = params[0].tol;
+  // classes: same compiled plans and precision
+  for (int j = 0; j < count; j++) {
+    const Geom& g = e->m[j]->g;
+    int c = -1;
+    for (size_t q = 0; q < e->cls.size(); q++) {
+      const nsp_handle* f = e->m[e->cls[q].idx[0]];
+      if (f->g.ntp == g.ntp && f->g.nh == g.nh && f->nb.mixed == e->m[j]->nb.mixed) c = (int)q;
+    }
+    if (c < 0) {
+      e->cls.emplace_back();
+      c = (int)e->cls.size() - 1;
+    }
+    nsp_ensemble::Cls& C = e->cls[c];
+    C.idx.push_back(j);
+    C.max_nr = std::max(C.max_nr, g.nr);
+    C.max_nz = std::max(C.max_nz, g.nz);
+    C.max_modes = std::max(C.max_modes, g.modes);
+    e->max_nr = std::max(e->max_nr, g.nr);
+    e->max_nt = std::max(e->max_nt, g.nt);
+    e->max_nz = std::max(e->max_nz, g.nz);
+    e->max_modes = std::max(e->max_modes, g.modes);
+  }
+  for (auto& C : e->cls) {
+    C.off = (int)e->order.size();
+    for (int j : C.idx) e->order.push_back(j);
+    // one variant per class (a later nsp_ensemble_autotune may change it)
+    for (int j : C.idx) e->m[j]->nb.var = e->m[C.idx[0]]->nb.var;
+  }
+  if (hipSetDevice(device) != hipSuccess) return bail(fail(NSP_ERR_HIP, "hipSetDevice"));
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(NSP_ERR_HIP, "stream"));
+  const int M = count;
+  e->stage_bytes = 3 * M * sizeof(NlMember) + 3 * M * sizeof(Lin2Args);
+  void* p = nullptr;
+  if (hipMalloc(&p, e->stage_bytes) != hipSuccess) return bail(fail(NSP_ERR_NOMEM, "ensemble tables"));
+  e->nl = static_cast<NlMember*>(p);
+  e->sw = reinterpret_cast<Lin2Args*>(static_cast<char*>(p) + 3 * M * sizeof(NlMember));
+  if (hipMalloc(&p, M * sizeof(StepRec)) != hipSuccess) return bail(fail(NSP_ERR_NOMEM, "ensemble records"));
+  e->rec = static_cast<StepRec*>(p);
+  if (hipMalloc(&p, M * sizeof(unsigned long long*)) != hipSuccess) return bail(fail(NSP_ERR_NOMEM, "norms"));
+  e->norms = static_cast<unsigned long long**>(p);
+  if (hipHostMalloc(&p, e->stage_bytes) != hipSuccess) return bail(fail(NSP_ERR_NOMEM, "pinned staging"));
+  e->stage_h = static_cast<char*>(p);
+  if (hipHostMalloc(&p, M * sizeof(StepRec)) != hipSuccess) return bail(fail(NSP_ERR_NOMEM, "pinned records"));
+  e->rec_h = static_cast<StepRec*>(p);
+  std::vector<unsigned long long*> nptr(M);
+  for (int j = 0; j < M; j++) nptr[j] = e->m[e->order[j]]->norm;
+  if (hipMemcpy(e->norms, nptr.data(), M * sizeof(unsigned long long*), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(NSP_ERR_HIP, "norm table"));
+  *out = e;
+  return NSP_OK;
+}
+
+nsp_status nsp_ensemble_destroy(nsp_ensemble* e) {
+  if (!e) return NSP_OK;
+  (void)hipSetDevice(e->device);
+  if (e->stream) (void)hipStreamSynchronize(e->stream);
+  ens_free(e);
+  delete e;
+  return NSP_OK;
+}
+
+int nsp_ensemble_size(nsp_ensemble* e) { return e ? (int)e->m.size() : 0; }
+
+nsp_handle* nsp_ensemble_member(nsp_ensemble* e, int i) {
+  if (!e || i < 0 || i >= (int)e->m.size()) return nullptr;
+  return e->m[i];
+}
+
+nsp_status nsp_ensemble_autotune(nsp_ensemble* e, int reps) {
+  if (!e) return fail(NSP_ERR_INVALID, "null ensemble");
+  for (const auto& C : e->cls) {
+    nsp_handle* f = e->m[C.idx[0]];
+    const nsp_status st = nsp_autotune(f, reps);
+    if (st) return st;
+    for (int j : C.idx) {
+      e->m[j]->nb.var = f->nb.var;
+      drop_graphs(e->m[j]);
+    }
+  }
+  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
+  e->gexec = nullptr;
+  return NSP_OK;
+}
+
+nsp_status nsp_ensemble_step(nsp_ensemble* e, int nsteps, nsp_step_info* info) {
+  if (!e) return fail(NSP_ERR_INVALID, "null ensemble");
+  if (nsteps < 0) return fail(NSP_ERR_INVALID, "nsteps < 0");
+  HIPCHK(hipSetDevice(e->device));
+  const int M = (int)e->m.size();
+  for (int j = 0; j < M; j++) HIPCHK(hipStreamSynchronize(e->m[j]->stream));  // caller work on members
+  if (!e->gexec) {
+    const nsp_status st = ens_capture(e);
+    if (st) return st;
+  }
+  for (int step = 0; step < nsteps; step++) {
+    ens_stage(e);
+    HIPCHK(hipGraphLaunch(e->gexec, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int j = 0; j < M; j++) {
+      const int mi = e->order[j];
+      nsp_handle* h = e->m[mi];
+      const StepRec& r = e->rec_h[j];
+      const int a = h->loc;
+      set_state_slot(h, (r.st[1] % 2 == 1) ? (a + 2) % 3 : (a + 1) % 3);
+      if (info) {
+        info[(size_t)step * M + mi].n_it = r.st[1];
+        info[(size_t)step * M + mi].err = r.err;
+        info[(size_t)step * M + mi].flags = r.st[2];
+      }
+    }
+  }
+  return NSP_OK;
+}
 }  // extern "C"

@@ -184,6 +184,48 @@ __global__ void widen_kernel(const float2* __restrict__ in, double2* __restrict_
   }
 }
 
+// u += f over `count` complex values (additive seeded perturbations).
+__global__ void add_kernel(double2* __restrict__ u, const double2* __restrict__ f, size_t count) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q < count) {
+    const double2 a = u[q], b = f[q];
+    u[q] = make_double2(a.x + b.x, a.y + b.y);
+  }
+}
+
+// Ensemble forms of the three bookkeeping kernels below (one thread per member).
+__global__ void ens_begin_kernel(StepRec* rec, int count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < count) {
+    rec[j].st[0] = 0;
+    rec[j].st[1] = 0;
+    rec[j].st[2] = 0;
+    rec[j].st[3] = 0;
+    rec[j].err = 1e98;
+  }
+}
+__global__ void ens_norm_reset_kernel(unsigned long long* const* norms, const StepRec* rec, int count) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j < count && !rec[j].st[0]) {
+    norms[j][0] = 0ull;
+    norms[j][1] = 0ull;
+  }
+}
+__global__ void ens_converge_kernel(unsigned long long* const* norms, StepRec* rec, int count, double tol, int it,
+                                    int maxit) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= count || rec[j].st[0]) return;
+  const unsigned long long* nm = norms[j];
+  const double e = __longlong_as_double((long long)nm[0]) / __longlong_as_double((long long)nm[1]);
+  rec[j].err = e;
+  rec[j].st[1] = it;
+  const bool nan = e != e;
+  if (!(e > tol) || it >= maxit) {
+    rec[j].st[0] = 1;
+    rec[j].st[2] = (nan ? 1 : 0) | ((!nan && e > tol) ? 2 : 0);
+  }
+}
+
 // Start of a captured step: not converged, no corrector yet.
 __global__ void step_begin_kernel(int* st, double* err) {
   if (threadIdx.x == 0) {
@@ -365,6 +407,19 @@ __global__ __launch_bounds__(256) void diag_kernel(Geom g, DevTables t, const do
 // ------------------------------------------------------------------ launchers
 void launch_norm_reset(unsigned long long* norm, const int* skip, hipStream_t s) {
   norm_reset_kernel<<<1, 64, 0, s>>>(norm, skip);
+}
+void launch_add(double2* u, const double2* f, size_t count, hipStream_t s) {
+  add_kernel<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(u, f, count);
+}
+void launch_ens_begin(StepRec* rec, int count, hipStream_t s) {
+  ens_begin_kernel<<<(count + 63) / 64, 64, 0, s>>>(rec, count);
+}
+void launch_ens_norm_reset(unsigned long long* const* norms, const StepRec* rec, int count, hipStream_t s) {
+  ens_norm_reset_kernel<<<(count + 63) / 64, 64, 0, s>>>(norms, rec, count);
+}
+void launch_ens_converge(unsigned long long* const* norms, StepRec* rec, int count, double tol, int it, int maxit,
+                         hipStream_t s) {
+  ens_converge_kernel<<<(count + 63) / 64, 64, 0, s>>>(norms, rec, count, tol, it, maxit);
 }
 void launch_widen(const float2* in, double2* out, size_t count, hipStream_t s) {
   widen_kernel<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(in, out, count);

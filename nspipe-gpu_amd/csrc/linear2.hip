@@ -200,12 +200,36 @@ __global__ __launch_bounds__(256) void factor_kernel(Geom g, const double* __res
 // chunk of 64 axial modes.  One kernel per pass (PASS 0..3 = A..D): the
 // column moves between passes through SR/SP in any case, and each pass then
 // gets its own register allocation (and occupancy).
-template <int PASS, int PRED>
-__global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A) {
+// B = 1: batched launch, one pipe per blockIdx.y with its arguments from tab
+// (the single-pipe form B = 0 reads only its own argument block).
+template <int PASS, int PRED, int B>
+__global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* tab) {
+  if constexpr (B != 0) {
+    A = tab[blockIdx.y];
+    A.r = gmem(tab, A.r);
+    A.rdr = gmem(tab, A.rdr);
+    A.uc = gmem(tab, A.uc);
+    A.qc = gmem(tab, A.qc);
+    A.Dc = gmem(tab, A.Dc);
+    A.Luc = gmem(tab, A.Luc);
+    A.Lu = gmem(tab, A.Lu);
+    A.Lp = gmem(tab, A.Lp);
+    A.F = gmem(tab, A.F);
+    A.PH = gmem(tab, A.PH);
+    A.invc = gmem(tab, A.invc);
+    A.N = gmem(tab, A.N);
+    A.E = gmem(tab, A.E);
+    A.u = gmem(tab, A.u);
+    A.out = gmem(tab, A.out);
+    A.SR = gmem(tab, A.SR);
+    A.SP = gmem(tab, A.SP);
+    A.norm = gmem(tab, A.norm);
+  }
   const Geom& g = A.g;
   if (step_done(g)) return;
   const int nch = (g.nz + 63) / 64;
   const int pair = blockIdx.x / nch, ch = blockIdx.x - pair * nch;
+  if (pair >= g.nt / 2) return;  // beyond this pipe's extent
   const int nth = g.nt / 2;
   const int it = (threadIdx.x < 64) ? pair : (pair == 0 ? nth : g.nt - pair);
   const int l = ch * 64 + (threadIdx.x & 63);
@@ -611,19 +635,25 @@ void launch_factor(const Geom& g, const double* r, const double* Lu, const doubl
   factor_kernel<<<grid, 256, 0, s>>>(g, r, Lu, Lp, F);
 }
 
+template <int PRED, int B>
+static void sweep_passes(const Lin2Args& a, const Lin2Args* tab, dim3 grid, hipStream_t s) {
+  sweep_kernel<0, PRED, B><<<grid, 128, 0, s>>>(a, tab);
+  sweep_kernel<1, PRED, B><<<grid, 128, 0, s>>>(a, tab);
+  sweep_kernel<2, PRED, B><<<grid, 128, 0, s>>>(a, tab);
+  sweep_kernel<3, PRED, B><<<grid, 128, 0, s>>>(a, tab);
+}
+
 void launch_sweep2(const Lin2Args& a, hipStream_t s) {
-  const int blocks = (a.g.nt / 2) * ((a.g.nz + 63) / 64);
-  if (a.predictor) {
-    sweep_kernel<0, 1><<<blocks, 128, 0, s>>>(a);
-    sweep_kernel<1, 1><<<blocks, 128, 0, s>>>(a);
-    sweep_kernel<2, 1><<<blocks, 128, 0, s>>>(a);
-    sweep_kernel<3, 1><<<blocks, 128, 0, s>>>(a);
-  } else {
-    sweep_kernel<0, 0><<<blocks, 128, 0, s>>>(a);
-    sweep_kernel<1, 0><<<blocks, 128, 0, s>>>(a);
-    sweep_kernel<2, 0><<<blocks, 128, 0, s>>>(a);
-    sweep_kernel<3, 0><<<blocks, 128, 0, s>>>(a);
-  }
+  const dim3 grid((a.g.nt / 2) * ((a.g.nz + 63) / 64));
+  if (a.predictor) sweep_passes<1, 0>(a, nullptr, grid, s);
+  else sweep_passes<0, 0>(a, nullptr, grid, s);
+}
+
+void launch_sweep_batch(const Lin2Args* tab, int count, int predictor, int max_nt, int max_nz, hipStream_t s) {
+  const dim3 grid((max_nt / 2) * ((max_nz + 63) / 64), count);
+  Lin2Args dummy{};
+  if (predictor) sweep_passes<1, 1>(dummy, tab, grid, s);
+  else sweep_passes<0, 1>(dummy, tab, grid, s);
 }
 
 }  // namespace nsp

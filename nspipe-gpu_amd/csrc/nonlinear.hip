@@ -32,10 +32,19 @@ __device__ __forceinline__ int m_of(int it, int nt) { return it < nt / 2 ? it : 
 // level of the three components does not fit in L2).  Levels up to PF ahead
 // are loaded early so each thread keeps several loads in flight.  The sum
 // runs over the same terms in the same order as derivr_k.
-template <typename OT>
+// B = 1: batched launch, one member per blockIdx.z with its arguments from tab.
+template <int B>
 __global__ __launch_bounds__(256) void dr3_kernel(Geom g, const double* __restrict__ D,
                                                   const double2* __restrict__ u,
-                                                  OT* __restrict__ dR) {
+                                                  void* __restrict__ dRv, int f32, const NlMember* tab) {
+  if constexpr (B != 0) {
+    const NlMember& mb = tab[blockIdx.z];
+    g = mb.g;
+    D = gmem(tab, mb.D);
+    u = gmem(tab, mb.u);
+    dRv = gmem(tab, mb.dR);
+    f32 = mb.f32;
+  }
   if (step_done(g)) return;
   constexpr int NB = 12, PF = 5;  // ring slots; levels i-3 .. i+3+PF live at step i
   static_assert(STEN + PF <= NB, "ring too small");
@@ -46,7 +55,7 @@ __global__ __launch_bounds__(256) void dr3_kernel(Geom g, const double* __restri
   const int m = m_of(k / g.nz, g.nt);
   const double* Dp = D + abs((abs(m) % 2) - (c < 2 ? 1 : 0)) * STEN * nr;
   const double2* src = u + (size_t)c * g.n + k;
-  OT* dst = dR + (size_t)c * g.n + k;
+  const size_t base = (size_t)c * g.n + k;
   const size_t ld = g.modes;
   double2 w[NB];  // level L in slot L % NB
 #pragma unroll
@@ -71,7 +80,10 @@ __global__ __launch_bounds__(256) void dr3_kernel(Geom g, const double* __restri
             py += de * a.y;
           }
         }
-        dst[i * ld] = OT{(decltype(OT::x))px, (decltype(OT::x))py};
+        if (f32)  // mixed form: d_r u only feeds the FP32 theta transforms
+          reinterpret_cast<float2*>(dRv)[base + i * ld] = make_float2(float(px), float(py));
+        else
+          reinterpret_cast<double2*>(dRv)[base + i * ld] = make_double2(px, py);
       }
     }
   }
@@ -332,12 +344,15 @@ const char* kernel_name(int id) {
   return (id >= 0 && id < K_COUNT) ? names[id] : "";
 }
 
-void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, bool f32, hipStream_t s) {
-  dim3 gd((g.modes + 255) / 256, 3);
-  if (f32)  // mixed form: d_r u only feeds the FP32 theta transforms
-    dr3_kernel<float2><<<gd, 256, 0, s>>>(g, D, u, reinterpret_cast<float2*>(dR));
-  else
-    dr3_kernel<double2><<<gd, 256, 0, s>>>(g, D, u, dR);
+void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, bool f32, hipStream_t s,
+                const Batch* bt) {
+  if (bt) {
+    dim3 gd((bt->max_modes + 255) / 256, 3, bt->count);
+    dr3_kernel<1><<<gd, 256, 0, s>>>(g, D, u, dR, f32 ? 1 : 0, bt->tab);
+  } else {
+    dim3 gd((g.modes + 255) / 256, 3);
+    dr3_kernel<0><<<gd, 256, 0, s>>>(g, D, u, dR, f32 ? 1 : 0, nullptr);
+  }
 }
 
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u, double2* N,

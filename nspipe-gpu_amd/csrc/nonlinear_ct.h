@@ -49,12 +49,22 @@ __device__ __forceinline__ void theta_block(const Geom& g, int TL, int ord, int&
 // The radial derivative (derivr_k, mod_radialFD.cu:10-30, flags 1,1,0) comes
 // from dR (dr3_kernel, stored at precision T); forming it here from seven
 // levels was slower.
-template <typename T, int TL, int NTHR, typename LY, typename PL>
+template <typename T, int TL, int NTHR, typename LY, typename PL, int BAT>
 __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                      const double* __restrict__ r,
                                                      const double2* __restrict__ u,
                                                      const cpx<T>* __restrict__ dR,
-                                                     cpx<T>* __restrict__ W, size_t ws, int ord) {
+                                                     cpx<T>* __restrict__ W, size_t ws, int ord,
+                                                     const NlMember* tab) {
+  if constexpr (BAT != 0) {
+    const NlMember& m = tab[blockIdx.y];
+    g = m.g;
+    r = gmem(tab, m.r);
+    u = gmem(tab, m.u);
+    dR = gmem(tab, reinterpret_cast<const cpx<T>*>(m.dR));
+    W = gmem(tab, reinterpret_cast<cpx<T>*>(m.W));
+    ws = m.ws;
+  }
   if (step_done(g)) return;
   constexpr int NTP = PL::L;
   constexpr int NT = 2 * NTP / 3;
@@ -63,6 +73,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
   const int nz = g.nz;
   int i, l0;
   theta_block(g, TL, ord, i, l0);
+  if (i >= g.nr || l0 >= nz) return;  // beyond this member's extent
   const int tid = threadIdx.x;
   const double ri = r[i];
   constexpr int half = NT / 2;
@@ -117,17 +128,26 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
 }
 
 // ------------------------------------------------------------------ z pass
-template <typename T, int NTHR, int MPT, typename LY, typename PL>
+template <typename T, int NTHR, int MPT, typename LY, typename PL, int BAT>
 __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                  const cpx<T>* __restrict__ twz,
                                                  const double* __restrict__ r,
                                                  const cpx<T>* __restrict__ W, size_t ws,
-                                                 cpx<T>* __restrict__ Y) {
+                                                 cpx<T>* __restrict__ Y, const NlMember* tab) {
+  if constexpr (BAT != 0) {
+    const NlMember& m = tab[blockIdx.y];
+    g = m.g;
+    r = gmem(tab, m.r);
+    W = gmem(tab, reinterpret_cast<const cpx<T>*>(m.W));
+    Y = gmem(tab, reinterpret_cast<cpx<T>*>(m.Y));
+    ws = m.ws;
+  }
   if (step_done(g)) return;
   constexpr int NH = PL::L;
   constexpr int NZF = 2 * NH;
   __shared__ cpx<T> buf[LY::size(NH)];
   const int row = blockIdx.x;
+  if (row >= g.nr * g.ntp) return;  // beyond this member's extent
   const int i = row / g.ntp;
   const int tid = threadIdx.x;
   const int nz = g.nz;
@@ -240,12 +260,20 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
 //  * with PF the 2 R0 inputs of the next transform are loaded before the
 //    current one's LDS passes start, overlapping HBM latency with them.
 // LDS rows of NH/R_last elements padded by PAD slots (bank spreading).
-template <typename T, int NTHR, int PF, int PAD, typename PL>
+template <typename T, int NTHR, int PF, int PAD, typename PL, int BAT>
 __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void zpass2_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                   const cpx<T>* __restrict__ twz,
                                                   const double* __restrict__ r,
                                                   const cpx<T>* __restrict__ W, size_t ws,
-                                                  cpx<T>* __restrict__ Y) {
+                                                  cpx<T>* __restrict__ Y, const NlMember* tab) {
+  if constexpr (BAT != 0) {
+    const NlMember& m = tab[blockIdx.y];
+    g = m.g;
+    r = gmem(tab, m.r);
+    W = gmem(tab, reinterpret_cast<const cpx<T>*>(m.W));
+    Y = gmem(tab, reinterpret_cast<cpx<T>*>(m.Y));
+    ws = m.ws;
+  }
   if (step_done(g)) return;
   constexpr int NH = PL::L;
   constexpr int R0 = PL::rad[0];
@@ -254,6 +282,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   using LY = Lay<NH / PL::rad[PL::P - 1], PAD>;
   __shared__ cpx<T> buf[LY::size(NH)];
   const int row = blockIdx.x;
+  if (row >= g.nr * g.ntp) return;  // beyond this member's extent
   const int i = row / g.ntp;
   const int tid = threadIdx.x;
   const bool own = tid < NB1;
@@ -390,17 +419,27 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
 }
 
 // ------------------------------------------------------------------ theta forward
-template <typename T, int TL, int NTHR, typename LY, typename PL>
+template <typename T, int TL, int NTHR, typename LY, typename PL, int BAT>
 __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                      const cpx<T>* __restrict__ Y, size_t ws,
                                                      const double* __restrict__ D,
                                                      const double2* __restrict__ u,
-                                                     double2* __restrict__ N, int ord) {
+                                                     double2* __restrict__ N, int ord, const NlMember* tab) {
+  if constexpr (BAT != 0) {
+    const NlMember& m = tab[blockIdx.y];
+    g = m.g;
+    Y = gmem(tab, reinterpret_cast<const cpx<T>*>(m.Y));
+    ws = m.ws;
+    D = gmem(tab, m.D);
+    u = gmem(tab, m.u);
+    N = gmem(tab, m.N);
+  }
   if (step_done(g)) return;
   constexpr int NTP = PL::L;
   __shared__ cpx<T> buf[LY::size(NTP * TL)];
   int i, l0;
   theta_block(g, TL, ord, i, l0);
+  if (i >= g.nr || l0 >= g.nz) return;  // beyond this member's extent
   const int tid = threadIdx.x;
   const int nt = g.nt, half = nt / 2, nz = g.nz;
   const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
@@ -447,12 +486,22 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
 //
 // padForward_k row map: row j < NT/2 holds it = j, rows NT/2 <= j < NT are
 // zero, row j >= NT holds it = j - NT/2 (NTP - NT = NT/2).
-template <typename T, int TL, int NTHR, typename LY, typename PL>
+template <typename T, int TL, int NTHR, typename LY, typename PL, int BAT>
 __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                       const double* __restrict__ r,
                                                       const double2* __restrict__ u,
                                                       const cpx<T>* __restrict__ dR,
-                                                      cpx<T>* __restrict__ W, size_t ws, int ord) {
+                                                      cpx<T>* __restrict__ W, size_t ws, int ord,
+                                                      const NlMember* tab) {
+  if constexpr (BAT != 0) {
+    const NlMember& m = tab[blockIdx.y];
+    g = m.g;
+    r = gmem(tab, m.r);
+    u = gmem(tab, m.u);
+    dR = gmem(tab, reinterpret_cast<const cpx<T>*>(m.dR));
+    W = gmem(tab, reinterpret_cast<cpx<T>*>(m.W));
+    ws = m.ws;
+  }
   if (step_done(g)) return;
   constexpr int NTP = PL::L;
   constexpr int NT = 2 * NTP / 3;
@@ -465,6 +514,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
   const int nz = g.nz;
   int i, l0;
   theta_block(g, TL, ord, i, l0);
+  if (i >= g.nr || l0 >= g.nz) return;  // beyond this member's extent
   const int tid = threadIdx.x;
   const double ri = r[i];
 #pragma unroll 1
@@ -515,12 +565,21 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
 // padBackward_k: output row j < NT/2 is it = j, row j >= NT is it = j - NT/2,
 // rows NT/2 <= j < NT are dropped; it = NT/2 (m = -NT/2) and l = NZ-1 are
 // zeroed.
-template <typename T, int TL, int NTHR, typename LY, typename PL>
+template <typename T, int TL, int NTHR, typename LY, typename PL, int BAT>
 __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                       const cpx<T>* __restrict__ Y, size_t ws,
                                                       const double* __restrict__ D,
                                                       const double2* __restrict__ u,
-                                                      double2* __restrict__ N, int ord) {
+                                                      double2* __restrict__ N, int ord, const NlMember* tab) {
+  if constexpr (BAT != 0) {
+    const NlMember& m = tab[blockIdx.y];
+    g = m.g;
+    Y = gmem(tab, reinterpret_cast<const cpx<T>*>(m.Y));
+    ws = m.ws;
+    D = gmem(tab, m.D);
+    u = gmem(tab, m.u);
+    N = gmem(tab, m.N);
+  }
   if (step_done(g)) return;
   constexpr int NTP = PL::L;
   constexpr int NT = 2 * NTP / 3;
@@ -532,6 +591,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
   __shared__ cpx<T> buf[LY::size(NTP * TL)];
   int i, l0;
   theta_block(g, TL, ord, i, l0);
+  if (i >= g.nr || l0 >= g.nz) return;  // beyond this member's extent
   const int tid = threadIdx.x;
   const int nz = g.nz;
   const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
@@ -601,59 +661,61 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
 template <typename PL, int TL>
 using ThetaRowPad = Lay<(PL::L / PL::rad[PL::P - 1]) * TL, TL>;
 
-template <typename T, typename PL>
+template <typename T, typename PL, int BAT>
 bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
-                    const double2* dR_, cpx<T>* W, size_t ws, hipStream_t s) {
+                    const double2* dR_, cpx<T>* W, size_t ws, hipStream_t s, const Batch* bt) {
   const cpx<T>* dR = reinterpret_cast<const cpx<T>*>(dR_);  // stored at precision T
   const int tl = v.tl;
   if (PL::L * tl * (int)sizeof(cpx<T>) > 163840) return false;
-  const int nb = ((g.nz + tl - 1) / tl) * g.nr;
+  const NlMember* tab = bt ? bt->tab : nullptr;
+  const dim3 nb = bt ? dim3(((bt->max_nz + tl - 1) / tl) * bt->max_nr, bt->count) : dim3(((g.nz + tl - 1) / tl) * g.nr);
   if (v.alg == 1) {
-    if (tl == 4 && v.nthr == 192) theta_inv2_ct<T, 4, 192, Dense, PL><<<nb, 192, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-    else if (tl == 8 && v.nthr == 256) theta_inv2_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-    else if (tl == 8 && v.nthr == 384) theta_inv2_ct<T, 8, 384, Dense, PL><<<nb, 384, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-    else if (tl == 16 && v.nthr == 256) theta_inv2_ct<T, 16, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+    if (tl == 4 && v.nthr == 192) theta_inv2_ct<T, 4, 192, Dense, PL, BAT><<<nb, 192, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+    else if (tl == 8 && v.nthr == 256) theta_inv2_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+    else if (tl == 8 && v.nthr == 384) theta_inv2_ct<T, 8, 384, Dense, PL, BAT><<<nb, 384, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+    else if (tl == 16 && v.nthr == 256) theta_inv2_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
     else return false;
     return true;
   }
   if (v.alg != 0) return false;
   using R4 = ThetaRowPad<PL, 4>;
   using R8 = ThetaRowPad<PL, 8>;
-  if (tl == 4 && v.pad == 0) theta_inv_ct<T, 4, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-  else if (tl == 4 && v.pad == 1) theta_inv_ct<T, 4, 256, Lay<16, 1>, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-  else if (tl == 4 && v.pad == 2) theta_inv_ct<T, 4, 256, R4, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-  else if (tl == 8 && v.pad == 0) theta_inv_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-  else if (tl == 8 && v.pad == 1) theta_inv_ct<T, 8, 256, Lay<8, 1>, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-  else if (tl == 8 && v.pad == 2) theta_inv_ct<T, 8, 256, R8, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
-  else if (tl == 16 && v.pad == 0) theta_inv_ct<T, 16, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord);
+  if (tl == 4 && v.pad == 0) theta_inv_ct<T, 4, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+  else if (tl == 4 && v.pad == 1) theta_inv_ct<T, 4, 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+  else if (tl == 4 && v.pad == 2) theta_inv_ct<T, 4, 256, R4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+  else if (tl == 8 && v.pad == 0) theta_inv_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+  else if (tl == 8 && v.pad == 1) theta_inv_ct<T, 8, 256, Lay<8, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+  else if (tl == 8 && v.pad == 2) theta_inv_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+  else if (tl == 16 && v.pad == 0) theta_inv_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
   else return false;
   return true;
 }
 
-template <typename T, typename PL>
+template <typename T, typename PL, int BAT>
 bool theta_fwd_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const cpx<T>* Y, size_t ws, const double* D,
-                    const double2* u, double2* N, hipStream_t s) {
+                    const double2* u, double2* N, hipStream_t s, const Batch* bt) {
   const int tl = v.tl;
   if (PL::L * tl * (int)sizeof(cpx<T>) > 163840) return false;
-  const int nb = ((g.nz + tl - 1) / tl) * g.nr;
+  const NlMember* tab = bt ? bt->tab : nullptr;
+  const dim3 nb = bt ? dim3(((bt->max_nz + tl - 1) / tl) * bt->max_nr, bt->count) : dim3(((g.nz + tl - 1) / tl) * g.nr);
   if (v.alg == 1) {
-    if (tl == 4 && v.nthr == 192) theta_fwd2_ct<T, 4, 192, Dense, PL><<<nb, 192, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-    else if (tl == 8 && v.nthr == 256) theta_fwd2_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-    else if (tl == 8 && v.nthr == 384) theta_fwd2_ct<T, 8, 384, Dense, PL><<<nb, 384, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-    else if (tl == 16 && v.nthr == 256) theta_fwd2_ct<T, 16, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+    if (tl == 4 && v.nthr == 192) theta_fwd2_ct<T, 4, 192, Dense, PL, BAT><<<nb, 192, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+    else if (tl == 8 && v.nthr == 256) theta_fwd2_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+    else if (tl == 8 && v.nthr == 384) theta_fwd2_ct<T, 8, 384, Dense, PL, BAT><<<nb, 384, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+    else if (tl == 16 && v.nthr == 256) theta_fwd2_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
     else return false;
     return true;
   }
   if (v.alg != 0) return false;
   using R4 = ThetaRowPad<PL, 4>;
   using R8 = ThetaRowPad<PL, 8>;
-  if (tl == 4 && v.pad == 0) theta_fwd_ct<T, 4, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-  else if (tl == 4 && v.pad == 1) theta_fwd_ct<T, 4, 256, Lay<16, 1>, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-  else if (tl == 4 && v.pad == 2) theta_fwd_ct<T, 4, 256, R4, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-  else if (tl == 8 && v.pad == 0) theta_fwd_ct<T, 8, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-  else if (tl == 8 && v.pad == 1) theta_fwd_ct<T, 8, 256, Lay<8, 1>, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-  else if (tl == 8 && v.pad == 2) theta_fwd_ct<T, 8, 256, R8, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
-  else if (tl == 16 && v.pad == 0) theta_fwd_ct<T, 16, 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord);
+  if (tl == 4 && v.pad == 0) theta_fwd_ct<T, 4, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+  else if (tl == 4 && v.pad == 1) theta_fwd_ct<T, 4, 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+  else if (tl == 4 && v.pad == 2) theta_fwd_ct<T, 4, 256, R4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+  else if (tl == 8 && v.pad == 0) theta_fwd_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+  else if (tl == 8 && v.pad == 1) theta_fwd_ct<T, 8, 256, Lay<8, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+  else if (tl == 8 && v.pad == 2) theta_fwd_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+  else if (tl == 16 && v.pad == 0) theta_fwd_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
   else return false;
   return true;
 }
@@ -665,72 +727,113 @@ constexpr int zpass2_pad() {
   return (PL::P == 3 && PL::rad[1] == 16 && PL::rad[2] == 16) ? PL::rad[0] : 0;
 }
 
-template <typename T, typename PL>
+template <typename T, typename PL, int BAT>
 bool zpass_plan(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* twz, const double* r,
-                const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s) {
+                const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s, const Batch* bt) {
   constexpr int LL = PL::L;
-  const int nb = g.nr * g.ntp;
+  const NlMember* tab = bt ? bt->tab : nullptr;
+  const dim3 nb = bt ? dim3(bt->max_nr * g.ntp, bt->count) : dim3(g.nr * g.ntp);
   if (v.z_alg == 1) {  // register form, 256 threads
     constexpr int PAD = zpass2_pad<PL>();
     if (v.z_pf)
-      zpass2_ct<T, 256, 1, PAD, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
+      zpass2_ct<T, 256, 1, PAD, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
     else
-      zpass2_ct<T, 256, 0, PAD, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
+      zpass2_ct<T, 256, 0, PAD, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
     return true;
   }
   if (v.z_alg != 0) return false;
   using RP = Lay<LL / PL::rad[PL::P - 1], zpass2_pad<PL>()>;
   if (v.z_pad == 2) {
     if (v.z_nthr == 256)
-      zpass_ct<T, 256, (2 * LL + 255) / 256, RP, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
+      zpass_ct<T, 256, (2 * LL + 255) / 256, RP, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
     else if (v.z_nthr == 512)
-      zpass_ct<T, 512, (2 * LL + 511) / 512, RP, PL><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y);
+      zpass_ct<T, 512, (2 * LL + 511) / 512, RP, PL, BAT><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
     else
       return false;
     return true;
   }
   if (v.z_nthr == 256 && !v.z_pad)
-    zpass_ct<T, 256, (2 * LL + 255) / 256, Dense, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
+    zpass_ct<T, 256, (2 * LL + 255) / 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
   else if (v.z_nthr == 256)
-    zpass_ct<T, 256, (2 * LL + 255) / 256, Lay<16, 1>, PL><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y);
+    zpass_ct<T, 256, (2 * LL + 255) / 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
   else if (v.z_nthr == 512 && !v.z_pad)
-    zpass_ct<T, 512, (2 * LL + 511) / 512, Dense, PL><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y);
+    zpass_ct<T, 512, (2 * LL + 511) / 512, Dense, PL, BAT><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
   else if (v.z_nthr == 512)
-    zpass_ct<T, 512, (2 * LL + 511) / 512, Lay<16, 1>, PL><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y);
+    zpass_ct<T, 512, (2 * LL + 511) / 512, Lay<16, 1>, PL, BAT><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
   else
     return false;
   return true;
 }
 
 // Return false when the requested variant is not compiled for this size.
-template <typename T>
-bool ct_theta_inv(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
-                  const double2* dR, cpx<T>* W, size_t ws, hipStream_t s) {
+// BAT = 1: the batched kernels (one member per blockIdx.y, arguments from
+// bt->tab); BAT = 0: the single-pipe kernels, whose arguments are only the
+// launch's own (so every field pointer stays a global-memory pointer).
+template <typename T, int BAT>
+bool ct_theta_inv_b(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
+                    const double2* dR, cpx<T>* W, size_t ws, hipStream_t s, const Batch* bt) {
 #define X(LL, ...) \
-  if (g.ntp == LL) return theta_inv_plan<T, __VA_ARGS__>(v.inv, g, tw, r, u, dR, W, ws, s);
+  if (g.ntp == LL) return theta_inv_plan<T, __VA_ARGS__, BAT>(v.inv, g, tw, r, u, dR, W, ws, s, bt);
   NSP_THETA_PLANS(X)
 #undef X
   return false;
+}
+
+template <typename T, int BAT>
+bool ct_theta_fwd_b(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* Y, size_t ws,
+                    const double* D, const double2* u, double2* N, hipStream_t s, const Batch* bt) {
+#define X(LL, ...) \
+  if (g.ntp == LL) return theta_fwd_plan<T, __VA_ARGS__, BAT>(v.fwd, g, tw, Y, ws, D, u, N, s, bt);
+  NSP_THETA_PLANS(X)
+#undef X
+  return false;
+}
+
+template <typename T, int BAT>
+bool ct_zpass_b(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* twz, const double* r,
+                const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s, const Batch* bt) {
+#define X(LL, ...) \
+  if (g.nh == LL) return zpass_plan<T, __VA_ARGS__, BAT>(v, g, tw, twz, r, W, ws, Y, s, bt);
+  NSP_ZHALF_PLANS(X)
+#undef X
+  return false;
+}
+
+// The batched instantiations are compiled in their own translation units.
+#define NSP_CT_EXTERN(T, B)                                                                                     \
+  extern template bool ct_theta_inv_b<T, B>(const NlVariant&, const Geom&, const cpx<T>*, const double*,        \
+                                            const double2*, const double2*, cpx<T>*, size_t, hipStream_t,      \
+                                            const Batch*);                                                      \
+  extern template bool ct_theta_fwd_b<T, B>(const NlVariant&, const Geom&, const cpx<T>*, const cpx<T>*, size_t, \
+                                            const double*, const double2*, double2*, hipStream_t, const Batch*); \
+  extern template bool ct_zpass_b<T, B>(const NlVariant&, const Geom&, const cpx<T>*, const cpx<T>*,            \
+                                        const double*, const cpx<T>*, size_t, cpx<T>*, hipStream_t, const Batch*);
+#define NSP_CT_INSTANTIATE(T, B)                                                                                \
+  template bool ct_theta_inv_b<T, B>(const NlVariant&, const Geom&, const cpx<T>*, const double*, const double2*, \
+                                     const double2*, cpx<T>*, size_t, hipStream_t, const Batch*);               \
+  template bool ct_theta_fwd_b<T, B>(const NlVariant&, const Geom&, const cpx<T>*, const cpx<T>*, size_t,       \
+                                     const double*, const double2*, double2*, hipStream_t, const Batch*);       \
+  template bool ct_zpass_b<T, B>(const NlVariant&, const Geom&, const cpx<T>*, const cpx<T>*, const double*,    \
+                                 const cpx<T>*, size_t, cpx<T>*, hipStream_t, const Batch*);
+
+template <typename T>
+bool ct_theta_inv(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
+                  const double2* dR, cpx<T>* W, size_t ws, hipStream_t s, const Batch* bt) {
+  return bt ? ct_theta_inv_b<T, 1>(v, g, tw, r, u, dR, W, ws, s, bt)
+            : ct_theta_inv_b<T, 0>(v, g, tw, r, u, dR, W, ws, s, bt);
 }
 
 template <typename T>
 bool ct_theta_fwd(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* Y, size_t ws, const double* D,
-                  const double2* u, double2* N, hipStream_t s) {
-#define X(LL, ...) \
-  if (g.ntp == LL) return theta_fwd_plan<T, __VA_ARGS__>(v.fwd, g, tw, Y, ws, D, u, N, s);
-  NSP_THETA_PLANS(X)
-#undef X
-  return false;
+                  const double2* u, double2* N, hipStream_t s, const Batch* bt) {
+  return bt ? ct_theta_fwd_b<T, 1>(v, g, tw, Y, ws, D, u, N, s, bt)
+            : ct_theta_fwd_b<T, 0>(v, g, tw, Y, ws, D, u, N, s, bt);
 }
 
 template <typename T>
 bool ct_zpass(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* twz, const double* r,
-              const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s) {
-#define X(LL, ...) \
-  if (g.nh == LL) return zpass_plan<T, __VA_ARGS__>(v, g, tw, twz, r, W, ws, Y, s);
-  NSP_ZHALF_PLANS(X)
-#undef X
-  return false;
+              const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s, const Batch* bt) {
+  return bt ? ct_zpass_b<T, 1>(v, g, tw, twz, r, W, ws, Y, s, bt) : ct_zpass_b<T, 0>(v, g, tw, twz, r, W, ws, Y, s, bt);
 }
 
 }  // namespace nsp

@@ -1,0 +1,7 @@
+// FP32 batched (ensemble) instantiation of the compiled-plan nonlinear-term
+// kernels (nonlinear_ct.h): one member per blockIdx.y.
+#include "nonlinear_ct.h"
+
+namespace nsp {
+NSP_CT_INSTANTIATE(float, 1)
+}  // namespace nsp

@@ -3,12 +3,14 @@
 #include "nonlinear_ct.h"
 
 namespace nsp {
+NSP_CT_EXTERN(double, 1)
+NSP_CT_INSTANTIATE(double, 0)
 template bool ct_theta_inv<double>(const NlVariant&, const Geom&, const cpx<double>*, const double*, const double2*,
-                                  const double2*, cpx<double>*, size_t, hipStream_t);
+                                  const double2*, cpx<double>*, size_t, hipStream_t, const Batch*);
 template bool ct_theta_fwd<double>(const NlVariant&, const Geom&, const cpx<double>*, const cpx<double>*, size_t,
-                                  const double*, const double2*, double2*, hipStream_t);
+                                  const double*, const double2*, double2*, hipStream_t, const Batch*);
 template bool ct_zpass<double>(const NlVariant&, const Geom&, const cpx<double>*, const cpx<double>*, const double*,
-                              const cpx<double>*, size_t, cpx<double>*, hipStream_t);
+                              const cpx<double>*, size_t, cpx<double>*, hipStream_t, const Batch*);
 
 bool ct_has_theta(int L) {
 #define X(LL, ...) if (L == LL) return true;

@@ -50,6 +50,19 @@ struct Geom {
 
 __device__ __forceinline__ bool step_done(const Geom& g) { return g.skip != nullptr && *g.skip != 0; }
 
+// A pointer read from a device table of launch arguments is a generic pointer
+// to the compiler, and every access through it becomes a flat memory
+// instruction.  Re-deriving it from the table pointer (a kernel argument, which
+// the compiler knows to be global memory) by its byte distance lets the
+// accesses use global instructions, as for pointers passed as kernel arguments.
+// Only for pointers that are never null: the compiler takes the result of the
+// pointer arithmetic as non-null (a null test of it becomes a test of base).
+template <typename T>
+__device__ __forceinline__ T* gmem(const void* base, T* p) {
+  const long long off = (long long)(size_t)p - (long long)(size_t)base;
+  return (T*)((char*)const_cast<void*>(base) + off);
+}
+
 // Device tables of one pipe.
 struct DevTables {
   double* r;     // [NR]
@@ -68,6 +81,35 @@ struct DevInfluence {
   double2* uh;    // [3][N] homogeneous velocity solutions
   double2* uhP;   // [3][N] gradient of the homogeneous pressure solution
   double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
+};
+
+// Outcome of one captured step of one pipe, written on the device: converged
+// flag, corrector iterations, flags (st[3] unused); final err.
+struct StepRec {
+  int st[4];
+  double err;
+};
+
+// One pipe of a batched nonlinear launch (ensembles): blockIdx.y (dr3: .z)
+// selects the member, whose geometry, tables and fields replace the launch's
+// own arguments.  Blocks beyond the member's extent return at once.
+struct NlMember {
+  Geom g;
+  const double* r;  // [NR]
+  const double* D;  // [2][NR][7]
+  const double2* u; // input state
+  double2* dR;      // d_r u (FP32 pairs when f32)
+  void* W;          // theta-inverse intermediates (class precision)
+  void* Y;          // z-forward intermediates
+  double2* N;       // N(u) (FP32 pairs when f32)
+  size_t ws;        // intermediate field stride
+  int f32;          // mixed form
+};
+// A batched launch: members of one compiled-plan class, and the largest
+// extents among them (grid sizes).
+struct Batch {
+  const NlMember* tab;
+  int count, max_nr, max_nz, max_modes;
 };
 
 // Streaming linear sweep (linear2.hip) argument block.
@@ -96,6 +138,9 @@ struct Lin2Args {
 };
 void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, hipStream_t s);
 void launch_sweep2(const Lin2Args& a, hipStream_t s);
+// one launch per pass for `count` pipes (tab in device memory); max_nt/max_nz
+// are the largest extents among them
+void launch_sweep_batch(const Lin2Args* tab, int count, int predictor, int max_nt, int max_nz, hipStream_t s);
 // centred stencil table: w-th weight of row i multiplies column i-3+w
 std::vector<double> centred_table(int nr, const std::vector<double>& T);
 
@@ -215,21 +260,30 @@ std::vector<int> ct_radices(int L, bool theta);
 std::vector<double2> pass_twiddles(const std::vector<int>& radices);
 template <typename T>
 bool ct_theta_inv(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
-                  const double2* dR, cpx<T>* W, size_t ws, hipStream_t s);
+                  const double2* dR, cpx<T>* W, size_t ws, hipStream_t s, const Batch* bt = nullptr);
 template <typename T>
 bool ct_theta_fwd(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* Y, size_t ws, const double* D,
-                  const double2* u, double2* N, hipStream_t s);
+                  const double2* u, double2* N, hipStream_t s, const Batch* bt = nullptr);
 template <typename T>
 bool ct_zpass(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>* twz, const double* r,
-              const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s);
-// dR is stored as FP32 pairs when f32 (the mixed form)
-void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, bool f32, hipStream_t s);
+              const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s, const Batch* bt = nullptr);
+// dR is stored as FP32 pairs when f32 (the mixed form); with a batch, one
+// launch covers every member of bt.tab
+void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, bool f32, hipStream_t s,
+                const Batch* bt = nullptr);
 
 // Launchers (HIP sources).
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u,
                       double2* N, hipStream_t s, Profiler* pf);
 void launch_norm_reset(unsigned long long* norm, const int* skip, hipStream_t s);
 void launch_widen(const float2* in, double2* out, size_t count, hipStream_t s);
+void launch_add(double2* u, const double2* f, size_t count, hipStream_t s);
+// Ensemble bookkeeping, one thread per member: reset the records, reset the
+// norms of members still iterating, and the loop test after corrector `it`.
+void launch_ens_begin(StepRec* rec, int count, hipStream_t s);
+void launch_ens_norm_reset(unsigned long long* const* norms, const StepRec* rec, int count, hipStream_t s);
+void launch_ens_converge(unsigned long long* const* norms, StepRec* rec, int count, double tol, int it, int maxit,
+                         hipStream_t s);
 // Step bookkeeping of a captured step: st = {converged, corrector iterations, flags}.
 void launch_step_begin(int* st, double* err, hipStream_t s);
 void launch_converge(const unsigned long long* norm, int* st, double* err, double tol, int it, int maxit, hipStream_t s);

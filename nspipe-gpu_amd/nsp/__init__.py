@@ -20,6 +20,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "libnsp.so")
 
 FP64, MIXED = 0, 1
+IC_FLAT, IC_KOLMOGOROV, IC_PUFF, IC_ADD = 0, 1, 2, 16
 HOST, DEVICE = 0, 1
 FLAG_NAN, FLAG_MAXIT = 1, 2
 
@@ -41,7 +42,8 @@ EXPORTS = ["nsp_default_params", "nsp_create", "nsp_destroy", "nsp_set_state", "
            "nsp_radial_tables", "nsp_read_state_files", "nsp_write_state_files", "nsp_seeded_ic",
            "nsp_profile_enable", "nsp_profile_count", "nsp_profile_name", "nsp_profile_read",
            "nsp_autotune", "nsp_tuning", "nsp_set_tuning", "nsp_stream", "nsp_synchronize",
-           "nsp_last_error"]
+           "nsp_last_error", "nsp_ensemble_create", "nsp_ensemble_destroy", "nsp_ensemble_size",
+           "nsp_ensemble_member", "nsp_ensemble_autotune", "nsp_ensemble_step"]
 
 
 class NspError(RuntimeError):
@@ -85,8 +87,15 @@ def lib():
     L.nsp_tuning.restype = ctypes.c_char_p
     L.nsp_tuning.argtypes = [vp]
     L.nsp_set_tuning.argtypes = [vp, ctypes.c_char_p]
+    L.nsp_ensemble_create.argtypes = [ctypes.POINTER(Params), ip, ip, ctypes.POINTER(vp)]
+    L.nsp_ensemble_destroy.argtypes = [vp]
+    L.nsp_ensemble_size.argtypes = [vp]
+    L.nsp_ensemble_member.argtypes = [vp, ip]
+    L.nsp_ensemble_member.restype = vp
+    L.nsp_ensemble_autotune.argtypes = [vp, ip]
+    L.nsp_ensemble_step.argtypes = [vp, ip, ctypes.POINTER(StepInfo)]
     for name in EXPORTS:
-        if name not in ("nsp_last_error", "nsp_stream", "nsp_profile_name", "nsp_tuning"):
+        if name not in ("nsp_last_error", "nsp_stream", "nsp_profile_name", "nsp_tuning", "nsp_ensemble_member"):
             getattr(L, name).restype = ctypes.c_int
     _lib = L
     return L
@@ -226,3 +235,73 @@ class Pipe:
 
     def stream(self):
         return lib().nsp_stream(self.h)
+
+
+class _Member(Pipe):
+    """A pipe owned by an Ensemble (closing it is a no-op)."""
+
+    def __init__(self, h, params):
+        self.h = ctypes.c_void_p(h)
+        self.params = params
+        self.nr, self.nt, self.nz = params.nr, params.nt, params.nz
+
+    def close(self):
+        self.h = None
+
+
+class Ensemble:
+    """Independent pipes stepped together (nsp_ensemble_*): every stage is one
+    launch per class of members with the same transform plans, the whole step
+    one hipGraph.  `members` is a list of dicts of Pipe keyword arguments."""
+
+    def __init__(self, members, device=0):
+        n = len(members)
+        arr = (Params * n)()
+        for j, kw in enumerate(members):
+            kw = dict(kw)
+            p = arr[j]
+            _check(lib().nsp_default_params(ctypes.byref(p), kw.pop("nr"), kw.pop("nt"), kw.pop("nz"),
+                                            kw.pop("lz"), kw.pop("re"), kw.pop("dt")))
+            p.precision = kw.pop("precision", FP64)
+            p.d_im = kw.pop("d_im", p.d_im)
+            p.maxit = kw.pop("maxit", p.maxit)
+            p.tol = kw.pop("tol", p.tol)
+            p.qdes = kw.pop("qdes", p.qdes)
+            if kw:
+                raise TypeError(f"unknown member parameters {sorted(kw)}")
+        e = ctypes.c_void_p()
+        _check(lib().nsp_ensemble_create(arr, n, device, ctypes.byref(e)))
+        self.e = e
+        self.params = arr
+        self.members = [_Member(lib().nsp_ensemble_member(e, j), arr[j]) for j in range(n)]
+
+    def __len__(self):
+        return len(self.members)
+
+    def __getitem__(self, j):
+        return self.members[j]
+
+    def autotune(self, reps=2):
+        _check(lib().nsp_ensemble_autotune(self.e, reps))
+
+    def step(self, nsteps=1):
+        """Returns (n_it, err, flags) arrays of shape (nsteps, members)."""
+        n = len(self.members)
+        info = (StepInfo * max(nsteps * n, 1))()
+        _check(lib().nsp_ensemble_step(self.e, nsteps, info))
+        k = nsteps * n
+        n_it = np.array([info[q].n_it for q in range(k)], np.int32).reshape(nsteps, n)
+        err = np.array([info[q].err for q in range(k)]).reshape(nsteps, n)
+        flags = np.array([info[q].flags for q in range(k)], np.int32).reshape(nsteps, n)
+        return n_it, err, flags
+
+    def close(self):
+        if getattr(self, "e", None):
+            for m in self.members:
+                m.close()
+            lib().nsp_ensemble_destroy(self.e)
+            self.e = None
+
+    def __del__(self):
+        self.close()
+

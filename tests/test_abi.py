@@ -16,7 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     text = open(os.path.join(ROOT, "include", "nsp.h")).read()
-    return sorted(set(re.findall(r"^\s*(?:nsp_status|int|void\s*\*|const char\s*\*)\s*(nsp_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:nsp_status|int|void\s*\*|const char\s*\*|nsp_handle\s*\*)\s*(nsp_\w+)\s*\(", text, re.M)))
 
 
 def test_library_exports_every_declared_symbol():

@@ -127,6 +127,27 @@ nsp_status nsp_read_state_files(nsp_handle* h, const char* ur_path, const char* 
 nsp_status nsp_write_state_files(nsp_handle* h, const char* ur_path, const char* ut_path,
                                  const char* uz_path);
 
+/* Remesh (tools/Re_Mesh/src/mod_remesh.cu: re_mesh :50-124, rmsh_k :8-30,
+ * modeZZ_k :35-41): the state of src, on its (NR, NT, NZ), interpolated onto
+ * the grid of dst and made dst's state.  Radially each new point is the
+ * Lagrange interpolant of 2*3 old points of the grid extended through the
+ * axis (mirrored values carry the parity of the field: ur, ut odd in r for
+ * even m, uz even); the wall value is copied.  Modes present on both grids
+ * are carried over, new modes are zero (padding), others dropped
+ * (truncation); ur(0,0) and the imaginary parts of ut(0,0), uz(0,0) are set
+ * to zero.  No projection follows, as in the reference.  flags:
+ * NSP_REMESH_REFERENCE keeps the reference's column assignment of the
+ * mirrored points (exact only for rows whose stencil starts at the outermost
+ * mirrored point, :92-103); NSP_REMESH_MIRROR gives every mirrored point its
+ * own column.  src and dst are different handles on one device. */
+#define NSP_REMESH_REFERENCE 0
+#define NSP_REMESH_MIRROR 1
+nsp_status nsp_remesh(nsp_handle* dst, nsp_handle* src, int flags);
+
+/* The radial remesh matrix [2][nr_new][nr_old] (half 0 for fields even in r,
+ * half 1 odd; mod_remesh.cu:80-111) without a handle or device. */
+nsp_status nsp_remesh_weights(int nr_old, int nr_new, int flags, double* RM);
+
 /* Seeded synthetic initial condition (no reference counterpart: the
  * reference's only IC is nsp_reference_ic).  Poiseuille alpha*(1-r^2) scaled
  * to bulk flux qdes, plus an analytically divergence-free, no-slip

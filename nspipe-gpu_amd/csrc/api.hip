@@ -809,6 +809,52 @@ nsp_status nsp_radial_tables(int nr, double* r, double* D, double* Lu, double* L
   return NSP_OK;
 }
 
+nsp_status nsp_remesh_weights(int nr_old, int nr_new, int flags, double* RM) {
+  if (nr_old < 2 * IW + 2 || nr_new < 2 * IW + 2) return fail(NSP_ERR_INVALID, "nr must be >= 8");
+  if (!RM) return fail(NSP_ERR_INVALID, "null matrix");
+  if (flags & ~NSP_REMESH_MIRROR) return fail(NSP_ERR_INVALID, "unknown remesh flags");
+  std::vector<double> ro(nr_old), rn(nr_new);
+  radial_grid(nr_old, ro.data());
+  radial_grid(nr_new, rn.data());
+  const std::vector<double> M = remesh_matrix(ro, rn, (flags & NSP_REMESH_MIRROR) != 0);
+  std::memcpy(RM, M.data(), M.size() * sizeof(double));
+  return NSP_OK;
+}
+
+nsp_status nsp_remesh(nsp_handle* dst, nsp_handle* src, int flags) {
+  if (!dst || !src) return fail(NSP_ERR_INVALID, "null handle");
+  if (dst == src) return fail(NSP_ERR_INVALID, "remesh needs two handles");
+  if (dst->device != src->device) return fail(NSP_ERR_INVALID, "remesh handles are on different devices");
+  if (flags & ~NSP_REMESH_MIRROR) return fail(NSP_ERR_INVALID, "unknown remesh flags");
+  const Geom &gn = dst->g, &go = src->g;
+  const std::vector<double> RM = remesh_matrix(src->host.r, dst->host.r, (flags & NSP_REMESH_MIRROR) != 0);
+  // nonzero column band of each new row over both parity halves
+  std::vector<int2> band(gn.nr);
+  for (int i = 0; i < gn.nr; i++) {
+    int lo = go.nr, hi = -1;
+    for (int h2 = 0; h2 < 2; h2++)
+      for (int j = 0; j < go.nr; j++)
+        if (RM[((size_t)h2 * gn.nr + i) * go.nr + j] != 0.0) {
+          lo = std::min(lo, j);
+          hi = std::max(hi, j);
+        }
+    band[i] = make_int2(lo, hi);
+  }
+  // the matrix and bands live in dst's sweep scratch for the launch
+  const size_t mbytes = RM.size() * sizeof(double), bbytes = band.size() * sizeof(int2);
+  if (mbytes + bbytes > 3 * gn.n * sizeof(double2)) return fail(NSP_ERR_INVALID, "remesh matrix exceeds scratch");
+  HIPCHK(hipSetDevice(dst->device));
+  HIPCHK(hipStreamSynchronize(src->stream));
+  char* scr = reinterpret_cast<char*>(dst->SR);
+  HIPCHK(hipMemcpyAsync(scr, RM.data(), mbytes, hipMemcpyHostToDevice, dst->stream));
+  HIPCHK(hipMemcpyAsync(scr + mbytes, band.data(), bbytes, hipMemcpyHostToDevice, dst->stream));
+  launch_remesh(gn, go, reinterpret_cast<const double*>(scr), reinterpret_cast<const int2*>(scr + mbytes), src->u,
+                dst->u, dst->stream);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(dst->stream));
+  return NSP_OK;
+}
+
 nsp_status nsp_diagnostics(nsp_handle* h, double out[7]) {
   if (!h || !out) return fail(NSP_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(h->device));

@@ -23,6 +23,10 @@ struct RadialTables {
   std::vector<double> r, D, Lu, Lp, rdr, dr0, dr1;
 };
 void build_radial_tables(int nr, RadialTables& T);
+void radial_grid(int nr, double* r);
+// [2][nrn][nro] radial interpolation of the remesh (half 0 even, 1 odd in r)
+std::vector<double> remesh_matrix(const std::vector<double>& ro, const std::vector<double>& rn, bool mirror);
+
 
 // ----------------------------------------------------------------- FFT plan
 // In-place decimation-in-time mixed-radix plan; input is placed at perm[n].
@@ -278,6 +282,10 @@ void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const
 void launch_norm_reset(unsigned long long* norm, const int* skip, hipStream_t s);
 void launch_widen(const float2* in, double2* out, size_t count, hipStream_t s);
 void launch_add(double2* u, const double2* f, size_t count, hipStream_t s);
+// new state = remesh of the old (both on this device); RM and band [nrn] of
+// (first, last) nonzero columns in device memory
+void launch_remesh(const Geom& gn, const Geom& go, const double* RM, const int2* band, const double2* uo,
+                   double2* un, hipStream_t s);
 // Ensemble bookkeeping, one thread per member: reset the records, reset the
 // norms of members still iterating, and the loop test after corrector `it`.
 void launch_ens_begin(StepRec* rec, int count, hipStream_t s);

@@ -38,6 +38,86 @@ static void fornberg(double* c, const double* rau, int i0, int left, int right, 
   }
 }
 
+// Cosine grid clustered to the wall, then stretched ten times so the axis sits
+// midway to its mirror point (mod_radialFD.cu:211-216; the remesh tool builds
+// the same grid, tools/Re_Mesh/src/mod_remesh.cu:61-79).
+void radial_grid(int nr, double* r) {
+  const int np = nr + int(std::sqrt(double(nr)));
+  for (int i = np - nr; i < np; i++) {
+    const int ir = np - i - 1;
+    r[i - np + nr] = 0.5 * (1.0 + std::cos(0.5 * PI2 * (double(ir) / double(np))));
+  }
+  for (int it = 0; it < 10; it++) {
+    const double dr = 1.5 * r[0] - 0.5 * r[1];
+    for (int j = 0; j < nr; j++) r[j] = r[j] * (1.0 + dr) - dr;
+  }
+}
+
+// Interpolation weights at x0 from the points x[0..n-1]: the derivative-free
+// case of Fornberg's recurrence (tools/Re_Mesh/src/mod_remesh.cu:147-178).
+static void lagrange_weights(double* c, const double* x, double x0, int n) {
+  c[0] = 1.0;
+  double c1 = 1.0, c4 = x[0] - x0;
+  for (int i = 1; i < n; i++) {
+    double c2 = 1.0;
+    const double c5 = c4;
+    c4 = x[i] - x0;
+    for (int j = 0; j < i; j++) {
+      const double c3 = x[i] - x[j];
+      c2 = c2 * c3;
+      if (j == i - 1) c[i] = -c1 * c5 * c[i - 1] / c2;
+      c[j] = c4 * c[j] / c3;
+    }
+    c1 = c2;
+  }
+}
+
+// Radial remesh matrix [2][nrn][nro] (tools/Re_Mesh/src/mod_remesh.cu:80-111):
+// row i interpolates the old grid at rn[i] from 2*IW points of the old grid
+// extended through the axis (rau[p] = -ro[IW-1-p] for p < IW).  Half 0 adds a
+// reflected point's weight to its column (functions even in r), half 1
+// subtracts it (odd).  The wall row copies the old wall value.  In the
+// reference's column assignment a reflected point at stencil position j goes
+// to column IW-1-j, which is its own column only when the stencil starts at
+// rau[0]; mirror = true assigns column IW-1-p for the point rau[p] instead.
+std::vector<double> remesh_matrix(const std::vector<double>& ro, const std::vector<double>& rn, bool mirror) {
+  const int nro = (int)ro.size(), nrn = (int)rn.size();
+  std::vector<double> RM(2 * (size_t)nro * nrn, 0.0);
+  RM[(size_t)nro * nrn - 1] = 1.0;
+  RM[2 * (size_t)nro * nrn - 1] = 1.0;
+  std::vector<double> rau(nro + IW);
+  for (int p = 0; p < nro + IW; p++) rau[p] = (p < IW) ? -ro[IW - p - 1] : ro[p - IW];
+  double x[2 * IW], c[2 * IW];
+  for (int i = 0; i < nrn - 1; i++) {
+    const double x0 = rn[i];
+    int first = nro + IW - 1;
+    for (int p = 0; p < nro + IW; p++)
+      if (rau[p] >= x0) {
+        first = p;
+        break;
+      }
+    const int a = first - IW;                        // first stencil point (extended index)
+    const int b = std::min(nro + IW - 1, a + 2 * IW - 1);
+    const int n = b - a + 1;
+    for (int j = 0; j < n; j++) x[j] = rau[a + j];
+    lagrange_weights(c, x, x0, n);
+    const int s = a - IW;                            // its old-grid index (negative: reflected)
+    for (int j = 0; j < n; j++) {
+      double* row0 = &RM[(size_t)i * nro];
+      double* row1 = &RM[(size_t)nro * nrn + (size_t)i * nro];
+      if (s + j < 0) {
+        const int col = mirror ? IW - 1 - (a + j) : IW - 1 - j;
+        row0[col] += c[j];
+        row1[col] -= c[j];
+      } else {
+        row0[s + j] += c[j];
+        row1[s + j] += c[j];
+      }
+    }
+  }
+  return RM;
+}
+
 // Radial grid, parity-folded derivative tables, quadrature weights and the
 // Laplacians of the velocity (Dirichlet wall row) and pressure (Neumann wall
 // row): mod_radialFD.cu:256-369.  Table index: [parity][row][7], parity 0
@@ -58,15 +138,7 @@ void build_radial_tables(int nr, RadialTables& T) {
 
   // Cosine grid clustered to the wall, then stretched so the axis sits
   // midway to its mirror point (mod_radialFD.cu:268-273).
-  const int np = nr + int(std::sqrt(double(nr)));
-  for (int i = np - nr; i < np; i++) {
-    const int ir = np - i - 1;
-    r[i - np + nr] = 0.5 * (1.0 + std::cos(0.5 * PI2 * (double(ir) / double(np))));
-  }
-  for (int it = 0; it < 10; it++) {
-    const double dr = 1.5 * r[0] - 0.5 * r[1];
-    for (int j = 0; j < nr; j++) r[j] = r[j] * (1.0 + dr) - dr;
-  }
+  radial_grid(nr, r);
   ri[0] = 0.0;
   for (int i = 0; i < nr; i++) ri[i + 1] = r[i];
   for (int i = 0; i < nr + IW; i++) rau[i] = (i < IW) ? -r[IW - i - 1] : r[i - IW];

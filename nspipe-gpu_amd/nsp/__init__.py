@@ -23,6 +23,7 @@ FP64, MIXED = 0, 1
 IC_FLAT, IC_KOLMOGOROV, IC_PUFF, IC_ADD = 0, 1, 2, 16
 HOST, DEVICE = 0, 1
 FLAG_NAN, FLAG_MAXIT = 1, 2
+REMESH_REFERENCE, REMESH_MIRROR = 0, 1
 
 
 class Params(ctypes.Structure):
@@ -43,7 +44,8 @@ EXPORTS = ["nsp_default_params", "nsp_create", "nsp_destroy", "nsp_set_state", "
            "nsp_profile_enable", "nsp_profile_count", "nsp_profile_name", "nsp_profile_read",
            "nsp_autotune", "nsp_tuning", "nsp_set_tuning", "nsp_stream", "nsp_synchronize",
            "nsp_last_error", "nsp_ensemble_create", "nsp_ensemble_destroy", "nsp_ensemble_size",
-           "nsp_ensemble_member", "nsp_ensemble_autotune", "nsp_ensemble_step"]
+           "nsp_ensemble_member", "nsp_ensemble_autotune", "nsp_ensemble_step", "nsp_remesh",
+           "nsp_remesh_weights"]
 
 
 class NspError(RuntimeError):
@@ -94,6 +96,8 @@ def lib():
     L.nsp_ensemble_member.restype = vp
     L.nsp_ensemble_autotune.argtypes = [vp, ip]
     L.nsp_ensemble_step.argtypes = [vp, ip, ctypes.POINTER(StepInfo)]
+    L.nsp_remesh.argtypes = [vp, vp, ip]
+    L.nsp_remesh_weights.argtypes = [ip, ip, ip, P]
     for name in EXPORTS:
         if name not in ("nsp_last_error", "nsp_stream", "nsp_profile_name", "nsp_tuning", "nsp_ensemble_member"):
             getattr(L, name).restype = ctypes.c_int
@@ -117,6 +121,18 @@ def radial_tables(nr):
     _check(lib().nsp_radial_tables(nr, _dp(r), _dp(D), _dp(Lu), _dp(Lp), _dp(rdr), _dp(dr0), _dp(dr1)))
     return dict(r=r, D=D.reshape(2, nr, 7), Lu=Lu.reshape(2, nr, 7), Lp=Lp.reshape(2, nr, 7),
                 rdr=rdr, dr0=dr0, dr1=dr1)
+
+
+def remesh_weights(nr_old, nr_new, flags=REMESH_REFERENCE):
+    """Host-only radial remesh matrix [2][nr_new][nr_old] (half 0 even in r, 1 odd)."""
+    RM = np.zeros((2, nr_new, nr_old))
+    _check(lib().nsp_remesh_weights(nr_old, nr_new, flags, _dp(RM)))
+    return RM
+
+
+def remesh(dst, src, flags=REMESH_REFERENCE):
+    """Interpolate src's state onto dst's grid and make it dst's state."""
+    _check(lib().nsp_remesh(dst.h, src.h, flags))
 
 
 class Pipe:

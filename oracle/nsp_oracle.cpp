@@ -910,7 +910,128 @@ struct Oracle {
 // C API (ctypes).  Complex fields are interleaved (re,im) doubles in the
 // reference layout double2[NR][NT][NZ], h = i*NT*NZ + it*NZ + l.
 // ============================================================================
+// ---------------------------------------------------------------------------
+// Remesh tool (tools/Re_Mesh/src/mod_remesh.cu).  Pinning (tests/test_oracle.py):
+// exact interpolation of even/odd polynomials of degree <= 5 on the rows the
+// reference's column assignment leaves intact (every row with the mirror
+// assignment), identity on an unchanged grid, spectral pad/truncate round trip.
+// ---------------------------------------------------------------------------
+namespace remesh {
+
+// mod_remesh.cu:61-66 and 72-79 (the same grid as mod_radialFD.cu:211-216)
+std::vector<double> grid(int nr) {
+  std::vector<double> r(nr);
+  int N_p = nr + int(std::sqrt(double(nr)));
+  for (int i = N_p - nr; i < N_p; i++) {
+    int ir = N_p - i - 1;
+    r[i - N_p + nr] = 0.5 * (1.0 + std::cos(0.5 * PI2 * (double(ir) / double(N_p))));
+  }
+  for (int it = 0; it < 10; it++) {
+    double dr = 1.5 * r[0] - 0.5 * r[1];
+    for (int j = 0; j < nr; j++) r[j] = r[j] * (1.0 + dr) - dr;
+  }
+  return r;
+}
+
+// mod_remesh.cu:147-178 with m = 0 (the derivative loops are empty)
+void fd_weights(double* c, const double* x, double x0, int n) {
+  int m = 0;
+  double c1, c2, c3, c4, c5;
+  c[0] = 1.0; c1 = 1.0; c4 = x[0] - x0;
+  for (int i = 1; i < n; i++) {
+    int mn = std::min(i, m);
+    (void)mn;
+    c2 = 1.0; c5 = c4; c4 = x[i] - x0;
+    for (int j = 0; j < i; j++) {
+      c3 = x[i] - x[j]; c2 = c2 * c3;
+      if (j == i - 1) c[i] = -c1 * c5 * c[i - 1] / c2;
+      c[j] = c4 * c[j] / c3;
+    }
+    c1 = c2;
+  }
+}
+
+// mod_remesh.cu:80-111; mirror = 1 puts a reflected point rau[p] in column
+// IW-1-p (the reference uses IW-1-j for stencil position j)
+std::vector<double> matrix(int NR, int Nrd, int mirror) {
+  std::vector<double> r_o = grid(NR), r_h = grid(Nrd);
+  std::vector<double> rau(NR + IW);
+  for (int i = 0; i < NR + IW; i++) rau[i] = (i < IW) ? -r_o[IW - i - 1] : r_o[i - IW];
+  std::vector<double> RM(2 * (size_t)NR * Nrd, 0.0);
+  RM[(size_t)NR * Nrd - 1] = 1.0; RM[2 * (size_t)NR * Nrd - 1] = 1.0;
+  for (int i = 0; i < Nrd - 1; i++) {
+    double x0 = r_h[i];
+    int jleft = 0;
+    for (int j = 0; j < NR + IW; j++) if (rau[j] >= x0) { jleft = j; break; }
+    jleft = jleft - IW;
+    int jright = std::min(NR + IW - 1, jleft + 2 * IW - 1), n = jright - jleft + 1;
+    std::vector<double> x(n), c(n, 0.0);
+    for (int j = 0; j < n; j++) x[j] = rau[jleft + j];
+    fd_weights(c.data(), x.data(), x0, n);
+    int jrau = jleft;
+    jleft = jleft - IW;
+    int jaux = jleft;
+    for (int j = 0; j < n; j++) {
+      if (jaux < 0) {
+        int jcol = mirror ? IW - 1 - (jrau + j) : IW - j - 1;
+        jaux += 1;
+        RM[(size_t)i * NR + jcol] += c[j]; RM[(size_t)NR * Nrd + (size_t)i * NR + jcol] -= c[j];
+      } else {
+        int jcol = jleft + j;
+        RM[(size_t)i * NR + jcol] += c[j]; RM[(size_t)NR * Nrd + (size_t)i * NR + jcol] += c[j];
+      }
+    }
+  }
+  return RM;
+}
+
+}  // namespace remesh
+
 extern "C" {
+
+// Remesh matrix [2][nr_new][nr_old] (half 0: ur/ut of odd m, uz of even m).
+void nspo_remesh_weights(int nr_old, int nr_new, int mirror, double* RM) {
+  std::vector<double> M = remesh::matrix(nr_old, nr_new, mirror);
+  std::copy(M.begin(), M.end(), RM);
+}
+
+// re_mesh (mod_remesh.cu:50-124): uo is [3][NR][NT][NZ] complex, un receives
+// [3][Nrd][Ntd][Nzd] (fully overwritten; modes with no source are zero).
+// rmsh_k (:8-30) scatters every old mode inside the new spectral range with
+// the dense radial sum; modeZZ_k (:35-41) then clears ur(0,0) and the
+// imaginary parts of ut(0,0), uz(0,0).
+void nspo_remesh(int NR, int NT, int NZ, const double* uo_, int Nrd, int Ntd, int Nzd, double* un_,
+                 int mirror) {
+  const cplx* uo = (const cplx*)uo_;
+  cplx* un = (cplx*)un_;
+  std::vector<double> RM = remesh::matrix(NR, Nrd, mirror);
+  const size_t no = (size_t)NR * NT * NZ, nn = (size_t)Nrd * Ntd * Nzd;
+  for (size_t q = 0; q < 3 * nn; q++) un[q] = {0.0, 0.0};
+  for (int c = 0; c < 3; c++) {
+    int flag = c < 2 ? 1 : 0;
+    for (int k_o = 0; k_o < NT * NZ; k_o++)
+      for (int h_or = 0; h_or < Nrd; h_or++) {
+        int h_oz = k_o % NZ, it = k_o / NZ, h_ot = it;
+        h_ot = h_ot < NT / 2 ? h_ot : h_ot - NT;
+        if (h_ot >= Ntd / 2 || h_ot < -Ntd / 2 || h_oz >= Nzd) continue;
+        size_t ip0 = (size_t)std::abs((std::abs(h_ot) % 2) - flag) * NR * Nrd;
+        if (h_ot < 0) h_ot = h_ot + Ntd;
+        size_t h = (size_t)h_or * Ntd * Nzd + (size_t)h_ot * Nzd + h_oz;
+        cplx du = {0.0, 0.0};
+        for (int j = 0; j < NR; j++) {
+          double rm = RM[(size_t)h_or * NR + j + ip0];
+          const cplx a = uo[c * no + (size_t)j * NZ * NT + k_o];
+          du.x += rm * a.x; du.y += rm * a.y;
+        }
+        un[c * nn + h] = du;
+      }
+  }
+  for (int i = 0; i < Nrd; i++) {
+    size_t h = (size_t)i * Ntd * Nzd;
+    un[h] = {0.0, 0.0}; un[nn + h].y = 0.0; un[2 * nn + h].y = 0.0;
+  }
+}
+
 
 void* nspo_create(int nr, int nt, int nz, double lz, double re, double dt, double d_im, int maxit,
                   double tol, double qdes) {

@@ -49,6 +49,8 @@ def lib():
         L.nspo_fft2_forward.argtypes = [vp, P, P]
         L.nspo_step.argtypes = [vp, ip, IP, P]
         L.nspo_diagnostics.argtypes = [vp, P]
+        L.nspo_remesh_weights.argtypes = [ip, ip, ip, P]
+        L.nspo_remesh.argtypes = [ip, ip, ip, P, ip, ip, ip, P, ip]
         _lib = L
     return _lib
 
@@ -198,3 +200,19 @@ def set_threads(n):
 
 def max_threads():
     return lib().nspo_max_threads()
+
+
+def remesh_weights(nr_old, nr_new, mirror=0):
+    """Remesh matrix [2][nr_new][nr_old] of the reference's remesh tool."""
+    RM = np.zeros((2, nr_new, nr_old))
+    lib().nspo_remesh_weights(nr_old, nr_new, mirror, _p(RM))
+    return RM
+
+
+def remesh(fields, new_shape, mirror=0):
+    """Remesh (ur, ut, uz) of shape (NR, NT, NZ) onto new_shape = (NR', NT', NZ')."""
+    uo = np.ascontiguousarray(np.stack(fields), np.complex128)
+    nr, nt, nz = uo.shape[1:]
+    un = np.zeros((3,) + tuple(new_shape), np.complex128)
+    lib().nspo_remesh(nr, nt, nz, _p(uo), *new_shape, _p(un), mirror)
+    return un[0], un[1], un[2]

@@ -49,3 +49,20 @@ def test_invalid_parameters_are_errors_not_exits():
         assert L.nsp_create(ctypes.byref(q), 0, ctypes.byref(h)) == 1, field
         assert L.nsp_last_error()
     assert L.nsp_radial_tables(4, *([None] * 7)) == 1
+
+
+@pytest.mark.parametrize("nro,nrn", [(8, 8), (48, 64), (64, 48), (32, 96), (96, 160)])
+@pytest.mark.parametrize("flags", [nsp.REMESH_REFERENCE, nsp.REMESH_MIRROR])
+def test_product_remesh_weights_equal_oracle(nro, nrn, flags):
+    """Host setup of nsp_remesh (the radial matrix) equals the oracle's
+    restatement of mod_remesh.cu:80-111 bit for bit."""
+    import oracle_lib
+    assert np.array_equal(nsp.remesh_weights(nro, nrn, flags), oracle_lib.remesh_weights(nro, nrn, flags))
+
+
+def test_remesh_weights_reject_bad_arguments():
+    L = nsp.lib()
+    buf = np.zeros(2 * 8 * 8)
+    assert L.nsp_remesh_weights(4, 8, 0, nsp._dp(buf)) == 1
+    assert L.nsp_remesh_weights(8, 8, 2, nsp._dp(buf)) == 1
+    assert L.nsp_remesh_weights(8, 8, 0, None) == 1

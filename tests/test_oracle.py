@@ -3,13 +3,15 @@ each of its parts independently of how the oracle computes them:
 polynomial exactness of the finite-difference tables, numpy's FFTs, dense
 solves, numpy's 4x4 inverse, analytic Helmholtz/Poisson solutions, the
 discrete Poiseuille fixed point, the closed-form decay of an m=0 swirl
-eigenvector, and a brute-force pseudo-spectral evaluation of the nonlinear
-term.  The reference ships no tests or golden vectors of its own
+eigenvector, a brute-force pseudo-spectral evaluation of the nonlinear
+term, and, for the remesh tool, parity-polynomial exactness of its radial
+interpolation, the identity on an unchanged grid and a spectral round trip.  The reference ships no tests or golden vectors of its own
 (its repository holds none), so these are the pins.
 """
 import numpy as np
 import pytest
 
+import oracle_lib
 from oracle_lib import Oracle
 
 IW = 3
@@ -444,3 +446,77 @@ def test_nonlinear_brute_force():
     ref[2][:, 0, 0] -= 2.0 / o.re * dw
     for c in range(3):
         assert np.abs(out[c] - ref[c]).max() < 1e-12 * np.abs(ref[c]).max(), c
+
+
+# ---------------------------------------------------------------- remesh tool
+REMESH_PAIRS = [(8, 8), (48, 64), (64, 48), (32, 96), (16, 12), (96, 160)]
+
+
+def remesh_grid(nr):
+    """Radial grid of the remesh tool (tools/Re_Mesh/src/mod_remesh.cu:61-79)."""
+    n_p = nr + int(np.sqrt(nr))
+    r = np.zeros(nr)
+    for i in range(n_p - nr, n_p):
+        ir = n_p - i - 1
+        r[i - n_p + nr] = 0.5 * (1.0 + np.cos(np.pi * ir / n_p))
+    for _ in range(10):
+        dr = 1.5 * r[0] - 0.5 * r[1]
+        r = r * (1.0 + dr) - dr
+    return r
+
+
+@pytest.mark.parametrize("nro,nrn", REMESH_PAIRS)
+def test_remesh_mirror_weights_interpolate_parity_polynomials(nro, nrn):
+    """Six-point Lagrange interpolation through the axis is exact for
+    polynomials of degree <= 5 of the field's parity: half 0 (mirrored with +)
+    for even powers of r, half 1 (mirrored with -) for odd powers."""
+    RM = oracle_lib.remesh_weights(nro, nrn, 1)
+    ro, rn = remesh_grid(nro), remesh_grid(nrn)
+    for p in range(3):
+        np.testing.assert_allclose(RM[0] @ ro ** (2 * p), rn ** (2 * p), rtol=0, atol=1e-8)
+        np.testing.assert_allclose(RM[1] @ ro ** (2 * p + 1), rn ** (2 * p + 1), rtol=0, atol=1e-8)
+    assert RM[0, -1, -1] == 1.0 and RM[1, -1, -1] == 1.0 and np.count_nonzero(RM[:, -1]) == 2
+
+
+@pytest.mark.parametrize("nro,nrn", REMESH_PAIRS)
+def test_remesh_reference_assignment_differs_only_where_stencil_starts_inside(nro, nrn):
+    """The reference maps a mirrored point at stencil position j to column
+    IW-1-j, which is its own column only when the stencil starts at the
+    outermost mirrored point.  That happens unless ro[0] < rn[i] <= ro[IW-1]
+    (stencil start rau[1] or rau[2]): exactly those rows differ from the mirror
+    assignment, and the others are identical."""
+    ref = oracle_lib.remesh_weights(nro, nrn, 0)
+    mir = oracle_lib.remesh_weights(nro, nrn, 1)
+    ro, rn = remesh_grid(nro), remesh_grid(nrn)
+    expect = {i for i in range(nrn - 1) if ro[0] < rn[i] <= ro[IW - 1]}
+    differ = {i for i in range(nrn) if not np.array_equal(ref[:, i], mir[:, i])}
+    assert differ <= expect
+    if nro != nrn:  # on an unchanged grid every row hits a node: no mirrored weight
+        assert differ == expect
+    # both assignments keep each row's weights summing to one on even fields
+    np.testing.assert_allclose(ref[0].sum(axis=1), 1.0, atol=1e-12)
+
+
+def test_remesh_same_grid_is_identity():
+    RM = oracle_lib.remesh_weights(24, 24, 0)
+    np.testing.assert_allclose(RM[0], np.eye(24), atol=1e-14)
+    np.testing.assert_allclose(RM[1], np.eye(24), atol=1e-14)
+
+
+def test_remesh_spectral_pad_then_truncate_round_trip():
+    """Padding to more azimuthal/axial modes and truncating back recovers the
+    field (same NR), apart from the (0,0) adjustment of modeZZ_k."""
+    rng = np.random.default_rng(11)
+    shape = (8, 8, 9)
+    f = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(3)]
+    big = oracle_lib.remesh(f, (8, 12, 17))
+    back = oracle_lib.remesh(big, shape)
+    for c in range(3):
+        g = f[c].copy()
+        if c == 0:
+            g[:, 0, 0] = 0.0
+        else:
+            g[:, 0, 0] = g[:, 0, 0].real
+        np.testing.assert_allclose(back[c], g, rtol=0, atol=1e-14)
+        # new modes are zero: m = +-4, +-5 rows and l >= 9 of the padded field
+        assert np.all(big[c][:, 4:8, :] == 0) and np.all(big[c][:, :, 9:] == 0)

@@ -30,7 +30,7 @@ typedef enum {
   NSP_OK = 0,
   NSP_ERR_INVALID = 1,   /* bad argument or unsupported size */
   NSP_ERR_HIP = 2,       /* HIP runtime error (no device, launch failure, ...) */
-  NSP_ERR_IO = 3,        /* restart file missing or short */
+  NSP_ERR_IO = 3,        /* file missing, short, or not writable */
   NSP_ERR_NOMEM = 4      /* device or host allocation failed */
 } nsp_status;
 
@@ -127,6 +127,18 @@ nsp_status nsp_read_state_files(nsp_handle* h, const char* ur_path, const char* 
 nsp_status nsp_write_state_files(nsp_handle* h, const char* ur_path, const char* ut_path,
                                  const char* uz_path);
 
+/* Restart directory with metadata.  The reference keeps no time, step or
+ * parameters with its state files (main.cu:42-44 writes the three raw
+ * files only).  nsp_write_restart writes dir/ur.bin, ut.bin, uz.bin in the
+ * reference's raw format and dir/restart.txt ("key value" lines: format,
+ * time, step, nr, nt, nz, lz, re, dt, d_im, maxit, tol, qdes, precision).
+ * nsp_read_restart reads them back into the state and returns time and step;
+ * it fails with NSP_ERR_INVALID if the metadata's grid differs from the
+ * handle's.  A directory holding only the three raw files (a reference
+ * restart) reads with time 0 and step 0.  t and step may be NULL. */
+nsp_status nsp_write_restart(nsp_handle* h, const char* dir, double t, long long step);
+nsp_status nsp_read_restart(nsp_handle* h, const char* dir, double* t, long long* step);
+
 /* Remesh (tools/Re_Mesh/src/mod_remesh.cu: re_mesh :50-124, rmsh_k :8-30,
  * modeZZ_k :35-41): the state of src, on its (NR, NT, NZ), interpolated onto
  * the grid of dst and made dst's state.  Radially each new point is the
@@ -147,6 +159,44 @@ nsp_status nsp_remesh(nsp_handle* dst, nsp_handle* src, int flags);
 /* The radial remesh matrix [2][nr_new][nr_old] (half 0 for fields even in r,
  * half 1 odd; mod_remesh.cu:80-111) without a handle or device. */
 nsp_status nsp_remesh_weights(int nr_old, int nr_new, int flags, double* RM);
+
+/* Physical-space velocity component comp (0 ur, 1 ut, 2 uz) of the current
+ * state: padForward + fftBackward of the reference (mod_io_nonlinear.cu:173-195,
+ * mod_fft.cu:43-49), the field its commented HDF5 writer stores (:486-521).
+ * out[NR][NTP][NZF+2] doubles, NTP = 3NT/2, NZF = 3(NZ-1): row (i, j) holds
+ * u(r_i, theta_j = 2 pi j/NTP, z_n = n LZ/NZF) for n < NZF, then the two
+ * padding doubles of the in-place real transform (zero).  where: NSP_HOST or
+ * NSP_DEVICE.  Uses the nonlinear-term workspace (nothing else changes). */
+nsp_status nsp_physical_field(nsp_handle* h, int comp, double* out, int where);
+
+/* Cross-section integrals at the NZF axial points (wrt_qcr, wrt_ucr,
+ * mod_io_nonlinear.cu:576-654: q0kernel_k/u0kernel_k :55-79, the periodic
+ * three-point rule of q1kernel_k :81-103, intq mod_cublasflux.cu:119-130):
+ * q[n] = sum_i rdr_i sum_j 0.5/(3 NTP) (f_{j-1} + 4 f_j + f_{j+1}) with
+ * f = ur^2 + ut^2, u2[n] the same with f = uz^2.  Host arrays; either may be
+ * NULL. */
+nsp_status nsp_cross_sections(nsp_handle* h, double* q, double* u2);
+
+/* Real part of uz at mode (0,0) on every level, prof[NR] (wrt_mnp_k,
+ * mod_io_nonlinear.cu:119-130). */
+nsp_status nsp_mean_profile(nsp_handle* h, double* prof);
+
+/* Diagnostics files in the reference's formats (setIO, mod_io_nonlinear.cu:255-312;
+ * wrt_frc :460-479; wrt_qcr/wrt_ucr :576-654; wrt_mpr :663-687).
+ * nsp_io_open creates (truncates) io_friction.txt, io_qcross.txt,
+ * io_ucross.txt and io_meanpr.txt in dir and writes the header rows (axial
+ * positions, radii).  nsp_io_write appends, for time t, the lines selected by
+ * mask: NSP_IO_FRICTION (t, u_cl, u_tau), NSP_IO_CROSS (t, q[NZF]; t,
+ * u2[NZF]), NSP_IO_MEANPR (t, uz00[NR]); values as "%.8e" separated by two
+ * spaces.  Lines are written at once (the reference writes batches of dc_*
+ * lines and drops a final partial batch).  nsp_io_close closes the files
+ * (nsp_destroy does too). */
+#define NSP_IO_FRICTION 1
+#define NSP_IO_CROSS 2
+#define NSP_IO_MEANPR 4
+nsp_status nsp_io_open(nsp_handle* h, const char* dir);
+nsp_status nsp_io_write(nsp_handle* h, double t, int mask);
+nsp_status nsp_io_close(nsp_handle* h);
 
 /* Seeded synthetic initial condition (no reference counterpart: the
  * reference's only IC is nsp_reference_ic).  Poiseuille alpha*(1-r^2) scaled

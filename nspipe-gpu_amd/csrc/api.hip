@@ -58,6 +58,8 @@ struct nsp_handle {
   StepRec* rec_h = nullptr;  // pinned host
   hipGraphExec_t gexec[3] = {};
   std::vector<void*> allocs;
+  // diagnostics files: friction, qcross, ucross, meanpr
+  FILE* io[4] = {};
   Profiler prof;
   std::string tune_report = "{\"tuned\": false}";
 };
@@ -81,7 +83,15 @@ void drop_graphs(nsp_handle* h) {
   }
 }
 
+void close_io(nsp_handle* h) {
+  for (FILE*& f : h->io) {
+    if (f) std::fclose(f);
+    f = nullptr;
+  }
+}
+
 void free_all(nsp_handle* h) {
+  close_io(h);
   drop_graphs(h);
   if (h->rec_h) (void)hipHostFree(h->rec_h);
   h->rec_h = nullptr;
@@ -855,6 +865,152 @@ nsp_status nsp_remesh(nsp_handle* dst, nsp_handle* src, int flags) {
   return NSP_OK;
 }
 
+#define CHECK(x)             \
+  do {                       \
+    nsp_status s_ = (x);     \
+    if (s_) return s_;       \
+  } while (0)
+
+// Workspace of the physical-space output: the nonlinear-term intermediates,
+// which hold nothing between steps.  Wt (NR*NTP*NZ complex) first, then the
+// physical rows (NR*NTP rows of NZF+2 doubles).
+static nsp_status phys_scratch(nsp_handle* h, double2** Wt, double** F) {
+  const Geom& g = h->g;
+  char* base = h->nb.mixed ? reinterpret_cast<char*>(h->nb.W32) : reinterpret_cast<char*>(h->nb.W);
+  const size_t cap = 9 * h->nb.wstride * (h->nb.mixed ? sizeof(float2) : sizeof(double2));
+  const size_t wt = (size_t)g.nr * g.ntp * g.nz * sizeof(double2);
+  const size_t fb = (size_t)g.nr * g.ntp * (g.nzf + 2) * sizeof(double);
+  if (wt + fb > cap) return fail(NSP_ERR_INVALID, "physical-space output exceeds the workspace");
+  *Wt = reinterpret_cast<double2*>(base);
+  *F = reinterpret_cast<double*>(base + wt);
+  return NSP_OK;
+}
+
+nsp_status nsp_physical_field(nsp_handle* h, int comp, double* out, int where) {
+  if (!h || !out) return fail(NSP_ERR_INVALID, "null argument");
+  if (comp < 0 || comp > 2) return fail(NSP_ERR_INVALID, "comp must be 0, 1 or 2");
+  if (where != NSP_HOST && where != NSP_DEVICE) return fail(NSP_ERR_INVALID, "bad location");
+  HIPCHK(hipSetDevice(h->device));
+  const Geom& g = h->g;
+  double2* Wt;
+  double* F;
+  CHECK(phys_scratch(h, &Wt, &F));
+  const size_t pitch = g.nzf + 2;
+  double* dst = where == NSP_DEVICE ? out : F;
+  launch_physical(g, h->nb, h->u + (size_t)comp * g.n, Wt, dst, pitch, 0, h->stream);
+  HIPCHK(hipGetLastError());
+  if (where == NSP_HOST)
+    HIPCHK(hipMemcpyAsync(out, F, (size_t)g.nr * g.ntp * pitch * sizeof(double), hipMemcpyDeviceToHost,
+                          h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return NSP_OK;
+}
+
+nsp_status nsp_cross_sections(nsp_handle* h, double* q, double* u2) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  HIPCHK(hipSetDevice(h->device));
+  const Geom& g = h->g;
+  double2* Wt;
+  double* F;
+  CHECK(phys_scratch(h, &Wt, &F));
+  const size_t pitch = g.nzf + 2;
+  double* q1 = reinterpret_cast<double*>(h->SR);
+  double* qd = q1 + (size_t)g.nr * g.nzf;
+  for (int which = 0; which < 2; which++) {
+    double* dst = which == 0 ? q : u2;
+    if (!dst) continue;
+    if (which == 0) {  // q0kernel_k: ur^2 + ut^2
+      launch_physical(g, h->nb, h->u, Wt, F, pitch, 1, h->stream);
+      launch_physical(g, h->nb, h->u + g.n, Wt, F, pitch, 2, h->stream);
+    } else {  // u0kernel_k: uz^2
+      launch_physical(g, h->nb, h->u + 2 * g.n, Wt, F, pitch, 1, h->stream);
+    }
+    launch_cross(g, F, pitch, h->t.rdr, q1, qd, h->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst, qd, g.nzf * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return NSP_OK;
+}
+
+nsp_status nsp_mean_profile(nsp_handle* h, double* prof) {
+  if (!h || !prof) return fail(NSP_ERR_INVALID, "null argument");
+  HIPCHK(hipSetDevice(h->device));
+  const Geom& g = h->g;
+  // real part of uz at mode (0,0) of every level (wrt_mnp_k)
+  HIPCHK(hipMemcpy2DAsync(prof, sizeof(double), h->u + 2 * g.n, g.modes * sizeof(double2), sizeof(double), g.nr,
+                          hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return NSP_OK;
+}
+
+nsp_status nsp_io_open(nsp_handle* h, const char* dir) {
+  if (!h || !dir) return fail(NSP_ERR_INVALID, "null argument");
+  close_io(h);
+  static const char* names[4] = {"io_friction.txt", "io_qcross.txt", "io_ucross.txt", "io_meanpr.txt"};
+  for (int f = 0; f < 4; f++) {
+    const std::string path = std::string(dir) + "/" + names[f];
+    h->io[f] = std::fopen(path.c_str(), "w");
+    if (!h->io[f]) {
+      close_io(h);
+      return fail(NSP_ERR_IO, "cannot create " + path);
+    }
+  }
+  // header rows: the axial sample positions (with a leading 0), the radii
+  const Geom& g = h->g;
+  const double lz = PI2 / g.kz0;
+  for (int f = 1; f <= 2; f++) {
+    std::fprintf(h->io[f], "%.8e  ", 0.0);
+    for (int n = 0; n < g.nzf; n++) std::fprintf(h->io[f], "%.8e  ", lz * double(n) / double(g.nzf));
+    std::fprintf(h->io[f], "\n");
+  }
+  std::fprintf(h->io[3], "%.8e  ", 0.0);
+  for (int i = 0; i < g.nr; i++) std::fprintf(h->io[3], "%.8e  ", h->host.r[i]);
+  std::fprintf(h->io[3], "\n");
+  for (FILE* f : h->io) std::fflush(f);
+  return NSP_OK;
+}
+
+nsp_status nsp_io_write(nsp_handle* h, double t, int mask) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  if (!h->io[0]) return fail(NSP_ERR_INVALID, "diagnostics files are not open");
+  const Geom& g = h->g;
+  if (mask & NSP_IO_FRICTION) {
+    double d[7];
+    CHECK(nsp_diagnostics(h, d));
+    std::fprintf(h->io[0], "%.8e  %.8e  %.8e\n", t, d[0], d[1]);
+    std::fflush(h->io[0]);
+  }
+  if (mask & NSP_IO_CROSS) {
+    std::vector<double> q(g.nzf), u2(g.nzf);
+    CHECK(nsp_cross_sections(h, q.data(), u2.data()));
+    for (int f = 1; f <= 2; f++) {
+      const std::vector<double>& v = f == 1 ? q : u2;
+      std::fprintf(h->io[f], "%.8e  ", t);
+      for (double x : v) std::fprintf(h->io[f], "%.8e  ", x);
+      std::fprintf(h->io[f], "\n");
+      std::fflush(h->io[f]);
+    }
+  }
+  if (mask & NSP_IO_MEANPR) {
+    std::vector<double> prof(g.nr);
+    CHECK(nsp_mean_profile(h, prof.data()));
+    std::fprintf(h->io[3], "%.8e  ", t);
+    for (double x : prof) std::fprintf(h->io[3], "%.8e  ", x);
+    std::fprintf(h->io[3], "\n");
+    std::fflush(h->io[3]);
+  }
+  return NSP_OK;
+}
+
+nsp_status nsp_io_close(nsp_handle* h) {
+  if (!h) return fail(NSP_ERR_INVALID, "null handle");
+  close_io(h);
+  return NSP_OK;
+}
+
+#undef CHECK
+
 nsp_status nsp_diagnostics(nsp_handle* h, double out[7]) {
   if (!h || !out) return fail(NSP_ERR_INVALID, "null argument");
   HIPCHK(hipSetDevice(h->device));
@@ -897,6 +1053,7 @@ nsp_status nsp_write_state_files(nsp_handle* h, const char* a, const char* b, co
   std::vector<double> buf(2 * h->g.n);
   const char* names[3] = {a, b, c};
   HIPCHK(hipSetDevice(h->device));
+  HIPCHK(hipStreamSynchronize(h->stream));
   for (int q = 0; q < 3; q++) {
     HIPCHK(hipMemcpy(buf.data(), h->u + q * h->g.n, h->g.n * sizeof(double2), hipMemcpyDeviceToHost));
     FILE* f = std::fopen(names[q], "wb");
@@ -920,6 +1077,62 @@ nsp_status nsp_read_state_files(nsp_handle* h, const char* a, const char* b, con
     if (r != h->g.n) return fail(NSP_ERR_IO, std::string("reading error (short file): ") + names[q]);
   }
   return nsp_set_state(h, buf.data(), buf.data() + 2 * h->g.n, buf.data() + 4 * h->g.n, NSP_HOST);
+}
+
+// Restart directory: the three raw state files of the reference plus a
+// metadata file, restart.txt, of "key value" lines.
+static const char* const kRestartMeta = "restart.txt";
+
+nsp_status nsp_write_restart(nsp_handle* h, const char* dir, double t, long long step) {
+  if (!h || !dir) return fail(NSP_ERR_INVALID, "null argument");
+  const std::string d(dir);
+  nsp_status st = nsp_write_state_files(h, (d + "/ur.bin").c_str(), (d + "/ut.bin").c_str(), (d + "/uz.bin").c_str());
+  if (st) return st;
+  const std::string path = d + "/" + kRestartMeta;
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) return fail(NSP_ERR_IO, "writing error: " + path);
+  const nsp_params& P = h->P;
+  std::fprintf(f, "format nspipe-restart-1\n");
+  std::fprintf(f, "time %.17g\nstep %lld\n", t, step);
+  std::fprintf(f, "nr %d\nnt %d\nnz %d\n", P.nr, P.nt, P.nz);
+  std::fprintf(f, "lz %.17g\nre %.17g\ndt %.17g\nd_im %.17g\nmaxit %d\ntol %.17g\nqdes %.17g\nprecision %d\n", P.lz,
+               P.re, P.dt, P.d_im, P.maxit, P.tol, P.qdes, P.precision);
+  const bool ok = std::fclose(f) == 0;
+  if (!ok) return fail(NSP_ERR_IO, "writing error: " + path);
+  return NSP_OK;
+}
+
+nsp_status nsp_read_restart(nsp_handle* h, const char* dir, double* t, long long* step) {
+  if (!h || !dir) return fail(NSP_ERR_INVALID, "null argument");
+  const std::string d(dir);
+  double tt = 0.0;
+  long long ss = 0;
+  const std::string path = d + "/" + kRestartMeta;
+  if (FILE* f = std::fopen(path.c_str(), "r")) {
+    char key[64];
+    char val[128];
+    int nr = -1, nt = -1, nz = -1;
+    bool format_ok = false;
+    while (std::fscanf(f, "%63s %127s", key, val) == 2) {
+      const std::string k(key);
+      if (k == "format") format_ok = std::string(val) == "nspipe-restart-1";
+      else if (k == "time") tt = std::strtod(val, nullptr);
+      else if (k == "step") ss = std::strtoll(val, nullptr, 10);
+      else if (k == "nr") nr = std::atoi(val);
+      else if (k == "nt") nt = std::atoi(val);
+      else if (k == "nz") nz = std::atoi(val);
+    }
+    std::fclose(f);
+    if (!format_ok) return fail(NSP_ERR_IO, "not a restart metadata file: " + path);
+    if (nr != h->P.nr || nt != h->P.nt || nz != h->P.nz)
+      return fail(NSP_ERR_INVALID, "restart grid " + std::to_string(nr) + "x" + std::to_string(nt) + "x" +
+                                       std::to_string(nz) + " differs from the handle's (convert it with nsp_remesh)");
+  }
+  nsp_status st = nsp_read_state_files(h, (d + "/ur.bin").c_str(), (d + "/ut.bin").c_str(), (d + "/uz.bin").c_str());
+  if (st) return st;
+  if (t) *t = tt;
+  if (step) *step = ss;
+  return NSP_OK;
 }
 
 // ---------------------------------------------------------------- initial conditions

@@ -276,6 +276,15 @@ bool ct_zpass(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T>*
 void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, bool f32, hipStream_t s,
                 const Batch* bt = nullptr);
 
+// Physical space (output.hip): component u (one field) through the padded
+// inverse transform into out rows of `pitch` doubles (mode 0 values, 1
+// squares, 2 add squares), Wt = NR*NTP*NZ complex scratch; cross-section
+// integrals of F (NR*NTP rows) into q[NZF] with q1[NR*NZF] scratch.
+void launch_physical(const Geom& g, const NlBufs& nb, const double2* u, double2* Wt, double* out, size_t pitch,
+                     int mode, hipStream_t s);
+void launch_cross(const Geom& g, const double* F, size_t pitch, const double* rdr, double* q1, double* q,
+                  hipStream_t s);
+
 // Launchers (HIP sources).
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u,
                       double2* N, hipStream_t s, Profiler* pf);

@@ -3,15 +3,20 @@
 // line instead of head.h #defines:
 //
 //   nsp_run --nr 48 --nt 64 --nz 257 --lz 100 --re 1850 --dt 0.01 --steps 1000
-//           [--restart] [--a_p 0.3] [--mixed] [--device 0] [--out_every N]
-//           [--seed S --rms R]
+//           [--restart | --restart_dir DIR] [--save_dir DIR] [--io_dir DIR]
+//           [--dt_frc 0.5 --dt_qcr 0.5 --dt_mnp 0.5] [--a_p 0.3] [--mixed]
+//           [--device 0] [--out_every N] [--seed S --rms R]
 //
-// Without --restart it starts from the reference initial condition (or the
-// seeded one with --seed); with --restart it reads ur_s.bin, ut_s.bin,
-// uz_s.bin (mod_io_nonlinear.cu:426-429).  Every --out_every steps it prints
-// step, time, err and the friction-file values (t, u_cl, u_tau) like
-// io_friction.txt rows (mod_io_nonlinear.cu:474); at the end it writes
-// ur.bin, ut.bin, uz.bin (main.cu:42-44) and the wall-clock time.
+// Without a restart it starts from the reference initial condition (or the
+// seeded one with --seed); --restart reads ur_s.bin, ut_s.bin, uz_s.bin
+// (mod_io_nonlinear.cu:426-429); --restart_dir reads a restart directory
+// (state plus time and step).  With --io_dir it writes io_friction.txt,
+// io_qcross.txt, io_ucross.txt and io_meanpr.txt there, each when the time
+// since its last row reaches its --dt_* interval (mod_integrate.cu:130-151,
+// head.h:33-40).  Every --out_every steps it prints step, err and the
+// friction values (t, u_cl, u_tau).  At the end it writes ur.bin, ut.bin,
+// uz.bin (main.cu:42-44), or a restart directory with --save_dir, and the
+// wall-clock time.
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -31,6 +36,8 @@ int main(int argc, char** argv) {
   int nr = 48, nt = 64, nz = 257, steps = 10, device = 0, out_every = 50, mixed = 0, restart = 0;
   double lz = 100.0, re = 1850.0, dt = 0.01, a_p = 0.3, rms = -1.0;
   unsigned long long seed = 0;
+  double dt_frc = 0.5, dt_qcr = 0.5, dt_mnp = 0.5;
+  std::string restart_dir, save_dir, io_dir;
   for (int a = 1; a < argc; a++) {
     const std::string k = argv[a];
     auto next = [&]() -> const char* {
@@ -54,6 +61,12 @@ int main(int argc, char** argv) {
     else if (k == "--restart") restart = 1;
     else if (k == "--seed") seed = std::strtoull(next(), nullptr, 10);
     else if (k == "--rms") rms = std::atof(next());
+    else if (k == "--restart_dir") restart_dir = next();
+    else if (k == "--save_dir") save_dir = next();
+    else if (k == "--io_dir") io_dir = next();
+    else if (k == "--dt_frc") dt_frc = std::atof(next());
+    else if (k == "--dt_qcr") dt_qcr = std::atof(next());
+    else if (k == "--dt_mnp") dt_mnp = std::atof(next());
     else {
       std::fprintf(stderr, "unknown option %s\n", k.c_str());
       return 2;
@@ -67,7 +80,12 @@ int main(int argc, char** argv) {
   std::printf("nsPipe on MI355X: NR=%d NT=%d NZ=%d LZ=%g Re=%g dt=%g (%s)\n", nr, nt, nz, lz, re, dt,
               mixed ? "mixed" : "fp64");
   if ((st = nsp_create(&p, device, &h))) die("nsp_create", st);
-  if (restart) {
+  double t = 0.0;
+  long long step0 = 0;
+  if (!restart_dir.empty()) {
+    if ((st = nsp_read_restart(h, restart_dir.c_str(), &t, &step0))) die("reading restart directory", st);
+    std::printf("restart at t=%.8e, step %lld\n", t, step0);
+  } else if (restart) {
     if ((st = nsp_read_state_files(h, "ur_s.bin", "ut_s.bin", "uz_s.bin"))) die("reading restart", st);
   } else if (rms >= 0.0) {
     if ((st = nsp_seeded_ic(h, seed, 4, 4, 0, rms))) die("nsp_seeded_ic", st);
@@ -75,12 +93,25 @@ int main(int argc, char** argv) {
     if ((st = nsp_reference_ic(h, a_p))) die("nsp_reference_ic", st);
   }
   if ((st = nsp_autotune(h, 2))) die("nsp_autotune", st);
+  if (!io_dir.empty() && (st = nsp_io_open(h, io_dir.c_str()))) die("nsp_io_open", st);
+  double t_frc = 0.0, t_qcr = 0.0, t_mnp = 0.0;
+  int done = 0;
   const auto t0 = std::chrono::steady_clock::now();
-  double t = 0.0;
   for (int s = 0; s < steps; s++) {
     nsp_step_info info;
     if ((st = nsp_step(h, 1, &info))) die("nsp_step", st);
+    done = s + 1;
     t += dt;
+    t_frc += dt;
+    t_qcr += dt;
+    t_mnp += dt;
+    if (!io_dir.empty()) {
+      int mask = 0;
+      if (t_frc >= dt_frc) { mask |= NSP_IO_FRICTION; t_frc = 0.0; }
+      if (t_qcr >= dt_qcr) { mask |= NSP_IO_CROSS; t_qcr = 0.0; }
+      if (t_mnp >= dt_mnp) { mask |= NSP_IO_MEANPR; t_mnp = 0.0; }
+      if (mask && (st = nsp_io_write(h, t, mask))) die("nsp_io_write", st);
+    }
     if (info.flags & NSP_FLAG_NAN) {
       std::fprintf(stderr, "step %d: convergence measure is NaN\n", s + 1);
       break;
@@ -94,8 +125,13 @@ int main(int argc, char** argv) {
   }
   if ((st = nsp_synchronize(h))) die("nsp_synchronize", st);
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-  if ((st = nsp_write_state_files(h, "ur.bin", "ut.bin", "uz.bin"))) die("writing state", st);
-  std::printf("Total time = %.3fs (%.4f steps/s)\n", secs, steps / secs);
+  if (!save_dir.empty()) {
+    if ((st = nsp_write_restart(h, save_dir.c_str(), t, step0 + done))) die("writing restart directory", st);
+  } else if ((st = nsp_write_state_files(h, "ur.bin", "ut.bin", "uz.bin"))) {
+    die("writing state", st);
+  }
+  if (!io_dir.empty()) nsp_io_close(h);
+  std::printf("Total time = %.3fs (%.4f steps/s)\n", secs, done / secs);
   nsp_destroy(h);
   return 0;
 }

@@ -24,6 +24,7 @@ IC_FLAT, IC_KOLMOGOROV, IC_PUFF, IC_ADD = 0, 1, 2, 16
 HOST, DEVICE = 0, 1
 FLAG_NAN, FLAG_MAXIT = 1, 2
 REMESH_REFERENCE, REMESH_MIRROR = 0, 1
+IO_FRICTION, IO_CROSS, IO_MEANPR = 1, 2, 4
 
 
 class Params(ctypes.Structure):
@@ -45,7 +46,8 @@ EXPORTS = ["nsp_default_params", "nsp_create", "nsp_destroy", "nsp_set_state", "
            "nsp_autotune", "nsp_tuning", "nsp_set_tuning", "nsp_stream", "nsp_synchronize",
            "nsp_last_error", "nsp_ensemble_create", "nsp_ensemble_destroy", "nsp_ensemble_size",
            "nsp_ensemble_member", "nsp_ensemble_autotune", "nsp_ensemble_step", "nsp_remesh",
-           "nsp_remesh_weights"]
+           "nsp_remesh_weights", "nsp_physical_field", "nsp_cross_sections", "nsp_mean_profile",
+           "nsp_io_open", "nsp_io_write", "nsp_io_close", "nsp_write_restart", "nsp_read_restart"]
 
 
 class NspError(RuntimeError):
@@ -98,6 +100,14 @@ def lib():
     L.nsp_ensemble_step.argtypes = [vp, ip, ctypes.POINTER(StepInfo)]
     L.nsp_remesh.argtypes = [vp, vp, ip]
     L.nsp_remesh_weights.argtypes = [ip, ip, ip, P]
+    L.nsp_physical_field.argtypes = [vp, ip, vp, ip]
+    L.nsp_cross_sections.argtypes = [vp, P, P]
+    L.nsp_mean_profile.argtypes = [vp, P]
+    L.nsp_io_open.argtypes = [vp, ctypes.c_char_p]
+    L.nsp_io_write.argtypes = [vp, dp, ip]
+    L.nsp_io_close.argtypes = [vp]
+    L.nsp_write_restart.argtypes = [vp, ctypes.c_char_p, dp, ctypes.c_longlong]
+    L.nsp_read_restart.argtypes = [vp, ctypes.c_char_p, ctypes.POINTER(dp), ctypes.POINTER(ctypes.c_longlong)]
     for name in EXPORTS:
         if name not in ("nsp_last_error", "nsp_stream", "nsp_profile_name", "nsp_tuning", "nsp_ensemble_member"):
             getattr(L, name).restype = ctypes.c_int
@@ -209,6 +219,49 @@ class Pipe:
         _check(lib().nsp_diagnostics(self.h, _dp(out)))
         return dict(u_cl=out[0], u_tau=out[1], ke=out[2], div_l2=out[3], wall_max=out[4],
                     wall_div_max=out[5], flux=out[6])
+
+    @property
+    def physical_shape(self):
+        ntp, nzf = 3 * self.nt // 2, 3 * (self.nz - 1)
+        return (self.nr, ntp, nzf + 2)
+
+    def physical(self, comp):
+        """(NR, NTP, NZF+2) physical samples of component comp (0 ur, 1 ut, 2 uz)."""
+        out = np.zeros(self.physical_shape)
+        _check(lib().nsp_physical_field(self.h, comp, out.ctypes.data, HOST))
+        return out
+
+    def cross_sections(self):
+        """(q, u2): cross-section integrals of ur^2+ut^2 and uz^2 at the NZF axial points."""
+        nzf = 3 * (self.nz - 1)
+        q, u2 = np.zeros(nzf), np.zeros(nzf)
+        _check(lib().nsp_cross_sections(self.h, _dp(q), _dp(u2)))
+        return q, u2
+
+    def mean_profile(self):
+        prof = np.zeros(self.nr)
+        _check(lib().nsp_mean_profile(self.h, _dp(prof)))
+        return prof
+
+    def write_restart(self, directory, t=0.0, step=0):
+        """Raw state files plus restart.txt (time, step, parameters) in directory."""
+        _check(lib().nsp_write_restart(self.h, str(directory).encode(), t, step))
+
+    def read_restart(self, directory):
+        """Reads a restart directory into the state; returns (time, step)."""
+        t = ctypes.c_double()
+        s = ctypes.c_longlong()
+        _check(lib().nsp_read_restart(self.h, str(directory).encode(), ctypes.byref(t), ctypes.byref(s)))
+        return t.value, s.value
+
+    def io_open(self, directory):
+        _check(lib().nsp_io_open(self.h, str(directory).encode()))
+
+    def io_write(self, t, mask=IO_FRICTION | IO_CROSS | IO_MEANPR):
+        _check(lib().nsp_io_write(self.h, t, mask))
+
+    def io_close(self):
+        _check(lib().nsp_io_close(self.h))
 
     def tables(self):
         nr = self.nr

@@ -1174,6 +1174,74 @@ void nspo_step(void* h, int nsteps, int* n_it, double* err) {
   }
 }
 
+// Physical-space component comp (0 ur, 1 ut, 2 uz) of the current state:
+// padForward + fftBackward (mod_io_nonlinear.cu:173-195, mod_fft.cu:43-49)
+// per radial level, as the reference leaves it in u_pad: out[NR][NTP][2*NZP]
+// doubles, NZF samples per row then the two padding doubles (zero here).
+void nspo_physical(void* h, int comp, double* out) {
+  Oracle* o = (Oracle*)h;
+  const int NR = o->NR, NTP = o->NTP, NZP = o->NZP, NZF = o->NZF, MODES = o->MODES;
+#pragma omp parallel
+  {
+    std::vector<cplx> pad((size_t)NTP * NZP), tmp, col, full;
+    std::vector<double> ph((size_t)NTP * NZF);
+#pragma omp for
+    for (int i = 0; i < NR; i++) {
+      o->padForward_level(pad.data(), o->u[comp].data() + (size_t)i * MODES);
+      o->fft2_inverse(pad.data(), ph.data(), tmp, col, full, o->NZ);
+      for (int j = 0; j < NTP; j++) {
+        double* row = out + ((size_t)i * NTP + j) * (2 * NZP);
+        for (int n = 0; n < NZF; n++) row[n] = ph[(size_t)j * NZF + n];
+        row[NZF] = 0.0; row[NZF + 1] = 0.0;
+      }
+    }
+  }
+}
+
+// Cross-section integrals of wrt_qcr/wrt_ucr (mod_io_nonlinear.cu:576-654):
+// q0kernel_k (:55-66) ur^2+ut^2 or u0kernel_k (:68-79) uz^2 per physical
+// point, q1kernel_k (:81-103) the periodic three-point rule
+// Int_j = 0.5/(3 NTP) (f_{j-1} + 4 f_j + f_{j+1}), then intq
+// (mod_cublasflux.cu:119-130): the sum over theta with weights qaux = 1 and the
+// sum over r with rdr.  q[n], u2[n] for the NZF axial points (either NULL).
+void nspo_cross(void* h, double* q, double* u2) {
+  Oracle* o = (Oracle*)h;
+  const int NR = o->NR, NTP = o->NTP, NZP = o->NZP, NZF = o->NZF;
+  const size_t P = (size_t)NR * NTP * 2 * NZP;
+  std::vector<double> a(P), b(P), f(P);
+  for (int which = 0; which < 2; which++) {
+    double* dst = which == 0 ? q : u2;
+    if (!dst) continue;
+    if (which == 0) {
+      nspo_physical(h, 0, a.data());
+      nspo_physical(h, 1, b.data());
+      for (size_t k = 0; k < P; k++) f[k] = a[k] * a[k] + b[k] * b[k];
+    } else {
+      nspo_physical(h, 2, a.data());
+      for (size_t k = 0; k < P; k++) f[k] = a[k] * a[k];
+    }
+    std::vector<double> q1((size_t)NR * 2 * NZP, 0.0);
+    for (int i = 0; i < NR; i++)
+      for (int l = 0; l < 2 * NZP; l++) {
+        double acc = 0.0;
+        for (int it = 0; it < NTP; it++) {
+          int ia = it < 1 ? NTP - 1 : it - 1, ib = it > NTP - 2 ? 0 : it + 1;
+          size_t ih = (size_t)i * NTP * 2 * NZP;
+          double fa = f[ih + (size_t)ia * 2 * NZP + l], fab = f[ih + (size_t)it * 2 * NZP + l],
+                 fb = f[ih + (size_t)ib * 2 * NZP + l];
+          double Int = 0.5 * (1.0 / (3.0 * double(NTP))) * (fa + 4.0 * fab + fb);
+          acc += 1.0 * Int;
+        }
+        q1[(size_t)i * 2 * NZP + l] = acc;
+      }
+    for (int l = 0; l < NZF; l++) {
+      double acc = 0.0;
+      for (int i = 0; i < NR; i++) acc += q1[(size_t)i * 2 * NZP + l] * o->rdr[i];
+      dst[l] = acc;
+    }
+  }
+}
+
 // Diagnostics of the current state: out[0]=u_cl, out[1]=u_tau,
 // out[2]=kinetic energy, out[3]=L2 divergence (weights |rdr|), out[4]=max |u| at the wall,
 // out[5]=max |wall divergence|, out[6]=bulk flux.

@@ -50,6 +50,8 @@ def lib():
         L.nspo_step.argtypes = [vp, ip, IP, P]
         L.nspo_diagnostics.argtypes = [vp, P]
         L.nspo_remesh_weights.argtypes = [ip, ip, ip, P]
+        L.nspo_physical.argtypes = [vp, ip, P]
+        L.nspo_cross.argtypes = [vp, P, P]
         L.nspo_remesh.argtypes = [ip, ip, ip, P, ip, ip, ip, P, ip]
         _lib = L
     return _lib
@@ -186,6 +188,19 @@ class Oracle:
         err = np.zeros(nsteps)
         lib().nspo_step(self.h, nsteps, n_it.ctypes.data_as(ctypes.POINTER(ctypes.c_int)), _p(err))
         return n_it, err
+
+    def physical(self, comp):
+        """(NR, NTP, 2*NZP) physical samples of component comp (0 ur, 1 ut, 2 uz)."""
+        out = np.zeros((self.nr, self.ntp, 2 * self.nzp))
+        lib().nspo_physical(self.h, comp, _p(out))
+        return out
+
+    def cross(self):
+        """Cross-section integrals (q of ur^2+ut^2, u2 of uz^2) at the NZF axial points."""
+        q = np.zeros(self.nzf)
+        u2 = np.zeros(self.nzf)
+        lib().nspo_cross(self.h, _p(q), _p(u2))
+        return q, u2
 
     def diagnostics(self):
         out = np.zeros(7)

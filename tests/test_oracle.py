@@ -520,3 +520,53 @@ def test_remesh_spectral_pad_then_truncate_round_trip():
         np.testing.assert_allclose(back[c], g, rtol=0, atol=1e-14)
         # new modes are zero: m = +-4, +-5 rows and l >= 9 of the padded field
         assert np.all(big[c][:, 4:8, :] == 0) and np.all(big[c][:, :, 9:] == 0)
+
+
+# ------------------------------------------------ physical fields and cross sections
+def brute_physical(f, nt, nz, ntp, nzf):
+    """u(theta_j, z_n) of one radial level by direct summation: the theta sum
+    over every stored m (row it holds m = it for it < NT/2, it - NT otherwise),
+    then the real axial synthesis Re Y(0) + 2 sum_{l>=1} Re(Y(l) e^{i 2 pi l n/NZF})."""
+    th = 2 * np.pi * np.arange(ntp) / ntp
+    m = np.array([it if it < nt // 2 else it - nt for it in range(nt)])
+    Y = np.exp(1j * np.outer(th, m)) @ f  # (ntp, nz)
+    n = np.arange(nzf)
+    E = np.exp(2j * np.pi * np.outer(np.arange(1, nz), n) / nzf)  # (nz-1, nzf)
+    return Y[:, :1].real + 2 * np.real(Y[:, 1:] @ E)
+
+
+def test_physical_field_brute_force():
+    rng = np.random.default_rng(5)
+    o = Oracle(**TINY)
+    shape = (TINY["nr"], TINY["nt"], TINY["nz"])
+    f = [rng.standard_normal(shape) + 1j * rng.standard_normal(shape) for _ in range(3)]
+    o.set_state(*f)
+    for c in range(3):
+        ph = o.physical(c)
+        assert ph.shape == (o.nr, o.ntp, o.nzf + 2)
+        assert np.all(ph[:, :, o.nzf:] == 0)
+        for i in range(o.nr):
+            want = brute_physical(f[c][i], o.nt, o.nz, o.ntp, o.nzf)
+            np.testing.assert_allclose(ph[i, :, :o.nzf], want, rtol=0, atol=1e-12)
+
+
+def test_cross_sections_closed_form():
+    """One mode (m=2, l=1) in ur and ut, one (m=1, l=2) in uz: the theta mean of
+    the square of 2|c| cos(phase) is 2|c|^2 at every z, and the periodic
+    three-point rule sums to the plain mean, so q(z) = sum_i rdr_i 2(|a_i|^2 +
+    |b_i|^2) and u2(z) = sum_i rdr_i 2|c_i|^2."""
+    o = Oracle(**SMALL)
+    rng = np.random.default_rng(3)
+    shape = (o.nr, o.nt, o.nz)
+    ur, ut, uz = (np.zeros(shape, np.complex128) for _ in range(3))
+    a = rng.standard_normal(o.nr) + 1j * rng.standard_normal(o.nr)
+    b = rng.standard_normal(o.nr) + 1j * rng.standard_normal(o.nr)
+    c = rng.standard_normal(o.nr) + 1j * rng.standard_normal(o.nr)
+    ur[:, 2, 1] = a
+    ut[:, 2, 1] = b
+    uz[:, 1, 2] = c
+    o.set_state(ur, ut, uz)
+    rdr = o.tables()["rdr"]
+    q, u2 = o.cross()
+    np.testing.assert_allclose(q, np.sum(rdr * 2 * (np.abs(a) ** 2 + np.abs(b) ** 2)), rtol=1e-13)
+    np.testing.assert_allclose(u2, np.sum(rdr * 2 * np.abs(c) ** 2), rtol=1e-13)

@@ -42,7 +42,7 @@ struct nsp_handle {
   double2* E = nullptr;     // explicit part of the step (+/-)
   double2* SR = nullptr;    // sweep scratch (3 fields)
   double2* SP = nullptr;    // sweep scratch (1 field)
-  double* F = nullptr;      // banded LU factors [4][NR][7][MODES]
+  double* F = nullptr;      // banded LU factors (u+ all modes, u- row NT/2, uz and p rows <= NT/2)
   double* PH = nullptr;     // homogeneous pressure data [NR][2][MODES]
   double* Dc = nullptr;     // centred D table
   double* Luc = nullptr;    // centred Lu table
@@ -333,6 +333,10 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   h->nb.use_ct = false;
   h->nb.mixed = p->precision == NSP_MIXED;
   h->nb.var = NlVariant{ThetaVar{0, 8, 256, 0, 1}, ThetaVar{0, 8, 256, 0, 1}, 1, 256, 0, 0};
+  if (!ct_has_zreg(g.nh)) {  // long axial rows: the 512-thread LDS form
+    h->nb.var.z_alg = 0;
+    h->nb.var.z_nthr = 512;
+  }
   const bool have_ct = ct_has_theta(g.ntp) && ct_has_zhalf(g.nh);
   if (h->nb.mixed && !have_ct)
     return bail(fail(NSP_ERR_INVALID, "the mixed form needs compiled transform plans for 3NT/2 and 3(NZ-1)/2"));
@@ -369,7 +373,9 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
     const char* ng = std::getenv("NSP_NO_GRAPH");
     h->graphs = !(ng && ng[0] && ng[0] != '0');
   }
-  TRY(dalloc(h, &h->Nl, 3 * n));
+  // N(u) and d_r u are FP32 pairs in the mixed form
+  const size_t nf = h->nb.mixed ? (3 * n + 1) / 2 : 3 * n;
+  TRY(dalloc(h, &h->Nl, nf));
   TRY(dalloc(h, &h->E, 3 * n));
   TRY(dalloc(h, &h->SR, 3 * n));
   TRY(dalloc(h, &h->SP, n));
@@ -377,7 +383,7 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
     h->F = share->F;
     h->PH = share->PH;
   } else {
-    TRY(dalloc(h, &h->F, 4 * 7 * n));
+    TRY(dalloc(h, &h->F, factor_count(g)));
     TRY(dalloc(h, &h->PH, 2 * n));
   }
   TRY(upload(h, &h->Dc, centred_table(g.nr, h->host.D)));
@@ -386,14 +392,7 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
     h->b.invc = share->b.invc;
   else
     TRY(dalloc(h, &h->b.invc, 4 * size_t(g.modes)));
-  TRY(dalloc(h, &h->nb.dR, 3 * n));
-  if (h->nb.mixed) {
-    TRY(dalloc(h, &h->nb.W32, 9 * h->nb.wstride));
-    TRY(dalloc(h, &h->nb.Y32, 3 * h->nb.wstride));
-  } else {
-    TRY(dalloc(h, &h->nb.W, 9 * h->nb.wstride));
-    TRY(dalloc(h, &h->nb.Y, 3 * h->nb.wstride));
-  }
+  TRY(dalloc(h, &h->nb.dR, nf));
   TRY(dalloc(h, &h->norm, 2));
   TRY(dalloc(h, &h->part, 4 * size_t(g.modes)));
   HTRY(hipHostMalloc((void**)&h->norm_h, 2 * sizeof(unsigned long long), hipHostMallocDefault));
@@ -425,6 +424,14 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
     h->b.uh = h->b.uhP = nullptr;
     if (e5 != hipSuccess || e6 != hipSuccess)
       return bail(fail(NSP_ERR_HIP, std::string("setup: ") + hipGetErrorString(e5 != hipSuccess ? e5 : e6)));
+  }
+  // transform intermediates after the setup buffers are freed (lower peak)
+  if (h->nb.mixed) {
+    TRY(dalloc(h, &h->nb.W32, 9 * h->nb.wstride));
+    TRY(dalloc(h, &h->nb.Y32, 3 * h->nb.wstride));
+  } else {
+    TRY(dalloc(h, &h->nb.W, 9 * h->nb.wstride));
+    TRY(dalloc(h, &h->nb.Y, 3 * h->nb.wstride));
   }
 #undef TRY
 #undef HTRY

@@ -58,9 +58,18 @@ __device__ __forceinline__ void rt_of(double2& a, double2& b) {  // decoupleBack
 }
 
 // factor slot c of operator `kind` at row i: c 0..2 = L multipliers of
-// columns i-3..i-1, 3 = 1/U_ii, 4..6 = U_{i,i+1..i+3}
-__device__ __forceinline__ size_t fidx(const Geom& g, int kind, int i, int c, int k) {
-  return ((size_t)(kind * g.nr + i) * 7 + c) * g.modes + k;
+// columns i-3..i-1, 3 = 1/U_ii, 4..6 = U_{i,i+1..i+3}.  Only the factors the
+// sweep reads are stored, mode-fastest: u+ for every mode; u- for the
+// azimuthal row NT/2 only (elsewhere u- at m is u+ at -m); uz and p for the
+// rows it <= NT/2 (they depend on |m|, read at k_abs).
+__host__ __device__ __forceinline__ size_t fidx(const Geom& g, int kind, int i, int c, int k) {
+  const size_t row = (size_t)i * 7 + c, r7 = (size_t)g.nr * 7;
+  const size_t mh = (size_t)(g.nt / 2 + 1) * g.nz;
+  if (kind == 0) return row * g.modes + k;
+  const size_t b1 = r7 * g.modes;
+  if (kind == 1) return b1 + row * g.nz + (size_t)(k - (g.nt / 2) * g.nz);
+  const size_t b2 = b1 + r7 * g.nz;
+  return b2 + (size_t)(kind - 2) * r7 * mh + row * mh + k;
 }
 
 // The operator rows of one mode (LUsolve_*_k): kind 0 u+, 1 u-, 2 uz
@@ -177,6 +186,8 @@ __global__ __launch_bounds__(256) void factor_kernel(Geom g, const double* __res
   const int kind = blockIdx.y;
   if (k >= g.modes) return;
   const ModeQ q = modeq(g, k);
+  if (kind == 1 && q.it != g.nt / 2) return;  // stored factor sets only (fidx)
+  if (kind >= 2 && q.it > g.nt / 2) return;
   const OpMode o = op_mode(g, Lu, Lp, kind, q.m, q.l);
   double Uw[IW][IW + 1];
 #pragma unroll
@@ -228,7 +239,9 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
   const Geom& g = A.g;
   if (step_done(g)) return;
   const int nch = (g.nz + 63) / 64;
-  const int pair = blockIdx.x / nch, ch = blockIdx.x - pair * nch;
+  // XCD-contiguous: the chunks of one azimuthal row, adjacent in memory, run in one L2
+  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  const int pair = lb / nch, ch = lb - pair * nch;
   if (pair >= g.nt / 2) return;  // beyond this pipe's extent
   const int nth = g.nt / 2;
   const int it = (threadIdx.x < 64) ? pair : (pair == 0 ? nth : g.nt - pair);
@@ -629,6 +642,8 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     }
   }
 }
+
+size_t factor_count(const Geom& g) { return fidx(g, 3, g.nr - 1, 6, (g.nt / 2 + 1) * g.nz - 1) + 1; }
 
 void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, hipStream_t s) {
   dim3 grid((g.modes + 255) / 256, 4);

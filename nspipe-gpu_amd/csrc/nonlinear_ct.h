@@ -19,16 +19,6 @@ __device__ __forceinline__ int opaque(int x) {
 }
 
 // ------------------------------------------------------------------ block order
-// Workgroups are dealt to the 8 XCDs round robin; remap so that each XCD
-// walks one contiguous range of logical blocks and neighbouring tiles and
-// radial levels meet in the same L2.
-__device__ __forceinline__ int xcd_block(int b, int G) {
-  constexpr int NX = 8;
-  const int q = G / NX, rmd = G % NX;
-  const int x = b % NX, j = b / NX;
-  return (x < rmd ? x * (q + 1) : rmd * (q + 1) + (x - rmd) * q) + j;
-}
-
 // (level i, first column l0) of a theta block.  ord 0: level fastest;
 // ord 1: column tile fastest, so the blocks an XCD runs together cover whole
 // rows of one level (contiguous HBM pages, write lines filled in one L2).
@@ -152,6 +142,13 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
   const int tid = threadIdx.x;
   const int nz = g.nz;
   const T rad = T(r[i]);
+  // x / r; the FP32 form multiplies by the reciprocal (within an ulp, without
+  // the IEEE division sequence per sample)
+  const T irad = T(1) / rad;
+  auto over_r = [&](T x) -> T {
+    if constexpr (sizeof(T) == 4) return x * irad;
+    else return x / rad;
+  };
   const size_t roff = (size_t)row * nz;
   const T kz0 = T(g.kz0);
 
@@ -222,8 +219,8 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
       const int n = lt + q * NTHR;
       if (n < NZF) {
         T v = acc[q];
-        if (c == 0) v = v - ut[q] * (ut[q] / rad);
-        else if (c == 1) v = v + ur[q] * (ut[q] / rad);
+        if (c == 0) v = v - ut[q] * over_r(ut[q]);
+        else if (c == 1) v = v + ur[q] * over_r(ut[q]);
         T* slot = reinterpret_cast<T*>(&buf[LY::pos(PL::digitrev(n >> 1))]) + (n & 1);
         *slot = -v;
       }
@@ -288,6 +285,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   const bool own = tid < NB1;
   const int nz = g.nz;
   const T rad = T(r[i]);
+  // x / r; the FP32 form multiplies by the reciprocal (within an ulp, without
+  // the IEEE division sequence per sample)
+  const T irad = T(1) / rad;
+  auto over_r = [&](T x) -> T {
+    if constexpr (sizeof(T) == 4) return x * irad;
+    else return x / rad;
+  };
   const size_t roff = (size_t)row * nz;
   const T kz0 = T(g.kz0);
   const cpx<T> zero{T(0), T(0)};
@@ -380,11 +384,11 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
           for (int q = 0; q < R0; q++) {
             T vx = acc[q].x + uz[q].x * sm[q].x, vy = acc[q].y + uz[q].y * sm[q].y;
             if (c == 0) {
-              vx = vx - ut[q].x * (ut[q].x / rad);
-              vy = vy - ut[q].y * (ut[q].y / rad);
+              vx = vx - ut[q].x * over_r(ut[q].x);
+              vy = vy - ut[q].y * over_r(ut[q].y);
             } else if (c == 1) {
-              vx = vx + ur[q].x * (ut[q].x / rad);
-              vy = vy + ur[q].y * (ut[q].y / rad);
+              vx = vx + ur[q].x * over_r(ut[q].x);
+              vy = vy + ur[q].y * over_r(ut[q].y);
             }
             v[q] = cpx<T>{-vx, -vy};
           }
@@ -643,16 +647,23 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
   X(144, Plan<9, 16>)      \
   X(192, Plan<12, 16>)     \
   X(288, Plan<2, 9, 16>)   \
-  X(576, Plan<4, 9, 16>)
+  X(576, Plan<4, 9, 16>)   \
+  X(1152, Plan<8, 9, 16>)
 
-#define NSP_ZHALF_PLANS(X)  \
-  X(96, Plan<6, 16>)        \
-  X(576, Plan<4, 9, 16>)    \
-  X(768, Plan<3, 16, 16>)   \
-  X(1536, Plan<6, 16, 16>)  \
-  X(2304, Plan<9, 16, 16>)
+#define NSP_ZHALF_PLANS(X)    \
+  X(96, Plan<6, 16>)          \
+  X(576, Plan<4, 9, 16>)      \
+  X(768, Plan<3, 16, 16>)     \
+  X(1536, Plan<6, 16, 16>)    \
+  X(2304, Plan<9, 16, 16>)    \
+  X(4608, Plan<2, 9, 16, 16>)
 
 // ------------------------------------------------------------------ stage launchers
+// A form is compiled for a plan only if its LDS buffer fits the 160 KB of a CU.
+template <typename T, typename LY>
+constexpr bool lds_fits(int n) {
+  return LY::size(n) * (int)sizeof(cpx<T>) <= 163840;
+}
 // Variant dispatch for one compiled plan PL.  The LDS pad shift is 4 (one
 // slot per 16 elements) for tile width 4 and 3 (one per 8) for width 8.
 // Row padding of the theta scatter forms: rows of (NTP / R_last) positions x
@@ -670,23 +681,23 @@ bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const do
   const NlMember* tab = bt ? bt->tab : nullptr;
   const dim3 nb = bt ? dim3(((bt->max_nz + tl - 1) / tl) * bt->max_nr, bt->count) : dim3(((g.nz + tl - 1) / tl) * g.nr);
   if (v.alg == 1) {
-    if (tl == 4 && v.nthr == 192) theta_inv2_ct<T, 4, 192, Dense, PL, BAT><<<nb, 192, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
-    else if (tl == 8 && v.nthr == 256) theta_inv2_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
-    else if (tl == 8 && v.nthr == 384) theta_inv2_ct<T, 8, 384, Dense, PL, BAT><<<nb, 384, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
-    else if (tl == 16 && v.nthr == 256) theta_inv2_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+    if (tl == 4 && v.nthr == 192) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_inv2_ct<T, 4, 192, Dense, PL, BAT><<<nb, 192, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+    else if (tl == 8 && v.nthr == 256) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_inv2_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+    else if (tl == 8 && v.nthr == 384) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_inv2_ct<T, 8, 384, Dense, PL, BAT><<<nb, 384, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+    else if (tl == 16 && v.nthr == 256) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_inv2_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
     else return false;
     return true;
   }
   if (v.alg != 0) return false;
   using R4 = ThetaRowPad<PL, 4>;
   using R8 = ThetaRowPad<PL, 8>;
-  if (tl == 4 && v.pad == 0) theta_inv_ct<T, 4, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
-  else if (tl == 4 && v.pad == 1) theta_inv_ct<T, 4, 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
-  else if (tl == 4 && v.pad == 2) theta_inv_ct<T, 4, 256, R4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
-  else if (tl == 8 && v.pad == 0) theta_inv_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
-  else if (tl == 8 && v.pad == 1) theta_inv_ct<T, 8, 256, Lay<8, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
-  else if (tl == 8 && v.pad == 2) theta_inv_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
-  else if (tl == 16 && v.pad == 0) theta_inv_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab);
+  if (tl == 4 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_inv_ct<T, 4, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 4 && v.pad == 1) { if constexpr (lds_fits<T, Lay<16, 1>>(PL::L * 4)) theta_inv_ct<T, 4, 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 4 && v.pad == 2) { if constexpr (lds_fits<T, R4>(PL::L * 4)) theta_inv_ct<T, 4, 256, R4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 8 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_inv_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 8 && v.pad == 1) { if constexpr (lds_fits<T, Lay<8, 1>>(PL::L * 8)) theta_inv_ct<T, 8, 256, Lay<8, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 8 && v.pad == 2) { if constexpr (lds_fits<T, R8>(PL::L * 8)) theta_inv_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 16 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_inv_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else return false;
   return true;
 }
@@ -699,23 +710,23 @@ bool theta_fwd_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const cp
   const NlMember* tab = bt ? bt->tab : nullptr;
   const dim3 nb = bt ? dim3(((bt->max_nz + tl - 1) / tl) * bt->max_nr, bt->count) : dim3(((g.nz + tl - 1) / tl) * g.nr);
   if (v.alg == 1) {
-    if (tl == 4 && v.nthr == 192) theta_fwd2_ct<T, 4, 192, Dense, PL, BAT><<<nb, 192, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
-    else if (tl == 8 && v.nthr == 256) theta_fwd2_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
-    else if (tl == 8 && v.nthr == 384) theta_fwd2_ct<T, 8, 384, Dense, PL, BAT><<<nb, 384, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
-    else if (tl == 16 && v.nthr == 256) theta_fwd2_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+    if (tl == 4 && v.nthr == 192) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_fwd2_ct<T, 4, 192, Dense, PL, BAT><<<nb, 192, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+    else if (tl == 8 && v.nthr == 256) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_fwd2_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+    else if (tl == 8 && v.nthr == 384) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_fwd2_ct<T, 8, 384, Dense, PL, BAT><<<nb, 384, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+    else if (tl == 16 && v.nthr == 256) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_fwd2_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
     else return false;
     return true;
   }
   if (v.alg != 0) return false;
   using R4 = ThetaRowPad<PL, 4>;
   using R8 = ThetaRowPad<PL, 8>;
-  if (tl == 4 && v.pad == 0) theta_fwd_ct<T, 4, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
-  else if (tl == 4 && v.pad == 1) theta_fwd_ct<T, 4, 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
-  else if (tl == 4 && v.pad == 2) theta_fwd_ct<T, 4, 256, R4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
-  else if (tl == 8 && v.pad == 0) theta_fwd_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
-  else if (tl == 8 && v.pad == 1) theta_fwd_ct<T, 8, 256, Lay<8, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
-  else if (tl == 8 && v.pad == 2) theta_fwd_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
-  else if (tl == 16 && v.pad == 0) theta_fwd_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab);
+  if (tl == 4 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_fwd_ct<T, 4, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 4 && v.pad == 1) { if constexpr (lds_fits<T, Lay<16, 1>>(PL::L * 4)) theta_fwd_ct<T, 4, 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 4 && v.pad == 2) { if constexpr (lds_fits<T, R4>(PL::L * 4)) theta_fwd_ct<T, 4, 256, R4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 8 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_fwd_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 8 && v.pad == 1) { if constexpr (lds_fits<T, Lay<8, 1>>(PL::L * 8)) theta_fwd_ct<T, 8, 256, Lay<8, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 8 && v.pad == 2) { if constexpr (lds_fits<T, R8>(PL::L * 8)) theta_fwd_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 16 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_fwd_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else return false;
   return true;
 }
@@ -733,29 +744,39 @@ bool zpass_plan(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T
   constexpr int LL = PL::L;
   const NlMember* tab = bt ? bt->tab : nullptr;
   const dim3 nb = bt ? dim3(bt->max_nr * g.ntp, bt->count) : dim3(g.nr * g.ntp);
-  if (v.z_alg == 1) {  // register form, 256 threads
-    constexpr int PAD = zpass2_pad<PL>();
-    if (v.z_pf)
-      zpass2_ct<T, 256, 1, PAD, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
-    else
-      zpass2_ct<T, 256, 0, PAD, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
-    return true;
+  if (v.z_alg == 1) {  // register form, 256 threads: one first-pass butterfly per thread
+    if constexpr (LL / PL::rad[0] <= 256) {
+      constexpr int PAD = zpass2_pad<PL>();
+      if (v.z_pf)
+        zpass2_ct<T, 256, 1, PAD, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+      else
+        zpass2_ct<T, 256, 0, PAD, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+      return true;
+    } else {
+      return false;
+    }
   }
   if (v.z_alg != 0) return false;
   using RP = Lay<LL / PL::rad[PL::P - 1], zpass2_pad<PL>()>;
+  constexpr bool t256 = (2 * LL + 255) / 256 <= 24;  // samples per thread of the 256-thread form
+  if (v.z_nthr == 256 && !t256) return false;
   if (v.z_pad == 2) {
-    if (v.z_nthr == 256)
-      zpass_ct<T, 256, (2 * LL + 255) / 256, RP, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
-    else if (v.z_nthr == 512)
+    if (v.z_nthr == 256) {
+      if constexpr (t256)
+        zpass_ct<T, 256, (2 * LL + 255) / 256, RP, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+    } else if (v.z_nthr == 512)
       zpass_ct<T, 512, (2 * LL + 511) / 512, RP, PL, BAT><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
     else
       return false;
     return true;
   }
-  if (v.z_nthr == 256 && !v.z_pad)
-    zpass_ct<T, 256, (2 * LL + 255) / 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
-  else if (v.z_nthr == 256)
-    zpass_ct<T, 256, (2 * LL + 255) / 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+  if (v.z_nthr == 256 && !v.z_pad) {
+    if constexpr (t256)
+      zpass_ct<T, 256, (2 * LL + 255) / 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+  } else if (v.z_nthr == 256) {
+    if constexpr (t256)
+      zpass_ct<T, 256, (2 * LL + 255) / 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+  }
   else if (v.z_nthr == 512 && !v.z_pad)
     zpass_ct<T, 512, (2 * LL + 511) / 512, Dense, PL, BAT><<<nb, 512, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
   else if (v.z_nthr == 512)

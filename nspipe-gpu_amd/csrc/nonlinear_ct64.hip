@@ -18,6 +18,13 @@ bool ct_has_theta(int L) {
 #undef X
   return false;
 }
+// the register form of the z pass (one first-pass butterfly per thread of 256)
+bool ct_has_zreg(int L) {
+#define X(LL, ...) if (L == LL) return LL / __VA_ARGS__::rad[0] <= 256;
+  NSP_ZHALF_PLANS(X)
+#undef X
+  return false;
+}
 bool ct_has_zhalf(int L) {
 #define X(LL, ...) if (L == LL) return true;
   NSP_ZHALF_PLANS(X)

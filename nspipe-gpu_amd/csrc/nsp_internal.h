@@ -54,6 +54,16 @@ struct Geom {
 
 __device__ __forceinline__ bool step_done(const Geom& g) { return g.skip != nullptr && *g.skip != 0; }
 
+// Workgroups are dealt to the 8 XCDs round robin; remap so that each XCD
+// walks one contiguous range of logical blocks and neighbouring tiles and
+// radial levels meet in the same L2.
+__device__ __forceinline__ int xcd_block(int b, int G) {
+  constexpr int NX = 8;
+  const int q = G / NX, rmd = G % NX;
+  const int x = b % NX, j = b / NX;
+  return (x < rmd ? x * (q + 1) : rmd * (q + 1) + (x - rmd) * q) + j;
+}
+
 // A pointer read from a device table of launch arguments is a generic pointer
 // to the compiler, and every access through it becomes a flat memory
 // instruction.  Re-deriving it from the table pointer (a kernel argument, which
@@ -127,7 +137,7 @@ struct Lin2Args {
   const double* Luc;    // [2][NR][7] same for Lu
   const double* Lu;     // [2][NR][7] operator tables (factor recurrence on the fly)
   const double* Lp;     // [2][NR][7]
-  const double* F;      // [4][NR][7][MODES] banded LU factors
+  const double* F;      // banded LU factors (layout: fidx in linear2.hip)
   const double* PH;     // [NR][2][MODES] dr p_h, p_h of the homogeneous pressure solution
   const double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
   const double2* N;     // [3] nonlinear term (r/theta); float2 storage when n32
@@ -141,6 +151,7 @@ struct Lin2Args {
   int predictor;
 };
 void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, hipStream_t s);
+size_t factor_count(const Geom& g);  // doubles of the stored LU factors
 void launch_sweep2(const Lin2Args& a, hipStream_t s);
 // one launch per pass for `count` pipes (tab in device memory); max_nt/max_nz
 // are the largest extents among them
@@ -258,6 +269,7 @@ struct Profiler {
 // Compiled-plan support (nonlinear_ct64.hip).
 bool ct_has_theta(int L);
 bool ct_has_zhalf(int L);
+bool ct_has_zreg(int L);
 std::vector<int> ct_radices(int L, bool theta);
 // per-pass twiddles of a plan: for each pass (radix R, sub-length S) S*(R-1)
 // entries exp(-2 pi i q k / (S R)) at [k*(R-1) + q-1]

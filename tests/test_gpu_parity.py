@@ -219,6 +219,67 @@ def test_every_kernel_variant_keeps_parity(shape):
     p.close()
 
 
+# the longest compiled plans: theta 1152 (NT = 768) and half-z 4608 (NZ = 3073)
+LONG = [dict(nr=8, nt=768, nz=65, lz=4 * np.pi, re=19000.0, dt=0.0005),
+        dict(nr=8, nt=64, nz=3073, lz=4 * np.pi, re=19000.0, dt=0.0005)]
+
+
+@pytest.mark.parametrize("shape", LONG, ids=lambda s: f"{s['nr']}x{s['nt']}x{s['nz']}")
+def test_long_plans_every_compiled_form(shape):
+    """Every form compiled for the long plans (forms whose LDS buffer exceeds a
+    CU, or the register z form for 4608, are not compiled and are refused)
+    keeps FP64 parity; the mixed form stays at FP32 accuracy."""
+    print(f"[gpu] long plans {shape}", flush=True)
+    p = nsp.Pipe(**shape)
+    o = Oracle(**shape)
+    p.seeded_ic(SEED1, 6, 6, 1, 0.1)
+    u = p.get_state()
+    b = o.nonlinear(*u)
+    tested = 0
+    for v in _variants():
+        try:
+            p.set_tuning(v)
+        except nsp.NspError:
+            if v["z_alg"] != 1:
+                continue
+            try:  # the same theta form with the z pass in its 512-thread LDS form
+                p.set_tuning(dict(v, z_alg=0, z_nthr=512, z_pad=0))
+            except nsp.NspError:
+                continue
+        a = p.nonlinear(*u)
+        for c in range(3):
+            e = rel_l2(a[c], b[c])
+            assert e <= 1e-12, (v, c, e)
+        tested += 1
+    assert tested >= 10
+    p.close()
+    q = nsp.Pipe(**shape, precision=nsp.MIXED)
+    a = q.nonlinear(*u)
+    errs = [rel_l2(x, y) for x, y in zip(a, b)]
+    print("[gpu] long plans mixed rel L2", errs, flush=True)
+    assert max(errs) < 1e-5
+    q.close()
+
+
+def test_config5_mixed_steps():
+    """The largest configuration, NR=160, NT=768, NZ=3073 (377M points per
+    component), in the mixed form on one GPU: a few steps from seeded
+    k^-5/3 noise converge without flags, hold the bulk flux and keep the
+    divergence at the level of the initial field."""
+    shape = dict(nr=160, nt=768, nz=3073, lz=4 * np.pi, re=19000.0, dt=0.0005)
+    print(f"[gpu] config 5 mixed {shape}", flush=True)
+    p = nsp.Pipe(**shape, precision=nsp.MIXED)
+    p.seeded_ic(20261202 + 5 * 1000, 4, 4, nsp.IC_KOLMOGOROV, 0.075)
+    d0 = p.diagnostics()
+    n_it, err, flags = p.step(3)
+    d = p.diagnostics()
+    print(f"[gpu] config 5 n_it {n_it} err {err} ke {d['ke']:.10e} div {d['div_l2']:.3e}", flush=True)
+    assert np.all(flags == 0) and np.all(n_it <= 5)
+    assert abs(d["flux"] - 0.5) < 1e-12 and np.isfinite(d["ke"]) and abs(d["ke"] / d0["ke"] - 1) < 1e-2
+    assert d["wall_max"] < 1e-12
+    p.close()
+
+
 def test_set_tuning_rejects_missing_variant():
     p = nsp.Pipe(**COMPILED[0])
     bad = dict(theta_inv=dict(alg=1, tl=32, nthr=256, pad=0, ord=0),
@@ -231,12 +292,13 @@ def test_set_tuning_rejects_missing_variant():
     p.close()
 
 
-def test_graph_and_eager_steps_agree(monkeypatch):
+@pytest.mark.parametrize("precision", [nsp.FP64, nsp.MIXED])
+def test_graph_and_eager_steps_agree(monkeypatch, precision):
     """The captured step (default) and the launch-by-launch step with a host
     convergence test give identical bits and iteration counts."""
-    p1 = nsp.Pipe(**CFG1)
+    p1 = nsp.Pipe(**CFG1, precision=precision)
     monkeypatch.setenv("NSP_NO_GRAPH", "1")
-    p2 = nsp.Pipe(**CFG1)
+    p2 = nsp.Pipe(**CFG1, precision=precision)
     for p in (p1, p2):
         p.seeded_ic(SEED1, 4, 4, 0, 1e-3)
     n1, e1, f1 = p1.step(5)
@@ -277,6 +339,22 @@ def test_mixed_nonlinear_accuracy():
     errs = [rel_l2(x, y) for x, y in zip(a, b)]
     print("[gpu] mixed nonlinear rel L2", errs, flush=True)
     assert max(errs) < 1e-5
+    p.close()
+
+
+def test_mixed_one_step_state_error():
+    """Mixed form (FP32 transforms, FP32 column storage and factors in the
+    solves, FP64 arithmetic and state) against the FP64 oracle after one
+    step: every field within 1e-6 relative L2, same corrector count."""
+    p = nsp.Pipe(**CFG1, precision=nsp.MIXED)
+    o = Oracle(**CFG1)
+    p.seeded_ic(SEED1, 4, 4, 0, 0.05)
+    o.set_state(*p.get_state())
+    n_p, _, _ = p.step(1)
+    n_o, _ = o.step(1)
+    errs = [rel_l2(a, b) for a, b in zip(p.get_state(), o.get_state())]
+    print("[gpu] mixed one-step state rel L2", errs, flush=True)
+    assert max(errs) <= 1e-6 and n_p[0] == n_o[0]
     p.close()
 
 

@@ -2,7 +2,8 @@
 
 Kernel names are grouped into the stages the library's own event profiler
 reports (nsp_profile_name): dr3_kernel, theta_inv_kernel, zpass_kernel,
-theta_fwd_kernel, linear_sweep_kernel(predictor|corrector).
+theta_fwd_kernel, linear_sweep_kernel(predictor|corrector); byte figures are
+per stage launch (a sweep launch is its four pass kernels).
 
 HBM traffic comes from two counter runs of the same command (FETCH_SIZE, then
 WRITE_SIZE; rocprofv3 reports both in KiB per dispatch).  Each counter is
@@ -12,8 +13,8 @@ any work: bytes = counter_KiB * 1024 / (counter_KiB_of_copy * 1024 / copy_bytes)
 
     python -m nsp.rocprof traffic --fetch F.csv --write W.csv --copy-bytes B --form mixed \
         --out profiles/hbm_traffic.json
-    python -m nsp.rocprof stats --stats S_kernel_stats.csv --title "..." --command "..." \
-        --out profiles/x.txt
+    python -m nsp.rocprof stats --stats S_kernel_stats.csv --trace S_kernel_trace.csv \
+        --title "..." --command "..." --out profiles/x.txt
 """
 import argparse
 import csv
@@ -36,10 +37,11 @@ def stage_of(name):
         return "zpass_kernel"
     if "theta_fwd" in n:
         return "theta_fwd_kernel"
-    m = re.search(r"sweep_kernel<(\d)>", n)
+    m = re.search(r"sweep_kernel<(\d), (\d)", n)  # <pass, predictor, ...>
     if m:
-        return "linear_sweep_kernel(predictor)" if m.group(1) == "1" else "linear_sweep_kernel(corrector)"
+        return "linear_sweep_kernel(predictor)" if m.group(2) == "1" else "linear_sweep_kernel(corrector)"
     return None
+
 
 
 def read_counters(path, counter):
@@ -55,17 +57,37 @@ def read_counters(path, counter):
 
 def calibrate(rows, copy_bytes):
     """KiB the counter reads per byte of the calibration copy: the copy kernels
-    dispatched before the first library kernel."""
-    total, n = 0.0, 0
+    of the calibration size dispatched before the first library kernel (the
+    small copies of table uploads are left out: under a quarter of the largest)."""
+    vals = []
     for _, name, v in rows:
-        if name.startswith("nsp::") or "nsp::" in name:
+        if "nsp::" in name:
             break
         if COPY_KERNEL in name:
-            total += v
-            n += 1
-    if n == 0:
+            vals.append(v)
+    big = [v for v in vals if vals and v >= 0.25 * max(vals)]
+    if not big:
         raise SystemExit("no calibration copy dispatches found before the first library kernel")
-    return total * 1024.0, n
+    return sum(big) * 1024.0, len(big)
+
+
+def stage_bytes(rows, fac):
+    """Bytes per executed launch of each stage.  Launches that return at entry
+    (correctors of a captured step after convergence) are left out: per kernel
+    name, dispatches under 5% of its largest.  A stage launch is one dispatch of
+    each of its kernels, so the per-name averages add up."""
+    by_name = {}
+    for _, name, v in rows:
+        if stage_of(name) is not None:
+            by_name.setdefault(name, []).append(v * 1024.0 / fac)
+    per, count = {}, {}
+    for name, vs in by_name.items():
+        top = max(vs)
+        kept = [v for v in vs if v >= 0.05 * top] if top > 0 else vs
+        st = stage_of(name)
+        per[st] = per.get(st, 0.0) + sum(kept) / len(kept)
+        count[st] = count.get(st, 0) + len(kept)
+    return per, count
 
 
 def traffic(args):
@@ -82,17 +104,10 @@ def traffic(args):
             out = json.load(f)
     per = {}
     for rows, key, fac in ((fetch, "fetch_bytes", ffac), (write, "write_bytes", wfac)):
-        acc = {}
-        for _, name, v in rows:
-            st = stage_of(name)
-            if st is None:
-                continue
-            a = acc.setdefault(st, [0.0, 0])
-            a[0] += v * 1024.0 / fac
-            a[1] += 1
-        for st, (tot, cnt) in acc.items():
-            per.setdefault(st, {})[key] = tot / cnt
-            per[st]["dispatches"] = cnt
+        b, n = stage_bytes(rows, fac)
+        for st in b:
+            per.setdefault(st, {})[key] = b[st]
+            per[st]["dispatches"] = n[st]
     for st, d in per.items():
         d["traffic"] = d.get("fetch_bytes", 0.0) + d.get("write_bytes", 0.0)
     out[args.form] = dict(calibration=dict(copy_bytes=args.copy_bytes, fetch_counter_per_byte=ffac,
@@ -104,13 +119,26 @@ def traffic(args):
 
 
 def stats(args):
+    """kernel_stats.csv as text, plus, from the kernel trace, the launches that
+    did the work: correctors captured in a step graph after convergence, and
+    their nonlinear evaluations, return at entry and are left out (launches
+    under 5% of the kernel's longest), with the per-stage sums per launch."""
     rows = []
     with open(args.stats) as f:
         for r in csv.DictReader(f):
             rows.append(r)
+    durs = {}
+    if args.trace:
+        with open(args.trace) as f:
+            for r in csv.DictReader(f):
+                d = (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3
+                durs.setdefault(r["Kernel_Name"], []).append(d)
     lines = [f"# {args.title}", f"# command: {args.command}",
-             "# source: rocprofv3 --kernel-trace --stats, kernel_stats.csv (durations in microseconds)",
-             f"{'kernel':70s} {'stage':32s} {'calls':>6s} {'total_us':>12s} {'avg_us':>10s} {'pct':>6s}"]
+             "# source: rocprofv3 --kernel-trace --stats (kernel_stats.csv, kernel_trace.csv); microseconds",
+             "# exec / exec_avg_us: launches that did the work (>= 5% of the kernel's longest) and their mean",
+             f"{'kernel':70s} {'stage':32s} {'calls':>6s} {'total_us':>12s} {'avg_us':>10s} {'pct':>6s} "
+             f"{'exec':>5s} {'exec_avg_us':>11s}"]
+    stage_sum = {}
     for r in rows:
         name = r["Name"]
         st = stage_of(name) or "-"
@@ -119,7 +147,18 @@ def stats(args):
         tot = float(r["TotalDurationNs"]) / 1e3
         avg = float(r["AverageNs"]) / 1e3
         pct = float(r["Percentage"])
-        lines.append(f"{short[:70]:70s} {st:32s} {calls:6d} {tot:12.1f} {avg:10.1f} {pct:6.2f}")
+        ds = durs.get(name, [])
+        kept = [d for d in ds if ds and d >= 0.05 * max(ds)]
+        ex = len(kept)
+        ea = sum(kept) / ex if ex else 0.0
+        if st != "-" and ex:
+            stage_sum.setdefault(st, []).append((short, ea))
+        lines.append(f"{short[:70]:70s} {st:32s} {calls:6d} {tot:12.1f} {avg:10.1f} {pct:6.2f} {ex:5d} {ea:11.1f}")
+    if stage_sum:
+        lines.append("# per stage launch (sum of its kernels' exec_avg_us; several forms of one stage")
+        lines.append("# appear when the run also timed other variants):")
+        for st, ks in sorted(stage_sum.items()):
+            lines.append(f"#   {st:32s} {sum(a for _, a in ks):10.1f} us  ({', '.join(k for k, _ in ks)})")
     text = "\n".join(lines) + "\n"
     with open(args.out, "w") as f:
         f.write(text)
@@ -139,6 +178,7 @@ def main(argv=None):
     t.add_argument("--out", required=True)
     s = sub.add_parser("stats")
     s.add_argument("--stats", required=True)
+    s.add_argument("--trace", default=None, help="kernel_trace.csv of the same run")
     s.add_argument("--title", required=True)
     s.add_argument("--command", required=True)
     s.add_argument("--out", required=True)

@@ -180,6 +180,7 @@ Lin2Args lin_args(nsp_handle* h, const Geom& g, const double2* N, const double2*
   a.SP = h->SP;
   a.norm = h->norm;
   a.predictor = predictor ? 1 : 0;
+  a.ord = h->nb.var.s_ord;
   return a;
 }
 
@@ -206,9 +207,9 @@ std::string variant_json(const NlVariant& v) {
                   t.nthr, t.pad, t.ord);
     return std::string(o);
   };
-  char z[112];
-  std::snprintf(z, sizeof z, ", \"z_alg\": %d, \"z_nthr\": %d, \"z_pad\": %d, \"z_pf\": %d}", v.z_alg, v.z_nthr,
-                v.z_pad, v.z_pf);
+  char z[144];
+  std::snprintf(z, sizeof z, ", \"z_alg\": %d, \"z_nthr\": %d, \"z_pad\": %d, \"z_pf\": %d, \"s_ord\": %d}", v.z_alg,
+                v.z_nthr, v.z_pad, v.z_pf, v.s_ord);
   return "{\"theta_inv\": " + tvj(v.inv) + ", \"theta_fwd\": " + tvj(v.fwd) + z;
 }
 
@@ -332,7 +333,7 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   h->nb.wstride = size_t(g.nr) * g.ntp * g.nz;
   h->nb.use_ct = false;
   h->nb.mixed = p->precision == NSP_MIXED;
-  h->nb.var = NlVariant{ThetaVar{0, 8, 256, 0, 1}, ThetaVar{0, 8, 256, 0, 1}, 1, 256, 0, 0};
+  h->nb.var = NlVariant{ThetaVar{0, 8, 256, 0, 1}, ThetaVar{0, 8, 256, 0, 1}, 1, 256, 0, 0, 1};
   if (!ct_has_zreg(g.nh)) {  // long axial rows: the 512-thread LDS form
     h->nb.var.z_alg = 0;
     h->nb.var.z_nthr = 512;
@@ -538,10 +539,17 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
                                         (const cpx<float>*)nb.W32, nb.wstride, (cpx<float>*)nb.Y32, s)
                       : ct_zpass<double>(v, g, (const cpx<double>*)nb.ct_tw_h, (const cpx<double>*)nb.tw_z, h->t.r,
                                          (const cpx<double>*)nb.W, nb.wstride, (cpx<double>*)nb.Y, s);
-    return nb.mixed ? ct_theta_fwd<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, (const cpx<float>*)nb.Y32,
-                                          nb.wstride, h->t.D, h->u, h->Nl, s)
-                    : ct_theta_fwd<double>(v, g, (const cpx<double>*)nb.ct_tw_t, (const cpx<double>*)nb.Y,
-                                           nb.wstride, h->t.D, h->u, h->Nl, s);
+    if (which == 2)
+      return nb.mixed ? ct_theta_fwd<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, (const cpx<float>*)nb.Y32,
+                                            nb.wstride, h->t.D, h->u, h->Nl, s)
+                      : ct_theta_fwd<double>(v, g, (const cpx<double>*)nb.ct_tw_t, (const cpx<double>*)nb.Y,
+                                             nb.wstride, h->t.D, h->u, h->Nl, s);
+    // a predictor sweep of the current state into the free iterate buffer
+    // (E and the iterate are recomputed by the next step)
+    Lin2Args a = lin_args(h, g, h->Nl, h->u, h->ui, true);
+    a.ord = v.s_ord;
+    launch_sweep2(a, s);
+    return true;
   };
   std::string rep = "{\"tuned\": true, \"candidates_ms\": {";
   bool first = true;
@@ -607,6 +615,16 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
     const double t = time_it(1, v, lab);
     if (t >= 0 && t < bt) { bt = t; best.z_alg = 1; best.z_pf = pf; }
   }
+  // linear sweep: order of its (m, -m, axial chunk) workgroups
+  bt = 1e30;
+  for (int so = 0; so < 2; so++) {
+    NlVariant v = best;
+    v.s_ord = so;
+    char lab[64];
+    std::snprintf(lab, sizeof lab, "sweep ord=%d", so);
+    const double t = time_it(3, v, lab);
+    if (t >= 0 && t < bt) { bt = t; best.s_ord = so; }
+  }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   HIPCHK(hipGetLastError());
@@ -643,6 +661,8 @@ nsp_status nsp_set_tuning(nsp_handle* h, const char* json) {
   v.z_nthr = int(zn);
   v.z_pad = int(zp);
   v.z_pf = int(zf);
+  const long so = json_int(t, base, "s_ord");  // absent in older tuning text: keep the current order
+  if (so >= 0) v.s_ord = so ? 1 : 0;
   // accept only variants that are compiled for this size: one nonlinear
   // evaluation of the current state into the scratch of the nonlinear term
   const NlVariant old = h->nb.var;

@@ -239,8 +239,8 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
   const Geom& g = A.g;
   if (step_done(g)) return;
   const int nch = (g.nz + 63) / 64;
-  // XCD-contiguous: the chunks of one azimuthal row, adjacent in memory, run in one L2
-  const int lb = xcd_block(blockIdx.x, gridDim.x);
+  // ord 1: the chunks of one azimuthal row, adjacent in memory, run in one XCD's L2
+  const int lb = A.ord ? xcd_block(blockIdx.x, gridDim.x) : (int)blockIdx.x;
   const int pair = lb / nch, ch = lb - pair * nch;
   if (pair >= g.nt / 2) return;  // beyond this pipe's extent
   const int nth = g.nt / 2;

@@ -149,6 +149,7 @@ struct Lin2Args {
   double2* SP;          // [1] scratch
   unsigned long long* norm;
   int predictor;
+  int ord;              // block order (NlVariant::s_ord)
 };
 void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, hipStream_t s);
 size_t factor_count(const Geom& g);  // doubles of the stored LU factors
@@ -175,6 +176,7 @@ struct NlVariant {
   int z_nthr;  // scatter form: threads per row (256 or 512)
   int z_pad;   // scatter form: 1 = one LDS slot per 16 elements, 2 = row padding as the register form
   int z_pf;    // register form: prefetch the next transform's inputs
+  int s_ord;   // linear sweep: 1 = the chunks of an azimuthal row XCD-contiguous, 0 = dispatch order
 };
 // Nonlinear-phase buffers and FFT data.
 struct NlBufs {

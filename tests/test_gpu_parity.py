@@ -261,6 +261,47 @@ def test_long_plans_every_compiled_form(shape):
     q.close()
 
 
+def profiles(pipe):
+    """Mean axial velocity uz(r, m=0, kz=0) and the rms of each velocity
+    component over the fluctuating modes, per radial level (Parseval over the
+    stored half spectrum: weight 1 for l = 0, 2 otherwise)."""
+    u = pipe.get_state()
+    w = np.full(pipe.nz, 2.0)
+    w[0] = 1.0
+    rms = []
+    for c in range(3):
+        a = np.abs(u[c]) ** 2 * w
+        a[:, 0, 0] = 0.0
+        rms.append(np.sqrt(a.sum(axis=(1, 2))))
+    return u[2][:, 0, 0].real, np.array(rms)
+
+
+def test_config2_mixed_against_fp64_long_run():
+    """Config 2 (NR=64, NT=192, NZ=385, Re 5300, dt 0.002, k^-5/3 noise of rms
+    0.1 U_bulk): the mixed form against the FP64 form on the GPU over 500
+    steps.  Mean profile, rms profiles, kinetic energy and wall shear stay
+    within 1e-5 (the mixed form's error budget)."""
+    cfg = dict(nr=64, nt=192, nz=385, lz=2 * np.pi, re=5300.0, dt=0.002)
+    print("[gpu] config 2, 500 steps, fp64 and mixed", flush=True)
+    out = {}
+    for prec in (nsp.FP64, nsp.MIXED):
+        p = nsp.Pipe(**cfg, precision=prec)
+        p.seeded_ic(20261202 + 2 * 1000, 4, 4, nsp.IC_KOLMOGOROV, 0.05)
+        n_it, _, flags = p.step(500)
+        assert np.all(flags == 0)
+        out[prec] = (profiles(p), p.diagnostics(), n_it)
+        p.close()
+    (m64, r64), d64, n64 = out[nsp.FP64]
+    (m32, r32), d32, n32 = out[nsp.MIXED]
+    e_mean = np.abs(m32 - m64).max() / np.abs(m64).max()
+    e_rms = np.abs(r32 - r64).max() / np.abs(r64).max()
+    e_ke = abs(d32["ke"] - d64["ke"]) / d64["ke"]
+    e_tau = abs(d32["u_tau"] - d64["u_tau"]) / d64["u_tau"]
+    print(f"[gpu] config 2 mixed vs fp64 after 500 steps: mean {e_mean:.3e} rms {e_rms:.3e} "
+          f"ke {e_ke:.3e} u_tau {e_tau:.3e}; correctors {n64.sum()} / {n32.sum()}", flush=True)
+    assert max(e_mean, e_rms, e_ke, e_tau) <= 1e-5
+
+
 def test_config5_mixed_steps():
     """The largest configuration, NR=160, NT=768, NZ=3073 (377M points per
     component), in the mixed form on one GPU: a few steps from seeded

@@ -41,7 +41,7 @@ def algorithmic_bytes(name, cfg, fp64):
     M = nr * (3 * nt // 2) * nz
     cst, ctr, cnl = 16, (16 if fp64 else 8), (16 if fp64 else 8)
     table = {
-        "dr3_kernel": 6 * N * cst,                       # outside the stage model
+        "dr3_kernel": 3 * N * cst + 3 * N * cnl,         # outside the stage model (d_r u at N(u)'s precision)
         "theta_inv_kernel": 3 * N * cst + 9 * M * ctr,
         "zpass_kernel": 12 * M * ctr,
         "theta_fwd_kernel": 3 * M * ctr + 3 * N * cnl,
@@ -144,6 +144,26 @@ def roofline(res, cfg, fp64, traffic):
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=tr, algorithmic_bytes=alg,
                 avg_launch_ms=round(avg_s * 1e3, 4), launches=cnt,
                 traffic_gbs=None if tr is None else round(tr / avg_s / 1e9, 1))
+
+
+def stage_rooflines(res, cfg, fp64, traffic):
+    """Per stage: time per launch, algorithmic and measured-traffic rates and
+    their fractions of the HBM peak."""
+    form = "fp64" if fp64 else "mixed"
+    out = {}
+    for name, (ms, cnt) in res["kernels"].items():
+        if not cnt:
+            continue
+        avg_s = ms / cnt / 1e3
+        alg = algorithmic_bytes(name, cfg, fp64)
+        d = dict(ms=round(avg_s * 1e3, 4), alg_gbs=round(alg / avg_s / 1e9, 1),
+                 alg_frac=round(alg / avg_s / 1e9 / HBM_PEAK_GBS, 4))
+        tk = (traffic or {}).get(form, {}).get("kernels", {}).get(name)
+        if tk:
+            d["traffic_gbs"] = round(tk["traffic"] / avg_s / 1e9, 1)
+            d["traffic_frac"] = round(tk["traffic"] / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+        out[name] = d
+    return out
 
 
 def load_traffic():
@@ -266,6 +286,7 @@ def main():
                    "nr": cfg["nr"], "nt": cfg["nt"], "nz": cfg["nz"], "re": cfg["re"], "dt": cfg["dt"],
                    "form": head, "n_it_mean": n_it_mean},
         "roofline": roofline(r, cfg, fp64, traffic),
+        "stage_roofline": stage_rooflines(r, cfg, fp64, traffic),
         "step_model_hbm_frac": round(step_bytes(cfg, fp64, n_it_mean) * r["steps_per_s"] / 1e9 / HBM_PEAK_GBS, 4),
     }
     for f in want:

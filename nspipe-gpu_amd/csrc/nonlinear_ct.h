@@ -321,6 +321,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
 
   cpx<T> ur[R0], ut[R0], uz[R0], acc[R0];
   cpx<T> A[R0], B[R0];
+  // split twiddles of the owned samples: the same for all 12 transforms
+  cpx<T> wz[R0];
+#pragma unroll
+  for (int q = 0; q < R0; q++) wz[q] = own ? twz[tid + NB1 * q] : zero;
   load(src(0), A, B);
 #pragma unroll 1
   for (int t = 0; t < 12; t++) {
@@ -342,7 +346,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
         }
         const cpx<T> E{a.x + b.x, a.y - b.y};
         const cpx<T> dl{a.x - b.x, a.y + b.y};
-        const cpx<T> O = cx_mulc(dl, twz[k]);
+        const cpx<T> O = cx_mulc(dl, wz[q]);
         v[q] = cpx<T>{E.x - O.y, E.y + O.x};
       }
       first_pass_store<T, +1, LY, 1>(PL{}, v, buf, lt);

@@ -66,6 +66,13 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
   if (i >= g.nr || l0 >= nz) return;  // beyond this member's extent
   const int tid = threadIdx.x;
   const double ri = r[i];
+  // m u / r: the FP32 form multiplies by the reciprocal (the result is rounded
+  // to FP32 anyway), the FP64 form divides as derivt_k does
+  const double iri = 1.0 / ri;
+  auto m_over_r = [&](double kt, double x) -> double {
+    if constexpr (sizeof(T) == 4) return kt * x * iri;
+    else return kt * x / ri;
+  };
   constexpr int half = NT / 2;
   for (int c = 0; c < 3; c++) {
     const double2* src = u + c * g.n;
@@ -99,7 +106,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
           cpx<T> w = (kind == 1) ? DR[q] : cpx<T>{T(v.x), T(v.y)};
           if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
             const double kt = double(it < half ? it : it - NT);
-            w = cpx<T>{T(-kt * v.y / ri), T(kt * v.x / ri)};
+            w = cpx<T>{T(m_over_r(-kt, v.y)), T(m_over_r(kt, v.x))};
           }
           const int j = it < half ? it : it + half;  // padForward_k row map
           buf[LY::pos(PL::digitrev(j) * TL + tl)] = w;
@@ -525,6 +532,13 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
   if (i >= g.nr || l0 >= g.nz) return;  // beyond this member's extent
   const int tid = threadIdx.x;
   const double ri = r[i];
+  // m u / r: the FP32 form multiplies by the reciprocal (the result is rounded
+  // to FP32 anyway), the FP64 form divides as derivt_k does
+  const double iri = 1.0 / ri;
+  auto m_over_r = [&](double kt, double x) -> double {
+    if constexpr (sizeof(T) == 4) return kt * x * iri;
+    else return kt * x / ri;
+  };
 #pragma unroll 1
   for (int f = 0; f < 9; f++) {
     const int c = f % 3, kind = f / 3;  // kind 0: u, 1: d_r u, 2: d_theta u / r
@@ -550,7 +564,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
                 const double2 a = src[(size_t)it * nz + l];
                 if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
                   const double kt = double(it < half ? it : it - NT);
-                  v[q] = cpx<T>{T(-kt * a.y / ri), T(kt * a.x / ri)};
+                  v[q] = cpx<T>{T(m_over_r(-kt, a.y)), T(m_over_r(kt, a.x))};
                 } else {
                   v[q] = cpx<T>{T(a.x), T(a.y)};
                 }

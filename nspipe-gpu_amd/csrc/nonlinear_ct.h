@@ -74,6 +74,23 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
     else return kt * x / ri;
   };
   constexpr int half = NT / 2;
+  // digit-reversed LDS positions of this thread's elements (padForward_k row
+  // map included): the same for all nine transforms of the block
+  int pos[EPT];
+#pragma unroll
+  for (int q = 0; q < EPT; q++) {
+    const int e = tid + q * NTHR;
+    const int it = e / TL, tl = e - it * TL;
+    const int j = it < half ? it : it + half;
+    pos[q] = e < NT * TL ? LY::pos(PL::digitrev(j) * TL + tl) : 0;
+  }
+  constexpr int EPZ = ((NTP - NT) * TL + NTHR - 1) / NTHR;  // padding-row elements per thread
+  int zpos[EPZ];
+#pragma unroll
+  for (int q = 0; q < EPZ; q++) {
+    const int e = tid + q * NTHR;
+    zpos[q] = e < (NTP - NT) * TL ? LY::pos(PL::digitrev(half + e / TL) * TL + e % TL) : -1;
+  }
   for (int c = 0; c < 3; c++) {
     const double2* src = u + c * g.n;
     double2 U[EPT];
@@ -93,23 +110,21 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
 #pragma unroll 1
     for (int kind = 0; kind < 3; kind++) {  // 0: u, 1: d_r u, 2: d_theta u / r
       const int lt = opaque(tid);
-      for (int e = lt; e < (NTP - NT) * TL; e += NTHR) {  // padding rows
-        const int j = half + e / TL, tl = e % TL;
-        buf[LY::pos(PL::digitrev(j) * TL + tl)] = cpx<T>{T(0), T(0)};
-      }
+#pragma unroll
+      for (int q = 0; q < EPZ; q++)  // padding rows
+        if (zpos[q] >= 0) buf[zpos[q]] = cpx<T>{T(0), T(0)};
 #pragma unroll
       for (int q = 0; q < EPT; q++) {
         const int e = lt + q * NTHR;
         if (e < NT * TL) {
-          const int it = e / TL, tl = e - it * TL;
+          const int it = e / TL;
           const double2 v = U[q];
           cpx<T> w = (kind == 1) ? DR[q] : cpx<T>{T(v.x), T(v.y)};
           if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
             const double kt = double(it < half ? it : it - NT);
             w = cpx<T>{T(m_over_r(-kt, v.y)), T(m_over_r(kt, v.x))};
           }
-          const int j = it < half ? it : it + half;  // padForward_k row map
-          buf[LY::pos(PL::digitrev(j) * TL + tl)] = w;
+          buf[pos[q]] = w;
         }
       }
       __syncthreads();

@@ -345,8 +345,12 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   cpx<T> A[R0], B[R0];
   // split twiddles of the owned samples: the same for all 12 transforms
   cpx<T> wz[R0];
+  int spos[R0];  // LDS positions of the owned samples after the inverse transform
 #pragma unroll
-  for (int q = 0; q < R0; q++) wz[q] = own ? twz[tid + NB1 * q] : zero;
+  for (int q = 0; q < R0; q++) {
+    wz[q] = own ? twz[tid + NB1 * q] : zero;
+    spos[q] = LY::pos(tid + NB1 * q);
+  }
   load(src(0), A, B);
 #pragma unroll 1
   for (int t = 0; t < 12; t++) {
@@ -379,7 +383,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
 #pragma unroll
     for (int q = 0; q < R0; q++) {
       sm[q] = zero;
-      if (own) sm[q] = buf[LY::pos(lt + NB1 * q)];
+      if (own) sm[q] = buf[spos[q]];
     }
     __syncthreads();
     if (t == 0) {

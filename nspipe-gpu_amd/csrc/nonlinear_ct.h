@@ -343,13 +343,18 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
 
   cpx<T> ur[R0], ut[R0], uz[R0], acc[R0];
   cpx<T> A[R0], B[R0];
-  // split twiddles of the owned samples: the same for all 12 transforms
-  cpx<T> wz[R0];
-  int spos[R0];  // LDS positions of the owned samples after the inverse transform
+  // split twiddles and LDS sample positions of the owned samples, the same
+  // for all 12 transforms: kept in registers when that costs no occupancy
+  // (the R0 = 9 plans already run at 2 waves per SIMD), else recomputed
+  constexpr bool HOIST = R0 >= 9;
+  cpx<T> wz[HOIST ? R0 : 1];
+  int spos[HOIST ? R0 : 1];
+  if constexpr (HOIST) {
 #pragma unroll
-  for (int q = 0; q < R0; q++) {
-    wz[q] = own ? twz[tid + NB1 * q] : zero;
-    spos[q] = LY::pos(tid + NB1 * q);
+    for (int q = 0; q < R0; q++) {
+      wz[q] = own ? twz[tid + NB1 * q] : zero;
+      spos[q] = LY::pos(tid + NB1 * q);
+    }
   }
   load(src(0), A, B);
 #pragma unroll 1
@@ -372,7 +377,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
         }
         const cpx<T> E{a.x + b.x, a.y - b.y};
         const cpx<T> dl{a.x - b.x, a.y + b.y};
-        const cpx<T> O = cx_mulc(dl, wz[q]);
+        cpx<T> w;
+        if constexpr (HOIST) w = wz[q];
+        else w = twz[k];
+        const cpx<T> O = cx_mulc(dl, w);
         v[q] = cpx<T>{E.x - O.y, E.y + O.x};
       }
       first_pass_store<T, +1, LY, 1>(PL{}, v, buf, lt);
@@ -383,7 +391,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
 #pragma unroll
     for (int q = 0; q < R0; q++) {
       sm[q] = zero;
-      if (own) sm[q] = buf[spos[q]];
+      if (own) {
+        if constexpr (HOIST) sm[q] = buf[spos[q]];
+        else sm[q] = buf[LY::pos(lt + NB1 * q)];
+      }
     }
     __syncthreads();
     if (t == 0) {

@@ -427,11 +427,14 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
       return bail(fail(NSP_ERR_HIP, std::string("setup: ") + hipGetErrorString(e5 != hipSuccess ? e5 : e6)));
   }
   // transform intermediates after the setup buffers are freed (lower peak)
+  // W carries nh spare elements: the z pass's buffer loads of a row stop at
+  // its nz stored modes by the resource's range check, and the spare tail
+  // keeps even the last row's unchecked extent (nh elements) inside the block
   if (h->nb.mixed) {
-    TRY(dalloc(h, &h->nb.W32, 9 * h->nb.wstride));
+    TRY(dalloc(h, &h->nb.W32, 9 * h->nb.wstride + g.nh));
     TRY(dalloc(h, &h->nb.Y32, 3 * h->nb.wstride));
   } else {
-    TRY(dalloc(h, &h->nb.W, 9 * h->nb.wstride));
+    TRY(dalloc(h, &h->nb.W, 9 * h->nb.wstride + g.nh));
     TRY(dalloc(h, &h->nb.Y, 3 * h->nb.wstride));
   }
 #undef TRY

@@ -18,6 +18,34 @@ __device__ __forceinline__ int opaque(int x) {
   return x;
 }
 
+// Raw buffer loads.  A load whose byte offset lies past num_records returns
+// zero instead of touching memory, which gives the zero padding of a row
+// beyond its stored modes without clamps or selects.  The resource is the
+// gfx9-family layout: base address, stride 0, num_records, dword 3 = 32-bit
+// data format without swizzle.  The loads are LLVM's buffer-load intrinsics
+// bound by name (this compiler's b64 load builtin emits a 32-bit load).
+typedef int nsp_i4v __attribute__((ext_vector_type(4)));
+typedef float nsp_f2v __attribute__((ext_vector_type(2)));
+typedef double nsp_d2v __attribute__((ext_vector_type(2)));
+__device__ nsp_f2v nsp_raw_load_f32x2(nsp_i4v rsrc, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.buffer.load.v2f32");
+__device__ nsp_d2v nsp_raw_load_f64x2(nsp_i4v rsrc, int voffset, int soffset, int aux) __asm(
+    "llvm.amdgcn.raw.buffer.load.v2f64");
+__device__ __forceinline__ nsp_i4v buf_rsrc(const void* base, int bytes) {
+  const unsigned long long a = (unsigned long long)base;
+  return nsp_i4v{(int)(unsigned)a, (int)(unsigned)(a >> 32), bytes, 0x00020000};
+}
+template <typename T>
+__device__ __forceinline__ cpx<T> buf_load(nsp_i4v rs, int off) {
+  if constexpr (sizeof(T) == 4) {
+    const nsp_f2v v = nsp_raw_load_f32x2(rs, off, 0, 0);
+    return cpx<T>{v[0], v[1]};
+  } else {
+    const nsp_d2v v = nsp_raw_load_f64x2(rs, off, 0, 0);
+    return cpx<T>{v[0], v[1]};
+  }
+}
+
 // ------------------------------------------------------------------ block order
 // (level i, first column l0) of a theta block.  ord 0: level fastest;
 // ord 1: column tile fastest, so the blocks an XCD runs together cover whole
@@ -304,7 +332,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   if (row >= g.nr * g.ntp) return;  // beyond this member's extent
   const int i = row / g.ntp;
   const int tid = threadIdx.x;
-  const bool own = tid < NB1;
+  const bool own = NB1 == NTHR || tid < NB1;  // thread owns a first-pass butterfly
   const int nz = g.nz;
   const T rad = T(r[i]);
   // x / r; the FP32 form multiplies by the reciprocal (within an ulp, without
@@ -325,23 +353,22 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
     return W + f * ws + roff;
   };
   // X[k] and X[NH-k] of the owned samples, zero beyond the stored NZ modes:
-  // clamped in-range loads and a select, no per-element branches
+  // raw buffer loads over the NZ stored values of the row return zero past
+  // its end, so no clamp or select per element (every offset is below NH).
   auto load = [&](const cpx<T>* X, cpx<T>* A, cpx<T>* B) {
     const int lt = opaque(tid);
-    if (lt < NB1) {
+    if (NB1 == NTHR || lt < NB1) {
+      const nsp_i4v rs = buf_rsrc(X, nz * (int)sizeof(cpx<T>));
 #pragma unroll
       for (int q = 0; q < R0; q++) {
         const int k = lt + NB1 * q, kc = NH - k;
-        const cpx<T> a = X[min(k, nz - 1)], b = X[min(kc, nz - 1)];
-        A[q].x = k < nz ? a.x : T(0);
-        A[q].y = k < nz ? a.y : T(0);
-        B[q].x = kc < nz ? b.x : T(0);
-        B[q].y = kc < nz ? b.y : T(0);
+        A[q] = buf_load<T>(rs, k * (int)sizeof(cpx<T>));
+        B[q] = buf_load<T>(rs, kc * (int)sizeof(cpx<T>));
       }
     }
   };
 
-  cpx<T> ur[R0], ut[R0], uz[R0], acc[R0];
+  cpx<T> ur[R0], ut[R0], uz[R0];
   cpx<T> A[R0], B[R0];
   // split twiddles and LDS sample positions of the owned samples, the same
   // for all 12 transforms: kept in registers when that costs no occupancy
@@ -356,13 +383,18 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
       spos[q] = LY::pos(tid + NB1 * q);
     }
   }
-  load(src(0), A, B);
-#pragma unroll 1
-  for (int t = 0; t < 12; t++) {
+  // Inverse transform t of the inputs in A, B (DZ: differentiated in z);
+  // leaves the physical values at the owned samples in sm.  With PF the
+  // inputs of transform t + 1 are loaded into A, B meanwhile.  Every call
+  // site is its own inlined copy, with DZ and the consumer of sm fixed, so
+  // the per-transform branches and register copies of a loop over t vanish.
+  auto inverse = [&](int t, auto dz_tag, cpx<T>(&sm)[R0]) {
+    constexpr bool DZ = decltype(dz_tag)::value;
     const int lt = opaque(tid);
     cpx<T> nA[R0], nB[R0];
-    if (PF && t + 1 < 12) load(src(t + 1), nA, nB);
-    const bool dz = t >= 3 && (t - 3) % 3 == 2;
+    if constexpr (PF) {
+      if (t + 1 < 12) load(src(t + 1), nA, nB);
+    }
     if (own) {
       cpx<T> v[R0];
 #pragma unroll
@@ -370,7 +402,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
         const int k = lt + NB1 * q, kc = NH - k;
         cpx<T> a = A[q], b = B[q];
         if (k == 0) { a.y = T(0); b.y = T(0); }
-        if (dz) {  // derivz_k (mod_deriv.cu:24-35)
+        if constexpr (DZ) {  // derivz_k (mod_deriv.cu:24-35)
           const T ka = kz0 * T(k), kb = kz0 * T(kc);
           a = cpx<T>{-ka * a.y, ka * a.x};
           b = cpx<T>{-kb * b.y, kb * b.x};
@@ -387,7 +419,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
     }
     __syncthreads();
     fft_rest<T, NTHR, +1, LY>(PL{}, buf, tw, lt);
-    cpx<T> sm[R0];
 #pragma unroll
     for (int q = 0; q < R0; q++) {
       sm[q] = zero;
@@ -397,69 +428,81 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
       }
     }
     __syncthreads();
-    if (t == 0) {
-#pragma unroll
-      for (int q = 0; q < R0; q++) ur[q] = sm[q];
-    } else if (t == 1) {
-#pragma unroll
-      for (int q = 0; q < R0; q++) ut[q] = sm[q];
-    } else if (t == 2) {
-#pragma unroll
-      for (int q = 0; q < R0; q++) uz[q] = sm[q];
-    } else {
-      // calcNonlinear_k (mod_io_nonlinear.cu:142-170), summed in its order
-      const int kk = (t - 3) % 3, c = (t - 3) / 3;
-      if (kk == 0) {
-#pragma unroll
-        for (int q = 0; q < R0; q++) acc[q] = cpx<T>{ur[q].x * sm[q].x, ur[q].y * sm[q].y};
-      } else if (kk == 1) {
-#pragma unroll
-        for (int q = 0; q < R0; q++) {
-          acc[q].x += ut[q].x * sm[q].x;
-          acc[q].y += ut[q].y * sm[q].y;
-        }
-      } else {
-        if (own) {
-          cpx<T> v[R0];
-#pragma unroll
-          for (int q = 0; q < R0; q++) {
-            T vx = acc[q].x + uz[q].x * sm[q].x, vy = acc[q].y + uz[q].y * sm[q].y;
-            if (c == 0) {
-              vx = vx - ut[q].x * over_r(ut[q].x);
-              vy = vy - ut[q].y * over_r(ut[q].y);
-            } else if (c == 1) {
-              vx = vx + ur[q].x * over_r(ut[q].x);
-              vy = vy + ur[q].y * over_r(ut[q].y);
-            }
-            v[q] = cpx<T>{-vx, -vy};
-          }
-          first_pass_store<T, -1, LY, 1>(PL{}, v, buf, lt);
-        }
-        __syncthreads();
-        fft_rest<T, NTHR, -1, LY>(PL{}, buf, tw, lt);
-        cpx<T>* dst = Y + c * ws + roff;
-        for (int k = lt; k < nz; k += NTHR) {  // half-length R2C post-processing
-          cpx<T> X = zero;
-          if (k < nz - 1) {
-            const cpx<T> zk = buf[LY::pos(k)], zc = buf[LY::pos(k == 0 ? 0 : NH - k)];
-            const cpx<T> E{T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y - zc.y)};
-            const cpx<T> O{T(0.5) * (zk.y + zc.y), T(-0.5) * (zk.x - zc.x)};
-            const cpx<T> wo = cx_mul(twz[k], O);
-            X = cpx<T>{E.x + wo.x, E.y + wo.y};
-          }
-          dst[k] = X;
-        }
-        __syncthreads();
-      }
-    }
-    if (t + 1 < 12) {
-      if (PF) {
+    if constexpr (PF) {
+      if (t + 1 < 12) {
 #pragma unroll
         for (int q = 0; q < R0; q++) { A[q] = nA[q]; B[q] = nB[q]; }
-      } else {
-        load(src(t + 1), A, B);
       }
+    } else {
+      if (t + 1 < 12) load(src(t + 1), A, B);
     }
+  };
+  using NoDz = std::integral_constant<bool, false>;
+  using Dz = std::integral_constant<bool, true>;
+
+  load(src(0), A, B);
+  // velocities: after three transforms ur, ut, uz hold transforms 0, 1, 2
+#pragma unroll
+  for (int q = 0; q < R0; q++) ut[q] = uz[q] = zero;
+#pragma unroll 1
+  for (int t = 0; t < 3; t++) {
+    cpx<T> sm[R0];
+    inverse(t, NoDz{}, sm);
+#pragma unroll
+    for (int q = 0; q < R0; q++) {
+      ur[q] = ut[q];
+      ut[q] = uz[q];
+      uz[q] = sm[q];
+    }
+  }
+  // calcNonlinear_k (mod_io_nonlinear.cu:142-170), summed in its order, per
+  // component c: ur d_r u_c + ut d_theta u_c / r + uz d_z u_c (+ curvature)
+#pragma unroll 1
+  for (int c = 0; c < 3; c++) {
+    const int t0 = 3 + 3 * c;
+    cpx<T> acc[R0], sm[R0];
+    inverse(t0, NoDz{}, sm);
+#pragma unroll
+    for (int q = 0; q < R0; q++) acc[q] = cpx<T>{ur[q].x * sm[q].x, ur[q].y * sm[q].y};
+    inverse(t0 + 1, NoDz{}, sm);
+#pragma unroll
+    for (int q = 0; q < R0; q++) {
+      acc[q].x += ut[q].x * sm[q].x;
+      acc[q].y += ut[q].y * sm[q].y;
+    }
+    inverse(t0 + 2, Dz{}, sm);
+    if (own) {
+      cpx<T> v[R0];
+#pragma unroll
+      for (int q = 0; q < R0; q++) {
+        T vx = acc[q].x + uz[q].x * sm[q].x, vy = acc[q].y + uz[q].y * sm[q].y;
+        if (c == 0) {
+          vx = vx - ut[q].x * over_r(ut[q].x);
+          vy = vy - ut[q].y * over_r(ut[q].y);
+        } else if (c == 1) {
+          vx = vx + ur[q].x * over_r(ut[q].x);
+          vy = vy + ur[q].y * over_r(ut[q].y);
+        }
+        v[q] = cpx<T>{-vx, -vy};
+      }
+      first_pass_store<T, -1, LY, 1>(PL{}, v, buf, opaque(tid));
+    }
+    __syncthreads();
+    const int lt = opaque(tid);
+    fft_rest<T, NTHR, -1, LY>(PL{}, buf, tw, lt);
+    cpx<T>* dst = Y + c * ws + roff;
+    for (int k = lt; k < nz; k += NTHR) {  // half-length R2C post-processing
+      cpx<T> X = zero;
+      if (k < nz - 1) {
+        const cpx<T> zk = buf[LY::pos(k)], zc = buf[LY::pos(k == 0 ? 0 : NH - k)];
+        const cpx<T> E{T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y - zc.y)};
+        const cpx<T> O{T(0.5) * (zk.y + zc.y), T(-0.5) * (zk.x - zc.x)};
+        const cpx<T> wo = cx_mul(twz[k], O);
+        X = cpx<T>{E.x + wo.x, E.y + wo.y};
+      }
+      dst[k] = X;
+    }
+    __syncthreads();
   }
 }
 

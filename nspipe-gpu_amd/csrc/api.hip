@@ -39,11 +39,11 @@ struct nsp_handle {
   double2* ui = nullptr;    // iterate
   double2* un = nullptr;    // next iterate
   double2* Nl = nullptr;    // nonlinear term
-  double2* E = nullptr;     // explicit part of the step (+/-)
-  double2* SR = nullptr;    // sweep scratch (3 fields)
-  double2* SP = nullptr;    // sweep scratch (1 field)
+  double2* E = nullptr;     // explicit part of the step (+/-), sweep-side layout
+  double2* SR = nullptr;    // sweep scratch (3 fields), sweep-side layout
+  double2* SP = nullptr;    // sweep scratch (1 field), sweep-side layout
   double* F = nullptr;      // banded LU factors (u+ all modes, u- row NT/2, uz and p rows <= NT/2)
-  double* PH = nullptr;     // homogeneous pressure data [NR][2][MODES]
+  double* PH = nullptr;     // homogeneous pressure data (rows it <= NT/2), sweep-side layout
   double* Dc = nullptr;     // centred D table
   double* Luc = nullptr;    // centred Lu table
   unsigned long long* norm = nullptr;  // device [2]
@@ -377,15 +377,17 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   // N(u) and d_r u are FP32 pairs in the mixed form
   const size_t nf = h->nb.mixed ? (3 * n + 1) / 2 : 3 * n;
   TRY(dalloc(h, &h->Nl, nf));
-  TRY(dalloc(h, &h->E, 3 * n));
-  TRY(dalloc(h, &h->SR, 3 * n));
-  TRY(dalloc(h, &h->SP, n));
+  // sweep-side layout (E, SR, SP, F, PH): per-workgroup columns, padded to
+  // whole chunks of 64 axial modes; SR also serves as 3N scratch elsewhere
+  TRY(dalloc(h, &h->E, 3 * sw_field(g)));
+  TRY(dalloc(h, &h->SR, 3 * sw_field(g)));
+  TRY(dalloc(h, &h->SP, sw_field(g)));
   if (share) {
     h->F = share->F;
     h->PH = share->PH;
   } else {
     TRY(dalloc(h, &h->F, factor_count(g)));
-    TRY(dalloc(h, &h->PH, 2 * n));
+    TRY(dalloc(h, &h->PH, sw_ph_count(g)));
   }
   TRY(upload(h, &h->Dc, centred_table(g.nr, h->host.D)));
   TRY(upload(h, &h->Luc, centred_table(g.nr, h->host.Lu)));

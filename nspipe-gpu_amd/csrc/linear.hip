@@ -306,12 +306,15 @@ __global__ __launch_bounds__(256) void setup_influence_kernel(Geom g, DevTables 
   // u_hP = grad p_h (derivr flag 0, derivt, derivz) in r/theta form
   {
     const int ip0 = abs((abs(q.m) % 2) - 0) * STEN * nr;
+    const bool habs = k / g.nz <= g.nt / 2;  // the sweep reads PH at |m| only
     for (int i = 0; i < nr; i++) {
       const size_t h = at(g, i, k);
       const double2 dr = stencil(g, t.D, ip0, i, P, k);
       const double2 ph = P[h];
-      PH[(size_t)(2 * i) * g.modes + k] = dr.x;  // p_h is real: its d/dr and value
-      PH[(size_t)(2 * i + 1) * g.modes + k] = ph.x;
+      if (habs) {  // p_h is real: its d/dr and value, stored for |m| (sweep-side layout)
+        PH[sw_abs_idx(g, 2, i, 0, k)] = dr.x;
+        PH[sw_abs_idx(g, 2, i, 1, k)] = ph.x;
+      }
       b.uhP[h] = dr;
       b.uhP[n + h] = make_double2(-q.kt * ph.y / t.r[i], q.kt * ph.x / t.r[i]);
       b.uhP[2 * n + h] = make_double2(-q.kz * ph.y, q.kz * ph.x);

@@ -25,7 +25,8 @@
 // of the 2m/r^2 term) and the uz and pressure factors and the pressure
 // homogeneous solution depend on |m|, so both waves read the same lines.
 // Scratch SR (3 complex per row) and SP (1 complex per row) carry the column
-// between passes.
+// between passes.  Scratch, E, the factors and PH are stored per workgroup
+// (sweep-side layout, nsp_internal.h): a workgroup's column is contiguous.
 #include "nsp_internal.h"
 
 namespace nsp {
@@ -59,17 +60,31 @@ __device__ __forceinline__ void rt_of(double2& a, double2& b) {  // decoupleBack
 
 // factor slot c of operator `kind` at row i: c 0..2 = L multipliers of
 // columns i-3..i-1, 3 = 1/U_ii, 4..6 = U_{i,i+1..i+3}.  Only the factors the
-// sweep reads are stored, mode-fastest: u+ for every mode; u- for the
-// azimuthal row NT/2 only (elsewhere u- at m is u+ at -m); uz and p for the
-// rows it <= NT/2 (they depend on |m|, read at k_abs).
+// sweep reads are stored, in the sweep-side layout:
+//   u+ (kind 0), every mode:          [workgroup][NR][7][128 lanes]
+//   u- (kind 1), azimuthal row NT/2:  [chunk][NR][7][128] (lanes 64..127);
+//                elsewhere u- at m is u+ at -m, the other wave of the workgroup
+//   uz, p (kinds 2, 3), rows it <= NT/2 (they depend on |m|):
+//                                     [workgroup or Nyquist chunk][NR][7][64] each
+// Per row the u+/u- sets step by FR_UP = 7*128 doubles and per slot by 128,
+// the uz/p sets by FR_ABS = 7*64 and 64.
+constexpr int FR_UP = 7 * 128, FR_ABS = 7 * 64;
+__host__ __device__ __forceinline__ size_t f_u1(const Geom& g) { return sw_nblk(g) * g.nr * FR_UP; }
+__host__ __device__ __forceinline__ size_t f_abs(const Geom& g) {
+  return f_u1(g) + (size_t)sw_nch(g) * g.nr * FR_UP;
+}
+__host__ __device__ __forceinline__ size_t f_absn(const Geom& g) {  // doubles of one |m| set
+  return (sw_nblk(g) + sw_nch(g)) * g.nr * FR_ABS;
+}
 __host__ __device__ __forceinline__ size_t fidx(const Geom& g, int kind, int i, int c, int k) {
-  const size_t row = (size_t)i * 7 + c, r7 = (size_t)g.nr * 7;
-  const size_t mh = (size_t)(g.nt / 2 + 1) * g.nz;
-  if (kind == 0) return row * g.modes + k;
-  const size_t b1 = r7 * g.modes;
-  if (kind == 1) return b1 + row * g.nz + (size_t)(k - (g.nt / 2) * g.nz);
-  const size_t b2 = b1 + r7 * g.nz;
-  return b2 + (size_t)(kind - 2) * r7 * mh + row * mh + k;
+  const int it = k / g.nz, l = k - it * g.nz, nth = g.nt / 2;
+  if (kind == 0) {
+    const int pair = it < nth ? it : (it == nth ? 0 : g.nt - it), w = it < nth ? 0 : 1;
+    const size_t lb = (size_t)pair * sw_nch(g) + (l >> 6);
+    return ((lb * g.nr + i) * 7 + c) * 128 + w * 64 + (l & 63);
+  }
+  if (kind == 1) return f_u1(g) + (((size_t)(l >> 6) * g.nr + i) * 7 + c) * 128 + 64 + (l & 63);
+  return f_abs(g) + (size_t)(kind - 2) * f_absn(g) + sw_abs_idx(g, 7, i, c, k);
 }
 
 // The operator rows of one mode (LUsolve_*_k): kind 0 u+, 1 u-, 2 uz
@@ -249,13 +264,23 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
   if (l >= g.nz) return;
   const int k = it * g.nz + l;
   const ModeQ q = modeq(g, k);
-  // stored factor columns: u+ at this mode; u- = u+ at -m (Nyquist row: its
-  // own u- factors); uz, p and the pressure homogeneous solution at |m|
-  const int k_abs = (it <= nth ? it : g.nt - it) * g.nz + l;
-  const int k_um = (it == nth) ? k : (it == 0 ? k : (g.nt - it) * g.nz + l);
-  const int kind_um = (it == nth) ? 1 : 0;
-  const size_t n = g.n;
+  // sweep-side bases of this lane: factors u+ at this mode; u- = u+ at -m,
+  // the other wave's lane (the Nyquist row has its own u- set, row 0 is its
+  // own mirror); uz, p and the pressure homogeneous solution at |m|, the
+  // wave-0 lane of the workgroup (Nyquist row: its own region)
   const int nr = g.nr;
+  const int lane = threadIdx.x & 63;
+  const bool nyq = pair == 0 && threadIdx.x >= 64;
+  const size_t fb_up = (size_t)lb * nr * FR_UP + threadIdx.x;
+  const size_t fb_um = nyq ? f_u1(g) + (size_t)ch * nr * FR_UP + threadIdx.x
+                           : (pair == 0 ? fb_up : (size_t)lb * nr * FR_UP + (threadIdx.x ^ 64));
+  const size_t ab = nyq ? sw_nblk(g) + ch : (size_t)lb;
+  const size_t fb_uz = f_abs(g) + ab * nr * FR_ABS + lane;
+  const size_t fb_p = fb_uz + f_absn(g);
+  const size_t phb = ab * nr * 128 + lane;              // PH row i: phb + 128 i (+64: p_h)
+  const size_t sb3 = (size_t)lb * nr * 384 + threadIdx.x;  // SR, E row i, component c: sb3 + 384 i + 128 c
+  const size_t sb1 = (size_t)lb * nr * 128 + threadIdx.x;  // SP row i: sb1 + 128 i
+  const size_t n = g.n;
   const int MS = g.modes;
   const double d_im = g.d_im;
   const int par_u = abs((abs(q.m) % 2) - 1);  // parity table of ur, ut (u+, u-)
@@ -345,10 +370,12 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
             px += idt * aux.x;
             py += idt * aux.y;
             rh[c] = make_double2(nn[c].x + px, nn[c].y + py);
-            A.E[c * n + h] = make_double2(px + (1.0 - d_im) * nn[c].x, py + (1.0 - d_im) * nn[c].y);
+            A.E[sb3 + (size_t)j * 384 + c * 128] =
+                make_double2(px + (1.0 - d_im) * nn[c].x, py + (1.0 - d_im) * nn[c].y);
           }
         } else {
-          const double2 e0 = A.E[h], e1 = A.E[n + h], e2 = A.E[2 * n + h];
+          const size_t he = sb3 + (size_t)j * 384;
+          const double2 e0 = A.E[he], e1 = A.E[he + 128], e2 = A.E[he + 256];
           rh[0] = make_double2(e0.x + d_im * n0.x, e0.y + d_im * n0.y);
           rh[1] = make_double2(e1.x + d_im * n1.x, e1.y + d_im * n1.y);
           rh[2] = make_double2(e2.x + d_im * n2.x, e2.y + d_im * n2.y);
@@ -357,9 +384,10 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         R0 = rh[0];
         R1 = rh[1];
         R2 = rh[2];
-        A.SR[h] = R0;
-        A.SR[n + h] = R1;
-        A.SR[2 * n + h] = R2;
+        const size_t hs = sb3 + (size_t)j * 384;
+        A.SR[hs] = R0;
+        A.SR[hs + 128] = R1;
+        A.SR[hs + 256] = R2;
       }
 #pragma unroll
       for (int w = 0; w < 6; w++) Rr[w] = Rr[w + 1];
@@ -393,15 +421,15 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
           py += q.kz * uz.x;
           dv = make_double2(px, py);
         }
-        const double l0 = A.F[fidx(g, 3, i, 0, k_abs)], l1 = A.F[fidx(g, 3, i, 1, k_abs)],
-                     l2 = A.F[fidx(g, 3, i, 2, k_abs)];
+        const double* fp = A.F + fb_p + (size_t)i * FR_ABS;
+        const double l0 = fp[0], l1 = fp[64], l2 = fp[128];
         dv.x -= l0 * yp[0].x;
         dv.y -= l0 * yp[0].y;
         dv.x -= l1 * yp[1].x;
         dv.y -= l1 * yp[1].y;
         dv.x -= l2 * yp[2].x;
         dv.y -= l2 * yp[2].y;
-        A.SP[(size_t)i * MS + k] = dv;
+        A.SP[sb1 + (size_t)i * 128] = dv;
         yp[0] = yp[1];
         yp[1] = yp[2];
         yp[2] = dv;
@@ -415,10 +443,10 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
 #pragma unroll
     for (int w = 0; w < 3; w++) pw[w] = z2;
     for (int t = nr - 1; t >= 0; t--) {
-      const double ru = A.F[fidx(g, 3, t, 3, k_abs)];
-      const double u1 = A.F[fidx(g, 3, t, 4, k_abs)], u2 = A.F[fidx(g, 3, t, 5, k_abs)],
-                   u3 = A.F[fidx(g, 3, t, 6, k_abs)];
-      const size_t h = (size_t)t * MS + k;
+      const double* fp = A.F + fb_p + (size_t)t * FR_ABS;
+      const double ru = fp[3 * 64];
+      const double u1 = fp[4 * 64], u2 = fp[5 * 64], u3 = fp[6 * 64];
+      const size_t h = sb1 + (size_t)t * 128;
       double2 x = A.SP[h];
       x.x -= u1 * pw[0].x;
       x.y -= u1 * pw[0].y;
@@ -447,11 +475,11 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     for (int t = 0; t < nr + IW; t++) {
 #pragma unroll
       for (int w = 0; w < 6; w++) pw[w] = pw[w + 1];
-      pw[6] = (t < nr) ? A.SP[(size_t)t * MS + k] : z2;
+      pw[6] = (t < nr) ? A.SP[sb1 + (size_t)t * 128] : z2;
       const int j = t - IW;
       if (j < 0) continue;
       // addGrad_k (mod_radialFD.cu:115-143)
-      const size_t h = (size_t)j * MS + k;
+      const size_t h = sb3 + (size_t)j * 384;
       const double* Dr = Dcp + STEN * j;
       double px = 0.0, py = 0.0;
 #pragma unroll
@@ -461,7 +489,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       }
       const double rK = A.r[j];
       const double2 ph = pw[3];
-      double2 rh[3] = {A.SR[h], A.SR[n + h], A.SR[2 * n + h]};
+      double2 rh[3] = {A.SR[h], A.SR[h + 128], A.SR[h + 256]};
       rh[0].x -= px;
       rh[0].y -= py;
       rh[1].x -= -q.kt * ph.y / rK;
@@ -472,10 +500,13 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       if (j == nr - 1) rh[0] = rh[1] = rh[2] = z2;
 #pragma unroll
       for (int c = 0; c < 3; c++) {  // forward elimination
-        const int fkind = c == 1 ? kind_um : c, fk = c == 0 ? k : (c == 1 ? k_um : k_abs);
+        // u+, u-: rows of FR_UP, slots of 128; uz: rows of FR_ABS, slots of 64
+        const double* fr = c == 0 ? A.F + fb_up + (size_t)j * FR_UP
+                                  : (c == 1 ? A.F + fb_um + (size_t)j * FR_UP : A.F + fb_uz + (size_t)j * FR_ABS);
+        const int fs = c == 2 ? 64 : 128;
         double Lc[IW];
 #pragma unroll
-        for (int w = 0; w < IW; w++) Lc[w] = A.F[fidx(g, fkind, j, w, fk)];
+        for (int w = 0; w < IW; w++) Lc[w] = fr[w * fs];
         double2 y = rh[c];
         y.x -= Lc[0] * yw[c][0].x;
         y.y -= Lc[0] * yw[c][0].y;
@@ -483,7 +514,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         y.y -= Lc[1] * yw[c][1].y;
         y.x -= Lc[2] * yw[c][2].x;
         y.y -= Lc[2] * yw[c][2].y;
-        A.SR[c * n + h] = y;
+        A.SR[h + c * 128] = y;
         yw[c][0] = yw[c][1];
         yw[c][1] = yw[c][2];
         yw[c][2] = y;
@@ -504,14 +535,15 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       }
     double2 xs[4][3];   // wall rows NR-1..NR-4
     double xhs[4][3];
-    const int fk[3] = {k, k_um, k_abs}, fkind[3] = {0, kind_um, 2};
     auto backsub = [&](int i, double2* xo, double* ho) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const double ru = A.F[fidx(g, fkind[c], i, 3, fk[c])];
-        const double u1 = A.F[fidx(g, fkind[c], i, 4, fk[c])], u2 = A.F[fidx(g, fkind[c], i, 5, fk[c])],
-                     u3 = A.F[fidx(g, fkind[c], i, 6, fk[c])];
-        double2 x = A.SR[c * n + (size_t)i * MS + k];
+        const double* fr = c == 0 ? A.F + fb_up + (size_t)i * FR_UP
+                                  : (c == 1 ? A.F + fb_um + (size_t)i * FR_UP : A.F + fb_uz + (size_t)i * FR_ABS);
+        const int fs = c == 2 ? 64 : 128;
+        const double ru = fr[3 * fs];
+        const double u1 = fr[4 * fs], u2 = fr[5 * fs], u3 = fr[6 * fs];
+        double2 x = A.SR[sb3 + (size_t)i * 384 + c * 128];
         x.x -= u1 * xw[c][0].x;
         x.y -= u1 * xw[c][0].y;
         x.x -= u2 * xw[c][1].x;
@@ -575,7 +607,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     auto emit = [&](int i, const double2* xo, const double* ho) {
       const size_t h = (size_t)i * MS + k;
       // pressure-gradient homogeneous solution: (dr p_h -+ kt p_h / r, 0), (0, kz p_h)
-      const double dph = A.PH[(size_t)(2 * i) * MS + k_abs], ph = A.PH[(size_t)(2 * i + 1) * MS + k_abs];
+      const double dph = A.PH[phb + (size_t)i * 128], ph = A.PH[phb + (size_t)i * 128 + 64];
       const double tq = q.kt * ph / A.r[i];
       const double2 hp[3] = {make_double2(dph - tq, 0.0), make_double2(dph + tq, 0.0), make_double2(-0.0, q.kz * ph)};
       const double2 hv[3] = {make_double2(ho[0], 0.0), make_double2(ho[1], 0.0), make_double2(0.0, ho[2])};
@@ -643,7 +675,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
   }
 }
 
-size_t factor_count(const Geom& g) { return fidx(g, 3, g.nr - 1, 6, (g.nt / 2 + 1) * g.nz - 1) + 1; }
+size_t factor_count(const Geom& g) { return f_abs(g) + 2 * f_absn(g); }
 
 void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, hipStream_t s) {
   dim3 grid((g.modes + 255) / 256, 4);

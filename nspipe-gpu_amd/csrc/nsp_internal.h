@@ -77,6 +77,32 @@ __device__ __forceinline__ T* gmem(const void* base, T* p) {
   return (T*)((char*)const_cast<void*>(base) + off);
 }
 
+// Sweep-side layout.  The buffers only the linear sweep reads and writes
+// (column scratch SR/SP, explicit part E, LU factors F, homogeneous pressure
+// data PH) are stored per sweep workgroup, so that one workgroup's radial
+// column is a contiguous run of memory instead of NR lines spread over the
+// whole field.  Workgroup lb = pair * nch + ch covers the axial chunk ch (64
+// modes, nch = ceil(NZ/64)) of the azimuthal rows it = pair (wave 0) and
+// NT - pair (wave 1; pair 0: the Nyquist row NT/2).  A workgroup has 128
+// lanes, lane = 64 * wave + l % 64.
+__host__ __device__ __forceinline__ int sw_nch(const Geom& g) { return (g.nz + 63) / 64; }
+__host__ __device__ __forceinline__ size_t sw_nblk(const Geom& g) { return (size_t)(g.nt / 2) * sw_nch(g); }
+// values of one component of a blocked field (>= NR*NT*NZ)
+__host__ __device__ __forceinline__ size_t sw_field(const Geom& g) { return sw_nblk(g) * g.nr * 128; }
+// Quantities that depend on |m| (uz and pressure factors, PH) are stored for
+// the rows it <= NT/2 only: rows it < NT/2 in their workgroup's wave-0 lanes
+// (64 per row), the Nyquist row in a region of its own after all workgroups.
+// Index of row i, value c (of nc per row) of the mode k = it * NZ + l, it <= NT/2.
+__host__ __device__ __forceinline__ size_t sw_abs_idx(const Geom& g, int nc, int i, int c, int k) {
+  const int it = k / g.nz, l = k - it * g.nz, ch = l >> 6, lane = l & 63;
+  const size_t blk = it < g.nt / 2 ? (size_t)it * sw_nch(g) + ch : sw_nblk(g) + ch;
+  return ((blk * g.nr + i) * nc + c) * 64 + lane;
+}
+// doubles of PH: [workgroup or Nyquist chunk][NR][2][64]
+__host__ __device__ __forceinline__ size_t sw_ph_count(const Geom& g) {
+  return (sw_nblk(g) + sw_nch(g)) * g.nr * 2 * 64;
+}
+
 // Device tables of one pipe.
 struct DevTables {
   double* r;     // [NR]
@@ -137,16 +163,16 @@ struct Lin2Args {
   const double* Luc;    // [2][NR][7] same for Lu
   const double* Lu;     // [2][NR][7] operator tables (factor recurrence on the fly)
   const double* Lp;     // [2][NR][7]
-  const double* F;      // banded LU factors (layout: fidx in linear2.hip)
-  const double* PH;     // [NR][2][MODES] dr p_h, p_h of the homogeneous pressure solution
+  const double* F;      // banded LU factors (sweep-side layout: fidx in linear2.hip)
+  const double* PH;     // dr p_h, p_h of the homogeneous pressure solution (sw_abs_idx, 2 per row)
   const double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
   const double2* N;     // [3] nonlinear term (r/theta); float2 storage when n32
   int n32;              // mixed form: N(u) stored as FP32 pairs
-  double2* E;           // [3] explicit part (+/-): written by the predictor
+  double2* E;           // [3] explicit part (+/-), written by the predictor (sweep-side layout)
   const double2* u;     // [3] predictor: u^n; corrector: u^k (r/theta)
   double2* out;         // [3] new iterate (r/theta)
-  double2* SR;          // [3] scratch
-  double2* SP;          // [1] scratch
+  double2* SR;          // [3] scratch (sweep-side layout)
+  double2* SP;          // [1] scratch (sweep-side layout)
   unsigned long long* norm;
   int predictor;
   int ord;              // block order (NlVariant::s_ord)

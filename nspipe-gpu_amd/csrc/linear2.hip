@@ -31,6 +31,10 @@
 
 namespace nsp {
 
+// read-only tables in the constant address space: loads at wave-uniform
+// addresses become scalar-cache loads
+typedef const double __attribute__((address_space(4)))* ctab;
+
 struct ModeQ {
   int k, it, l, m;
   double kt, kz;
@@ -283,10 +287,14 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
   const size_t n = g.n;
   const int MS = g.modes;
   const double d_im = g.d_im;
-  const int par_u = abs((abs(q.m) % 2) - 1);  // parity table of ur, ut (u+, u-)
-  const int par_p = abs(q.m) % 2;             // parity table of uz, p
-  const double* Dcu = A.Dc + par_u * STEN * nr;
-  const double* Dcp = A.Dc + par_p * STEN * nr;
+  // Radial tables are read through the scalar cache: the parity of |m| is
+  // the same for every lane of a wave (rows pair and NT - pair, or 0 and
+  // NT/2), and the row index of a pass is uniform.
+  const int par_u = __builtin_amdgcn_readfirstlane(abs((abs(q.m) % 2) - 1));  // parity table of ur, ut (u+, u-)
+  const int par_p = 1 - par_u;                                                 // parity table of uz, p
+  const ctab Dcu = ctab(A.Dc) + par_u * STEN * nr;
+  const ctab Dcp = ctab(A.Dc) + par_p * STEN * nr;
+  const ctab rt = ctab(A.r);
   const double2 z2 = make_double2(0.0, 0.0);
 
   // ============================== pass A (up)
@@ -348,11 +356,11 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         double2 rh[3];
         if (PRED) {
           // addLap_k (mod_linear.cu:12-51) on u^n, then rhs = N + that
-          const double rj = A.r[j], r2 = rj * rj;
+          const double rj = rt[j], r2 = rj * rj;
           const double2 nn[3] = {n0, n1, n2};
 #pragma unroll
           for (int c = 0; c < 3; c++) {
-            const double* Lc = A.Luc + ((c < 2) ? par_u : par_p) * STEN * nr + STEN * j;
+            const ctab Lc = ctab(A.Luc) + ((c < 2) ? par_u : par_p) * STEN * nr + STEN * j;
             double px = 0.0, py = 0.0;
 #pragma unroll
             for (int w = 0; w < 7; w++) {
@@ -404,14 +412,14 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       if (i >= 0 && i < nr) {
         double2 dv = z2;
         if (i < nr - 1) {  // div_k (mod_radialFD.cu:67-96), wall row zero
-          const double* Dr = Dcu + STEN * i;
+          const ctab Dr = Dcu + STEN * i;
           double px = 0.0, py = 0.0;
 #pragma unroll
           for (int w = 0; w < 7; w++) {
             px += Dr[w] * Rr[w].x;
             py += Dr[w] * Rr[w].y;
           }
-          const double rK = A.r[i];
+          const double rK = rt[i];
           const double2 ur = Rr[3], ut = Rt[0], uz = Rz[0];
           px += ur.x / rK;
           py += ur.y / rK;
@@ -480,14 +488,14 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       if (j < 0) continue;
       // addGrad_k (mod_radialFD.cu:115-143)
       const size_t h = sb3 + (size_t)j * 384;
-      const double* Dr = Dcp + STEN * j;
+      const ctab Dr = Dcp + STEN * j;
       double px = 0.0, py = 0.0;
 #pragma unroll
       for (int w = 0; w < 7; w++) {
         px += Dr[w] * pw[w].x;
         py += Dr[w] * pw[w].y;
       }
-      const double rK = A.r[j];
+      const double rK = rt[j];
       const double2 ph = pw[3];
       double2 rh[3] = {A.SR[h], A.SR[h + 128], A.SR[h + 256]};
       rh[0].x -= px;
@@ -579,7 +587,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     }
     double2 dv;
     {
-      const double* Dr = Dcu + STEN * (nr - 1);
+      const ctab Dr = Dcu + STEN * (nr - 1);
       double px = 0.0, py = 0.0;
 #pragma unroll
       for (int w = 0; w < 4; w++) {  // columns NR-4..NR-1 = urw[3..0]
@@ -608,7 +616,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       const size_t h = (size_t)i * MS + k;
       // pressure-gradient homogeneous solution: (dr p_h -+ kt p_h / r, 0), (0, kz p_h)
       const double dph = A.PH[phb + (size_t)i * 128], ph = A.PH[phb + (size_t)i * 128 + 64];
-      const double tq = q.kt * ph / A.r[i];
+      const double tq = q.kt * ph / rt[i];
       const double2 hp[3] = {make_double2(dph - tq, 0.0), make_double2(dph + tq, 0.0), make_double2(-0.0, q.kz * ph)};
       const double2 hv[3] = {make_double2(ho[0], 0.0), make_double2(ho[1], 0.0), make_double2(0.0, ho[2])};
       double2 o[3] = {xo[0], xo[1], xo[2]};

@@ -31,10 +31,6 @@
 
 namespace nsp {
 
-// read-only tables in the constant address space: loads at wave-uniform
-// addresses become scalar-cache loads
-typedef const double __attribute__((address_space(4)))* ctab;
-
 struct ModeQ {
   int k, it, l, m;
   double kt, kz;

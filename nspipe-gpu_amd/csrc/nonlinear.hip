@@ -53,40 +53,57 @@ __global__ __launch_bounds__(256) void dr3_kernel(Geom g, const double* __restri
   if (k >= g.modes) return;
   const int nr = g.nr;
   const int m = m_of(k / g.nz, g.nt);
-  const double* Dp = D + abs((abs(m) % 2) - (c < 2 ? 1 : 0)) * STEN * nr;
+  // Parity table of this mode, read through the scalar cache.  A wave whose
+  // 64 modes span two azimuthal rows (of opposite parity) reads both tables
+  // and selects per lane; every other wave reads one table at uniform addresses.
+  const bool odd = abs((abs(m) % 2) - (c < 2 ? 1 : 0)) != 0;
+  const unsigned long long odd_lanes = __ballot(odd), lanes = __ballot(1);
+  const ctab D0 = ctab(D), D1 = ctab(D) + STEN * nr;
   const double2* src = u + (size_t)c * g.n + k;
   const size_t base = (size_t)c * g.n + k;
   const size_t ld = g.modes;
-  double2 w[NB];  // level L in slot L % NB
+  auto walk = [&](auto split_tag) {
+    constexpr bool SPLIT = decltype(split_tag)::value;
+    const ctab Dw = odd_lanes != 0 ? D1 : D0;  // uniform table (not SPLIT)
+    double2 w[NB];  // level L in slot L % NB
 #pragma unroll
-  for (int L = 0; L < IW + PF; L++) w[L] = L < nr ? src[L * ld] : make_double2(0.0, 0.0);
-  for (int ib = 0; ib < nr; ib += NB) {
+    for (int L = 0; L < IW + PF; L++) w[L] = L < nr ? src[L * ld] : make_double2(0.0, 0.0);
+    for (int ib = 0; ib < nr; ib += NB) {
 #pragma unroll
-    for (int s = 0; s < NB; s++) {
-      const int i = ib + s;
-      if (i < nr) {
-        const int L = i + IW + PF;
-        if (L < nr) w[(s + IW + PF) % NB] = src[L * ld];
-        const int left = max(0, i - IW), right = min(i + IW, nr - 1);
-        const double* Di = Dp + STEN * i + (i + IW - right) - left;  // Di[lev] = D[ip0 + 7i + j + i0]
-        double px = 0.0, py = 0.0;
+      for (int s = 0; s < NB; s++) {
+        const int i = ib + s;
+        if (i < nr) {
+          const int L = i + IW + PF;
+          if (L < nr) w[(s + IW + PF) % NB] = src[L * ld];
+          const int left = max(0, i - IW), right = min(i + IW, nr - 1);
+          const int oi = STEN * i + (i + IW - right) - left;  // D[ip0 + oi + lev] = D[ip0 + 7i + j + i0]
+          double px = 0.0, py = 0.0;
 #pragma unroll
-        for (int q = 0; q < STEN; q++) {
-          const int lev = i - IW + q;
-          if (lev >= 0 && lev < nr) {
-            const double de = Di[lev];
-            const double2 a = w[(s - IW + q + NB) % NB];
-            px += de * a.x;
-            py += de * a.y;
+          for (int q = 0; q < STEN; q++) {
+            const int lev = i - IW + q;
+            if (lev >= 0 && lev < nr) {
+              double de;
+              if constexpr (SPLIT) {
+                const double d0 = D0[oi + lev], d1 = D1[oi + lev];
+                de = odd ? d1 : d0;
+              } else {
+                de = Dw[oi + lev];
+              }
+              const double2 a = w[(s - IW + q + NB) % NB];
+              px += de * a.x;
+              py += de * a.y;
+            }
           }
+          if (f32)  // mixed form: d_r u only feeds the FP32 theta transforms
+            reinterpret_cast<float2*>(dRv)[base + i * ld] = make_float2(float(px), float(py));
+          else
+            reinterpret_cast<double2*>(dRv)[base + i * ld] = make_double2(px, py);
         }
-        if (f32)  // mixed form: d_r u only feeds the FP32 theta transforms
-          reinterpret_cast<float2*>(dRv)[base + i * ld] = make_float2(float(px), float(py));
-        else
-          reinterpret_cast<double2*>(dRv)[base + i * ld] = make_double2(px, py);
       }
     }
-  }
+  };
+  if (odd_lanes != 0 && odd_lanes != lanes) walk(std::true_type{});
+  else walk(std::false_type{});
 }
 
 // ------------------------------------------------------------------ theta inverse

@@ -103,6 +103,10 @@ __host__ __device__ __forceinline__ size_t sw_ph_count(const Geom& g) {
   return (sw_nblk(g) + sw_nch(g)) * g.nr * 2 * 64;
 }
 
+// Read-only tables seen through the constant address space: loads at
+// wave-uniform addresses become scalar-cache loads instead of vector loads.
+typedef const double __attribute__((address_space(4)))* ctab;
+
 // Device tables of one pipe.
 struct DevTables {
   double* r;     // [NR]

@@ -119,20 +119,35 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
     const int e = tid + q * NTHR;
     zpos[q] = e < (NTP - NT) * TL ? LY::pos(PL::digitrev(half + e / TL) * TL + e % TL) : -1;
   }
+  // A thread's elements share its column tl = tid % TL (NTHR is a multiple
+  // of TL) and step by NTHR / TL rows: one column test, affine offsets, and
+  // the loads of a component issued together without per-element branches.
+  static_assert(NTHR % TL == 0, "a thread keeps one column");
+  constexpr int FULL = (NT * TL) / NTHR;  // elements every thread has
+  const int tcol = tid % TL;
+  const bool lok = l0 + tcol < nz;
+  const size_t o0 = (size_t)i * g.modes + (size_t)(tid / TL) * nz + (lok ? l0 + tcol : 0);
+  const size_t ostep = (size_t)(NTHR / TL) * nz;
   for (int c = 0; c < 3; c++) {
     const double2* src = u + c * g.n;
     double2 U[EPT];
     cpx<T> DR[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
-      const int e = tid + q * NTHR;
-      const int it = e / TL, tl = e - it * TL, l = l0 + tl;
-      U[q] = make_double2(0.0, 0.0);
-      DR[q] = cpx<T>{T(0), T(0)};
-      if (e < NT * TL && l < nz) {
-        const size_t off = (size_t)i * g.modes + (size_t)it * nz + l;
-        U[q] = src[off];
-        DR[q] = dR[c * g.n + off];
+      const bool in = q < FULL || tid + q * NTHR < NT * TL;
+      const size_t off = in ? o0 + q * ostep : o0;
+      U[q] = src[off];
+      DR[q] = dR[c * g.n + off];
+      if (q >= FULL && !in) {
+        U[q] = make_double2(0.0, 0.0);
+        DR[q] = cpx<T>{T(0), T(0)};
+      }
+    }
+    if (!lok) {  // columns past NZ (last tile only)
+#pragma unroll
+      for (int q = 0; q < EPT; q++) {
+        U[q] = make_double2(0.0, 0.0);
+        DR[q] = cpx<T>{T(0), T(0)};
       }
     }
 #pragma unroll 1
@@ -144,7 +159,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
 #pragma unroll
       for (int q = 0; q < EPT; q++) {
         const int e = lt + q * NTHR;
-        if (e < NT * TL) {
+        if (q < FULL || e < NT * TL) {
           const int it = e / TL;
           const double2 v = U[q];
           cpx<T> w = (kind == 1) ? DR[q] : cpx<T>{T(v.x), T(v.y)};
@@ -157,10 +172,13 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
       }
       __syncthreads();
       fft_run<T, TL, NTHR, +1, LY>(PL{}, buf, tw, lt);
-      cpx<T>* dst = W + (kind * 3 + c) * ws + (size_t)i * NTP * nz;
-      for (int e = lt; e < NTP * TL; e += NTHR) {
-        const int j = e / TL, tl = e - j * TL, l = l0 + tl;
-        if (l < nz) dst[(size_t)j * nz + l] = buf[LY::pos(e)];
+      // rows lt / TL + q NTHR / TL of column tcol: one pointer step per store
+      constexpr int WFULL = (NTP * TL) / NTHR, WREM = (NTP * TL) % NTHR;
+      if (lok) {
+        cpx<T>* d = W + (kind * 3 + c) * ws + (size_t)i * NTP * nz + (size_t)(lt / TL) * nz + l0 + tcol;
+#pragma unroll
+        for (int q = 0; q < WFULL; q++) d[q * ostep] = buf[LY::pos(lt + q * NTHR)];
+        if (WREM && lt < WREM) d[WFULL * ostep] = buf[LY::pos(lt + WFULL * NTHR)];
       }
       __syncthreads();
     }

@@ -708,20 +708,34 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
   const int tid = threadIdx.x;
   const int nz = g.nz;
   const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
+  // the FP32 form scales by the reciprocal (its result is rounded to FP32),
+  // the FP64 form divides as the reference does
+  const double inorm = 1.0 / norm;
+  auto scale = [&](T x) -> double {
+    if constexpr (sizeof(T) == 4) return double(x) * inorm;
+    else return x / norm;
+  };
+  // A thread's first-pass pairs p = tid + s NTHR keep one column (NTHR is a
+  // multiple of TL): one column test, and the loads of a pair issued
+  // together with clamped offsets instead of per-element branches.
+  static_assert(NTHR % TL == 0, "a thread keeps one column");
+  const int tcol = tid % TL;
+  const bool lok = l0 + tcol < nz - 1;  // l = NZ-1 and columns past NZ enter as zero
 #pragma unroll 1
   for (int c = 0; c < 3; c++) {
     const int lt = opaque(tid);
-    const cpx<T>* src = Y + c * ws + (size_t)i * NTP * nz;
+    const cpx<T>* src = Y + c * ws + (size_t)i * NTP * nz + (lok ? l0 + tcol : 0);
 #pragma unroll
     for (int s = 0; s < PPT; s++) {
       const int p = lt + s * NTHR;
       if ((PAIRS % NTHR) == 0 || p < PAIRS) {
-        const int col = p % TL, n0 = p / TL, l = l0 + col;
+        const int col = p % TL, n0 = p / TL;
         cpx<T> v[R0];
 #pragma unroll
-        for (int q = 0; q < R0; q++) {
-          v[q] = cpx<T>{T(0), T(0)};
-          if (l < nz - 1) v[q] = src[(size_t)(n0 + NB1 * q) * nz + l];
+        for (int q = 0; q < R0; q++) v[q] = src[(size_t)(n0 + NB1 * q) * nz];
+        if (!lok) {
+#pragma unroll
+          for (int q = 0; q < R0; q++) v[q] = cpx<T>{T(0), T(0)};
         }
         first_pass_store<T, -1, LY, TL>(PL{}, v, buf, n0, col);
       }
@@ -733,7 +747,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
       const int l = l0 + col;
       if (l < nz && (j < half || j >= NT)) {
         const int it = j < half ? j : j - half;
-        double vx = double(w.x) / norm, vy = double(w.y) / norm;
+        double vx = scale(w.x), vy = scale(w.y);
         if (it == half || l == nz - 1) vx = vy = 0.0;
         if (c == 2 && it == 0 && l == 0) {  // pressGrad_k (mod_radialFD.cu:100-111)
           double du = 0.0;
@@ -741,7 +755,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
             du += D[STEN * (g.nr - 1) + j2 + IW] * u[2 * g.n + (size_t)(g.nr - 1 - IW + j2) * g.modes].x;
           vx -= (2.0 / g.re) * du;
         }
-        dst[(size_t)it * nz + l] = cpx<T>{T(vx), T(vy)};
+        dst[(unsigned)it * (unsigned)nz + (unsigned)l] = cpx<T>{T(vx), T(vy)};  // < NT*NZ: 32-bit offset
       }
     });
     __syncthreads();

@@ -56,7 +56,11 @@ struct nsp_handle {
   bool graphs = true;
   StepRec* rec = nullptr;    // device
   StepRec* rec_h = nullptr;  // pinned host
+  // captured step from state slot a: predictor and the first corrector
+  // (gexec), the remaining correctors (gexec2, launched only when the first
+  // corrector has not converged)
   hipGraphExec_t gexec[3] = {};
+  hipGraphExec_t gexec2[3] = {};
   std::vector<void*> allocs;
   // diagnostics files: friction, qcross, ucross, meanpr
   FILE* io[4] = {};
@@ -79,7 +83,8 @@ nsp_status dalloc(nsp_handle* h, T** ptr, size_t count) {
 void drop_graphs(nsp_handle* h) {
   for (int a = 0; a < 3; a++) {
     if (h->gexec[a]) (void)hipGraphExecDestroy(h->gexec[a]);
-    h->gexec[a] = nullptr;
+    if (h->gexec2[a]) (void)hipGraphExecDestroy(h->gexec2[a]);
+    h->gexec[a] = h->gexec2[a] = nullptr;
   }
 }
 
@@ -704,10 +709,13 @@ nsp_status nsp_set_tuning(nsp_handle* h, const char* json) {
 namespace {
 
 // Captures one whole time step starting from the state in U[a]: the predictor
-// writes U[a+1], correctors alternate U[a+1] -> U[a+2] -> U[a+1] ...; every
+// writes U[a+1], correctors alternate U[a+1] -> U[a+2] -> U[a+1] ...  Every
 // corrector launch carries the converged flag and returns at once after the
-// loop has ended, so the graph always holds maxit corrector iterations and
-// the host only reads the step record at the end.
+// loop has ended.  The step is two graphs: the predictor with the first
+// corrector, then correctors 2..maxit, which the host launches only when the
+// step record says the first corrector did not end the loop (the common
+// single-corrector step then carries no idle corrector launches).  Each graph
+// ends by copying the step record to the host.
 nsp_status capture_step(nsp_handle* h, int a) {
   hipStream_t s = h->stream;
   const Geom& g = h->g;
@@ -716,29 +724,42 @@ nsp_status capture_step(nsp_handle* h, int a) {
   double2* ua = h->U[a];
   double2* ub = h->U[(a + 1) % 3];
   double2* uc = h->U[(a + 2) % 3];
-  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  launch_step_begin(&h->rec->st[0], &h->rec->err, s);
-  launch_nonlinear(g, h->t, h->nb, ua, h->Nl, s, nullptr);
-  launch_sweep2(lin_args(h, g, h->Nl, ua, ub, true), s);
-  for (int it = 1; it <= h->P.maxit; it++) {
+  auto corrector = [&](int it) {
     double2* src = (it % 2) ? ub : uc;
     double2* dst = (it % 2) ? uc : ub;
     launch_nonlinear(gk, h->t, h->nb, src, h->Nl, s, nullptr);
     launch_norm_reset(h->norm, gk.skip, s);
     launch_sweep2(lin_args(h, gk, h->Nl, src, dst, false), s);
     launch_converge(h->norm, &h->rec->st[0], &h->rec->err, h->P.tol, it, h->P.maxit, s);
-  }
-  (void)hipMemcpyAsync(h->rec_h, h->rec, sizeof(StepRec), hipMemcpyDeviceToHost, s);
-  hipGraph_t graph = nullptr;
-  const hipError_t e = hipStreamEndCapture(s, &graph);
-  if (e != hipSuccess || !graph) return fail(NSP_ERR_HIP, std::string("step capture: ") + hipGetErrorString(e));
-  const hipError_t e2 = hipGraphInstantiate(&h->gexec[a], graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (e2 != hipSuccess) {
+  };
+  auto finish = [&](hipGraphExec_t* out, const char* what) -> nsp_status {
+    (void)hipMemcpyAsync(h->rec_h, h->rec, sizeof(StepRec), hipMemcpyDeviceToHost, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(s, &graph);
+    if (e != hipSuccess || !graph) return fail(NSP_ERR_HIP, std::string(what) + " capture: " + hipGetErrorString(e));
+    const hipError_t e2 = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) {
+      *out = nullptr;
+      return fail(NSP_ERR_HIP, std::string(what) + " instantiate: " + hipGetErrorString(e2));
+    }
+    return NSP_OK;
+  };
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  launch_step_begin(&h->rec->st[0], &h->rec->err, s);
+  launch_nonlinear(g, h->t, h->nb, ua, h->Nl, s, nullptr);
+  launch_sweep2(lin_args(h, g, h->Nl, ua, ub, true), s);
+  if (h->P.maxit >= 1) corrector(1);
+  nsp_status st = finish(&h->gexec[a], "step graph");
+  if (st != NSP_OK || h->P.maxit < 2) return st;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int it = 2; it <= h->P.maxit; it++) corrector(it);
+  st = finish(&h->gexec2[a], "corrector graph");
+  if (st != NSP_OK) {
+    (void)hipGraphExecDestroy(h->gexec[a]);
     h->gexec[a] = nullptr;
-    return fail(NSP_ERR_HIP, std::string("step graph instantiate: ") + hipGetErrorString(e2));
   }
-  return NSP_OK;
+  return st;
 }
 
 void set_state_slot(nsp_handle* h, int loc) {
@@ -773,6 +794,10 @@ nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {
       }
       HIPCHK(hipGraphLaunch(h->gexec[a], s));
       HIPCHK(hipStreamSynchronize(s));
+      if (h->rec_h->st[0] == 0 && h->gexec2[a]) {  // the loop goes on past the first corrector
+        HIPCHK(hipGraphLaunch(h->gexec2[a], s));
+        HIPCHK(hipStreamSynchronize(s));
+      }
       ite = h->rec_h->st[1];
       flags = h->rec_h->st[2];
       err = h->rec_h->err;
@@ -1404,14 +1429,20 @@ struct nsp_ensemble {
   char* stage_h = nullptr;
   size_t stage_bytes = 0;
   StepRec* rec_h = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec = nullptr;   // tables, predictor, first corrector
+  hipGraphExec_t gexec2 = nullptr;  // correctors 2..maxit (when a member needs them)
 };
 
 namespace {
 
-void ens_free(nsp_ensemble* e) {
+void ens_drop_graphs(nsp_ensemble* e) {
   if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-  e->gexec = nullptr;
+  if (e->gexec2) (void)hipGraphExecDestroy(e->gexec2);
+  e->gexec = e->gexec2 = nullptr;
+}
+
+void ens_free(nsp_ensemble* e) {
+  ens_drop_graphs(e);
   if (e->nl) (void)hipFree(e->nl);
   if (e->rec) (void)hipFree(e->rec);
   if (e->norms) (void)hipFree(e->norms);
@@ -1511,36 +1542,50 @@ bool ens_nonlinear(nsp_ensemble* e, int role, hipStream_t s) {
   return ok;
 }
 
+// Two graphs, as for one pipe: the table refresh, predictor and first
+// corrector, then correctors 2..maxit, launched only when some member's loop
+// goes on; both end by copying the step records to the host.
 nsp_status ens_capture(nsp_ensemble* e) {
   const int M = (int)e->m.size();
   hipStream_t s = e->stream;
-  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-  (void)hipMemcpyAsync(e->nl, e->stage_h, e->stage_bytes, hipMemcpyHostToDevice, s);
-  launch_ens_begin(e->rec, M, s);
-  bool ok = ens_nonlinear(e, 0, s);
-  launch_sweep_batch(e->sw, M, 1, e->max_nt, e->max_nz, s);
-  for (int it = 1; it <= e->maxit; it++) {
+  bool ok = true;
+  auto corrector = [&](int it) {
     const int role = (it % 2) ? 1 : 2;
     ok &= ens_nonlinear(e, role, s);
     launch_ens_norm_reset(e->norms, e->rec, M, s);
     launch_sweep_batch(e->sw + role * M, M, 0, e->max_nt, e->max_nz, s);
     launch_ens_converge(e->norms, e->rec, M, e->tol, it, e->maxit, s);
-  }
-  (void)hipMemcpyAsync(e->rec_h, e->rec, M * sizeof(StepRec), hipMemcpyDeviceToHost, s);
-  hipGraph_t graph = nullptr;
-  const hipError_t err = hipStreamEndCapture(s, &graph);
-  if (err != hipSuccess || !graph) return fail(NSP_ERR_HIP, std::string("ensemble capture: ") + hipGetErrorString(err));
-  if (!ok) {
+  };
+  auto finish = [&](hipGraphExec_t* out) -> nsp_status {
+    (void)hipMemcpyAsync(e->rec_h, e->rec, M * sizeof(StepRec), hipMemcpyDeviceToHost, s);
+    hipGraph_t graph = nullptr;
+    const hipError_t err = hipStreamEndCapture(s, &graph);
+    if (err != hipSuccess || !graph) return fail(NSP_ERR_HIP, std::string("ensemble capture: ") + hipGetErrorString(err));
+    if (!ok) {
+      (void)hipGraphDestroy(graph);
+      return fail(NSP_ERR_INVALID, "an ensemble class names a kernel variant that is not compiled");
+    }
+    const hipError_t e2 = hipGraphInstantiate(out, graph, nullptr, nullptr, 0);
     (void)hipGraphDestroy(graph);
-    return fail(NSP_ERR_INVALID, "an ensemble class names a kernel variant that is not compiled");
-  }
-  const hipError_t e2 = hipGraphInstantiate(&e->gexec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  if (e2 != hipSuccess) {
-    e->gexec = nullptr;
-    return fail(NSP_ERR_HIP, std::string("ensemble graph instantiate: ") + hipGetErrorString(e2));
-  }
-  return NSP_OK;
+    if (e2 != hipSuccess) {
+      *out = nullptr;
+      return fail(NSP_ERR_HIP, std::string("ensemble graph instantiate: ") + hipGetErrorString(e2));
+    }
+    return NSP_OK;
+  };
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  (void)hipMemcpyAsync(e->nl, e->stage_h, e->stage_bytes, hipMemcpyHostToDevice, s);
+  launch_ens_begin(e->rec, M, s);
+  ok &= ens_nonlinear(e, 0, s);
+  launch_sweep_batch(e->sw, M, 1, e->max_nt, e->max_nz, s);
+  if (e->maxit >= 1) corrector(1);
+  nsp_status st = finish(&e->gexec);
+  if (st != NSP_OK || e->maxit < 2) return st;
+  HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+  for (int it = 2; it <= e->maxit; it++) corrector(it);
+  st = finish(&e->gexec2);
+  if (st != NSP_OK) ens_drop_graphs(e);
+  return st;
 }
 
 }  // namespace
@@ -1653,8 +1698,7 @@ nsp_status nsp_ensemble_autotune(nsp_ensemble* e, int reps) {
       drop_graphs(e->m[j]);
     }
   }
-  if (e->gexec) (void)hipGraphExecDestroy(e->gexec);
-  e->gexec = nullptr;
+  ens_drop_graphs(e);
   return NSP_OK;
 }
 
@@ -1672,6 +1716,12 @@ nsp_status nsp_ensemble_step(nsp_ensemble* e, int nsteps, nsp_step_info* info) {
     ens_stage(e);
     HIPCHK(hipGraphLaunch(e->gexec, e->stream));
     HIPCHK(hipStreamSynchronize(e->stream));
+    bool more = false;
+    for (int j = 0; j < M; j++) more |= e->rec_h[j].st[0] == 0;
+    if (more && e->gexec2) {  // some member's loop goes on past the first corrector
+      HIPCHK(hipGraphLaunch(e->gexec2, e->stream));
+      HIPCHK(hipStreamSynchronize(e->stream));
+    }
     for (int j = 0; j < M; j++) {
       const int mi = e->order[j];
       nsp_handle* h = e->m[mi];

@@ -63,6 +63,30 @@ def test_ensemble_members_converge_independently():
     ens.close()
 
 
+def test_ensemble_second_graph_matches_eager_pipes(monkeypatch):
+    """With a tolerance that takes several correctors the ensemble step runs
+    its second graph (correctors 2..maxit); every member still gives the bits,
+    counts and flags of the same pipe stepped launch by launch."""
+    ms = [dict(A, tol=1e-30, maxit=5), dict(A, tol=1e-30, maxit=5, precision=nsp.MIXED)]
+    ens = nsp.Ensemble(ms)
+    for j in range(len(ms)):
+        seed_member(ens[j], j)
+    n_e, err_e, fl_e = ens.step(2)
+    assert n_e.max() > 1
+    monkeypatch.setenv("NSP_NO_GRAPH", "1")
+    for j, kw in enumerate(ms):
+        p = nsp.Pipe(**kw)
+        seed_member(p, j)
+        n_p, err_p, fl_p = p.step(2)
+        assert np.array_equal(n_p, n_e[:, j]), (j, n_p, n_e[:, j])
+        assert np.array_equal(err_p, err_e[:, j]) and np.array_equal(fl_p, fl_e[:, j]), j
+        a, b = ens[j].get_state(), p.get_state()
+        for c in range(3):
+            assert np.array_equal(a[c], b[c]), (j, c)
+        p.close()
+    ens.close()
+
+
 def test_puff_ic_spectrum_is_a_gaussian_window():
     """Kind IC_PUFF: for m >= 1 the radial velocity of every axial mode l is the
     l = 0 coefficient times the spectrum of a Gaussian window of sigma 2.5R

@@ -353,6 +353,30 @@ def test_graph_and_eager_steps_agree(monkeypatch, precision):
     p2.close()
 
 
+@pytest.mark.parametrize("tol,maxit", [(1e-30, 6), (0.0, 4), (5e-8, 1)])
+def test_graph_steps_match_eager_any_corrector_count(monkeypatch, tol, maxit):
+    """The captured step is the predictor with the first corrector, then a
+    second graph of correctors 2..maxit launched only when the loop goes on.
+    Whatever the corrector count (several, maxit reached, maxit 1), it gives
+    the bits, counts and flags of the launch-by-launch step."""
+    print(f"[gpu] graph vs eager tol={tol} maxit={maxit}", flush=True)
+    p1 = nsp.Pipe(**CFG1, tol=tol, maxit=maxit)
+    monkeypatch.setenv("NSP_NO_GRAPH", "1")
+    p2 = nsp.Pipe(**CFG1, tol=tol, maxit=maxit)
+    for p in (p1, p2):
+        p.seeded_ic(SEED1, 4, 4, 0, 1e-3)
+    n1, e1, f1 = p1.step(3)
+    n2, e2, f2 = p2.step(3)
+    assert np.array_equal(n1, n2) and np.array_equal(f1, f2) and np.array_equal(e1, e2)
+    if maxit > 1:
+        assert n1.max() > 1  # the second graph ran
+    a, b = p1.get_state(), p2.get_state()
+    for c in range(3):
+        assert np.array_equal(a[c], b[c])
+    p1.close()
+    p2.close()
+
+
 def test_profile_counts_executed_launches():
     """Corrector iterations skipped inside a captured step are not counted."""
     p = nsp.Pipe(**CFG1)

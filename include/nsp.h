@@ -247,7 +247,7 @@ nsp_status nsp_set_tuning(nsp_handle* h, const char* json);
  * share their operator data (LU factors, influence data).  nsp_ensemble_step
  * advances every member nsteps time steps: each stage is one launch per class
  * of members with the same transform plans (one for all members in the radial
- * derivative and the solves), the whole step one hipGraph; members converge
+ * derivative and the solves), the step captured in hipGraphs; members converge
  * independently.  info, if not NULL, receives nsteps*count entries, member
  * fastest.  nsp_ensemble_member returns a member's handle for the per-pipe
  * calls (state, ICs, diagnostics, restart files); it is owned by the

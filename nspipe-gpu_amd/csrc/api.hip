@@ -772,7 +772,8 @@ void set_state_slot(nsp_handle* h, int loc) {
 }  // namespace
 
 // integrate() loop body (mod_integrate.cu:112-131).  Default: one hipGraph
-// launch per step (capture_step).  With per-kernel profiling on, or with
+// launch per step (capture_step), and a second one when the first corrector
+// does not end the corrector loop.  With per-kernel profiling on, or with
 // NSP_NO_GRAPH=1, the kernels are launched one by one with a host-side
 // convergence test after every corrector (HIP events recorded inside a
 // captured graph do not yield elapsed times, so timing needs eager launches).
@@ -1403,9 +1404,10 @@ nsp_status nsp_seeded_ic(nsp_handle* h, unsigned long long seed, int mmax, int l
 // derivative and the four sweep passes: one launch for all members), with
 // blockIdx.y selecting the member through device tables of per-member
 // arguments.  Members converge independently: each has its own converged flag
-// in the corrector launches.  The whole ensemble step is one hipGraph; the
-// tables (which state buffer plays which role) are refreshed from pinned host
-// memory by the graph's first node, so one graph serves every step.
+// in the corrector launches.  The ensemble step is captured like a pipe's
+// (two graphs, see ens_capture); the tables (which state buffer plays which
+// role) are refreshed from pinned host memory by the first graph's first
+// node, so the same graphs serve every step.
 struct nsp_ensemble {
   int device = 0;
   hipStream_t stream = nullptr;

@@ -1,14 +1,16 @@
-// Linear phase of the time step: one thread per Fourier mode (it, l) walks
-// the whole radial column, fusing what the reference launches as ~25
-// separate whole-field kernels per predictor/corrector
-// (mod_integrate.cu:164-228):
-//   rhs assembly (decoupleForward, addRHS_k, addLap_k), decoupleBackward,
-//   div_k + write_bcon, LUsolve_p_k, addGrad_k, decoupleForward, write_bcon,
-//   LUsolve_up/um/uz_k, corBCon (divWall_k, compcoef_k, addBCon_k),
-//   decoupleBackward, modeZZ_k, adj_flux (mode (0,0) only), and for the
-//   corrector the mcor_k / iden_max convergence measure as a device max.
-// Column vectors live in mode-fastest global arrays, so every access of a
-// wavefront is one contiguous 1 KiB (FP64 complex) line.
+// Linear-phase set-up and bookkeeping (the per-step solve itself is the
+// streaming sweep of linear2.hip):
+//   * setup_influence_kernel: the homogeneous wall-unit and pressure
+//     solutions and the inverse influence matrix (setBCon,
+//     mod_boundary.cu:205-255) and the flux profile (setCublasFlux,
+//     mod_cublasflux.cu:297-321), one thread per Fourier mode (it, l) walking
+//     its radial column with the reference's banded LU (LUsolve_*_k);
+//   * the step record and convergence-test kernels of a captured step
+//     (single pipe and ensemble), the mixed form's N(u) widening, additive
+//     perturbations;
+//   * diag_kernel: per-mode partial sums of the diagnostics.
+// Column vectors of the set-up live in mode-fastest global arrays, so every
+// access of a wavefront is one contiguous 1 KiB (FP64 complex) line.
 #include "nsp_internal.h"
 
 namespace nsp {

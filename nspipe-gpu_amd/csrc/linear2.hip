@@ -16,8 +16,8 @@
 //
 // The banded LU factors of the four operators depend only on (m, kz), dt and
 // Re, so they are computed once at setup with the reference's recurrence
-// (LUsolve_*_k, mod_linear.cu:55-646) and stored mode-fastest, 7 per row:
-// every access of a wavefront is one contiguous line.  (Recomputing the L
+// (LUsolve_*_k, mod_linear.cu:55-646) and stored per workgroup, 7 per row
+// (fidx): every access of a wavefront is one contiguous line.  (Recomputing the L
 // multipliers in the upward passes saves 3 of the 7 but needs the U window of
 // three operators in registers, which halves occupancy: slower.)
 // Workgroups pair the modes m and -m of one axial chunk: the u- factors at m

@@ -320,8 +320,9 @@ class _Member(Pipe):
 
 class Ensemble:
     """Independent pipes stepped together (nsp_ensemble_*): every stage is one
-    launch per class of members with the same transform plans, the whole step
-    one hipGraph.  `members` is a list of dicts of Pipe keyword arguments."""
+    launch per class of members with the same transform plans, the step
+    captured in hipGraphs.  `members` is a list of dicts of Pipe keyword
+    arguments."""
 
     def __init__(self, members, device=0):
         n = len(members)

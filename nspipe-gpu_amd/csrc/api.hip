@@ -335,7 +335,7 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   TRY(upload(h, &h->nb.tw_t, twiddles(g.ntp, g.ntp)));
   TRY(upload(h, &h->nb.tw_h, twiddles(g.nh, g.nh)));
   TRY(upload(h, &h->nb.tw_z, twiddles(g.nzf, g.nh + 1)));
-  h->nb.wstride = size_t(g.nr) * g.ntp * g.nz;
+  h->nb.wstride = size_t(g.nr) * tp_plane(g);  // tile-major planes (tp_plane)
   h->nb.use_ct = false;
   h->nb.mixed = p->precision == NSP_MIXED;
   h->nb.var = NlVariant{ThetaVar{0, 8, 256, 0, 1}, ThetaVar{0, 8, 256, 0, 1}, 1, 256, 0, 0, 1};

@@ -141,10 +141,10 @@ __global__ __launch_bounds__(256) void theta_inv_kernel(Geom g, FftPlan P, const
     }
     __syncthreads();
     fft_passes<double>(buf, P, TL, 1.0, tw, tid, nthr);
-    double2* dst = W + f * ws + (size_t)i * g.ntp * g.nz;
+    double2* dst = W + f * ws + (size_t)i * tp_plane(g);  // tile-major W
     for (int e = tid; e < g.ntp * TL; e += nthr) {
       const int j = e / TL, tl = e - j * TL, l = l0 + tl;
-      if (l < g.nz) dst[(size_t)j * g.nz + l] = make_double2(buf[e].x, buf[e].y);
+      if (l < g.nz) dst[(size_t)j * 64 + tp_col(g, l)] = make_double2(buf[e].x, buf[e].y);
     }
     __syncthreads();
   }
@@ -172,7 +172,8 @@ __global__ __launch_bounds__(256) void zpass_kernel(Geom g, FftPlan P, const int
   const int tid = threadIdx.x, nthr = blockDim.x;
   const int nh = g.nh, nz = g.nz, nzf = g.nzf;
   const double rad = r[i];
-  const size_t roff = (size_t)row * nz;
+  // row th of level i in the tile-major planes of W, Y: value k at roff + tp_col(k)
+  const size_t roff = (size_t)(row / g.ntp) * tp_plane(g) + (size_t)(row % g.ntp) * 64;
 
   // inverse real transform of spectral row X (entries l < nz, zero beyond);
   // dz: differentiate in z first (derivz_k, mod_deriv.cu:24-35)
@@ -180,8 +181,8 @@ __global__ __launch_bounds__(256) void zpass_kernel(Geom g, FftPlan P, const int
     for (int k = tid; k < nh; k += nthr) {
       const int kc = nh - k;
       C a{0.0, 0.0}, b{0.0, 0.0};
-      if (k < nz) { const double2 v = X[k]; a = C{v.x, v.y}; }
-      if (kc < nz) { const double2 v = X[kc]; b = C{v.x, v.y}; }
+      if (k < nz) { const double2 v = X[tp_col(g, k)]; a = C{v.x, v.y}; }
+      if (kc < nz) { const double2 v = X[tp_col(g, kc)]; b = C{v.x, v.y}; }
       if (k == 0) { a.y = 0.0; b.y = 0.0; }
       if (dz) {
         const double ka = g.kz0 * double(k), kb = g.kz0 * double(kc);
@@ -254,7 +255,7 @@ __global__ __launch_bounds__(256) void zpass_kernel(Geom g, FftPlan P, const int
         const C w = twz[k];
         X = C{E.x + (w.x * O.x - w.y * O.y), E.y + (w.x * O.y + w.y * O.x)};
       }
-      dst[k] = make_double2(X.x, X.y);
+      dst[tp_col(g, k)] = make_double2(X.x, X.y);
     }
     __syncthreads();
   }
@@ -276,11 +277,11 @@ __global__ __launch_bounds__(256) void theta_fwd_kernel(Geom g, FftPlan P, const
   const int half = g.nt / 2;
   const double norm = double(g.ntp) * double(g.nzf);  // mod_fft.cu:41
   for (int c = 0; c < 3; c++) {
-    const double2* src = Y + c * ws + (size_t)i * g.ntp * g.nz;
+    const double2* src = Y + c * ws + (size_t)i * tp_plane(g);  // tile-major Y
     for (int e = tid; e < g.ntp * TL; e += nthr) {
       const int j = e / TL, tl = e - j * TL, l = l0 + tl;
       C v{0.0, 0.0};
-      if (l < g.nz - 1) { const double2 a = src[(size_t)j * g.nz + l]; v = C{a.x, a.y}; }
+      if (l < g.nz - 1) { const double2 a = src[(size_t)j * 64 + tp_col(g, l)]; v = C{a.x, a.y}; }
       buf[perm[j] * TL + tl] = v;
     }
     __syncthreads();

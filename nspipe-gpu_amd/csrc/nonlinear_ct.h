@@ -85,6 +85,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
   }
   if (step_done(g)) return;
   constexpr int NTP = PL::L;
+  static_assert(64 % TL == 0, "a column tile lies in one 64-column tile of W, Y");
   constexpr int NT = 2 * NTP / 3;
   constexpr int EPT = (NT * TL + NTHR - 1) / NTHR;
   __shared__ cpx<T> buf[LY::size(NTP * TL)];
@@ -172,13 +173,15 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
       }
       __syncthreads();
       fft_run<T, TL, NTHR, +1, LY>(PL{}, buf, tw, lt);
-      // rows lt / TL + q NTHR / TL of column tcol: one pointer step per store
+      // rows lt / TL + q NTHR / TL of column tcol (tile-major W: rows of a
+      // tile are 64 values apart): one pointer step per store
       constexpr int WFULL = (NTP * TL) / NTHR, WREM = (NTP * TL) % NTHR;
+      constexpr int wstep = (NTHR / TL) * 64;
       if (lok) {
-        cpx<T>* d = W + (kind * 3 + c) * ws + (size_t)i * NTP * nz + (size_t)(lt / TL) * nz + l0 + tcol;
+        cpx<T>* d = W + (kind * 3 + c) * ws + (size_t)i * tp_plane(g) + (size_t)(lt / TL) * 64 + tp_col(g, l0 + tcol);
 #pragma unroll
-        for (int q = 0; q < WFULL; q++) d[q * ostep] = buf[LY::pos(lt + q * NTHR)];
-        if (WREM && lt < WREM) d[WFULL * ostep] = buf[LY::pos(lt + WFULL * NTHR)];
+        for (int q = 0; q < WFULL; q++) d[q * wstep] = buf[LY::pos(lt + q * NTHR)];
+        if (WREM && lt < WREM) d[WFULL * wstep] = buf[LY::pos(lt + WFULL * NTHR)];
       }
       __syncthreads();
     }
@@ -217,7 +220,8 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
     if constexpr (sizeof(T) == 4) return x * irad;
     else return x / rad;
   };
-  const size_t roff = (size_t)row * nz;
+  // row th of level i in the tile-major planes of W, Y: value k at roff + tp_col(k)
+  const size_t roff = (size_t)i * tp_plane(g) + (size_t)(row - i * g.ntp) * 64;
   const T kz0 = T(g.kz0);
 
   auto c2r = [&](const cpx<T>* X, bool dz) {
@@ -225,8 +229,8 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
     for (int k = lt; k < NH; k += NTHR) {
       const int kc = NH - k;
       cpx<T> a{T(0), T(0)}, b{T(0), T(0)};
-      if (k < nz) a = X[k];
-      if (kc < nz) b = X[kc];
+      if (k < nz) a = X[tp_col(g, k)];
+      if (kc < nz) b = X[tp_col(g, kc)];
       if (k == 0) { a.y = T(0); b.y = T(0); }
       if (dz) {  // derivz_k (mod_deriv.cu:24-35)
         const T ka = kz0 * T(k), kb = kz0 * T(kc);
@@ -305,7 +309,7 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
         const cpx<T> wo = cx_mul(twz[k], O);
         X = cpx<T>{E.x + wo.x, E.y + wo.y};
       }
-      dst[k] = X;
+      dst[tp_col(g, k)] = X;
     }
     __syncthreads();
   }
@@ -360,7 +364,14 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
     if constexpr (sizeof(T) == 4) return x * irad;
     else return x / rad;
   };
-  const size_t roff = (size_t)row * nz;
+  // row th of level i in the tile-major planes of W, Y: value k at roff + tp_col(k)
+  const int th = row - i * g.ntp;
+  const size_t roff = (size_t)i * tp_plane(g) + (size_t)th * 64;
+  // buffer range: from the row's first value to the end of its plane, past
+  // every stored value of the row; modes past NZ load from an offset beyond
+  // it, which the range check turns into zero
+  const int rbytes = (int)((tp_plane(g) - (size_t)th * 64) * sizeof(cpx<T>));
+  const int t64 = g.ntp * 64;  // values per 64-column tile
   const T kz0 = T(g.kz0);
   const cpx<T> zero{T(0), T(0)};
 
@@ -371,17 +382,32 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
     return W + f * ws + roff;
   };
   // X[k] and X[NH-k] of the owned samples, zero beyond the stored NZ modes:
-  // raw buffer loads over the NZ stored values of the row return zero past
-  // its end, so no clamp or select per element (every offset is below NH).
+  // raw buffer loads of the row, whose modes past NZ take an offset past the
+  // buffer range (OOB) and so load zero without touching memory.
+  constexpr int CS = (int)sizeof(cpx<T>);
+  constexpr int OOB = 0x7ffffff0;
   auto load = [&](const cpx<T>* X, cpx<T>* A, cpx<T>* B) {
     const int lt = opaque(tid);
     if (NB1 == NTHR || lt < NB1) {
-      const nsp_i4v rs = buf_rsrc(X, nz * (int)sizeof(cpx<T>));
+      const nsp_i4v rs = buf_rsrc(X, rbytes);
+      if constexpr (NB1 % 64 == 0 && NH % 64 == 0) {
+        // k = lt + NB1 q and NH - k step by whole tiles: affine offsets
+        const int ok0 = ((lt >> 6) * t64 + (lt & 63)) * CS;
+        const int oc0 = (((NH >> 6) - ((lt + 63) >> 6)) * t64 + ((-lt) & 63)) * CS;
+        const int dq = (NB1 / 64) * t64 * CS;
 #pragma unroll
-      for (int q = 0; q < R0; q++) {
-        const int k = lt + NB1 * q, kc = NH - k;
-        A[q] = buf_load<T>(rs, k * (int)sizeof(cpx<T>));
-        B[q] = buf_load<T>(rs, kc * (int)sizeof(cpx<T>));
+        for (int q = 0; q < R0; q++) {
+          const int k = lt + NB1 * q, kc = NH - k;
+          A[q] = buf_load<T>(rs, k < nz ? ok0 + q * dq : OOB);
+          B[q] = buf_load<T>(rs, kc < nz ? oc0 - q * dq : OOB);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < R0; q++) {
+          const int k = lt + NB1 * q, kc = NH - k;
+          A[q] = buf_load<T>(rs, k < nz ? (int)tp_col(g, k) * CS : OOB);
+          B[q] = buf_load<T>(rs, kc < nz ? (int)tp_col(g, kc) * CS : OOB);
+        }
       }
     }
   };
@@ -518,7 +544,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
         const cpx<T> wo = cx_mul(twz[k], O);
         X = cpx<T>{E.x + wo.x, E.y + wo.y};
       }
-      dst[k] = X;
+      dst[tp_col(g, k)] = X;
     }
     __syncthreads();
   }
@@ -542,6 +568,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
   }
   if (step_done(g)) return;
   constexpr int NTP = PL::L;
+  static_assert(64 % TL == 0, "a column tile lies in one 64-column tile of W, Y");
   __shared__ cpx<T> buf[LY::size(NTP * TL)];
   int i, l0;
   theta_block(g, TL, ord, i, l0);
@@ -552,11 +579,11 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
 #pragma unroll 1
   for (int c = 0; c < 3; c++) {
     const int lt = opaque(tid);
-    const cpx<T>* src = Y + c * ws + (size_t)i * NTP * nz;
+    const cpx<T>* src = Y + c * ws + (size_t)i * tp_plane(g);  // tile-major Y
     for (int e = lt; e < NTP * TL; e += NTHR) {
       const int j = e / TL, tl = e - j * TL, l = l0 + tl;
       cpx<T> v{T(0), T(0)};
-      if (l < nz - 1) v = src[(size_t)j * nz + l];
+      if (l < nz - 1) v = src[(size_t)j * 64 + tp_col(g, l)];
       buf[LY::pos(PL::digitrev(j) * TL + tl)] = v;
     }
     __syncthreads();
@@ -610,6 +637,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
   }
   if (step_done(g)) return;
   constexpr int NTP = PL::L;
+  static_assert(64 % TL == 0, "a column tile lies in one 64-column tile of W, Y");
   constexpr int NT = 2 * NTP / 3;
   constexpr int half = NT / 2;
   constexpr int R0 = PL::rad[0];
@@ -666,9 +694,10 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
         }
       }
       __syncthreads();
-      cpx<T>* dst = W + (kind * 3 + c) * ws + (size_t)i * NTP * nz + l0;
+      // tile-major W: the TL columns from l0 lie in one 64-column tile
+      cpx<T>* dst = W + (kind * 3 + c) * ws + (size_t)i * tp_plane(g) + tp_col(g, l0);
       fft_rest_emit<T, TL, NTHR, +1, LY>(PL{}, buf, tw, lt, [&](int col, int j, cpx<T> w) {
-        if (l0 + col < nz) dst[(size_t)j * nz + col] = w;
+        if (l0 + col < nz) dst[(size_t)j * 64 + col] = w;
       });
       __syncthreads();
     }
@@ -695,6 +724,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
   }
   if (step_done(g)) return;
   constexpr int NTP = PL::L;
+  static_assert(64 % TL == 0, "a column tile lies in one 64-column tile of W, Y");
   constexpr int NT = 2 * NTP / 3;
   constexpr int half = NT / 2;
   constexpr int R0 = PL::rad[0];
@@ -724,7 +754,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
 #pragma unroll 1
   for (int c = 0; c < 3; c++) {
     const int lt = opaque(tid);
-    const cpx<T>* src = Y + c * ws + (size_t)i * NTP * nz + (lok ? l0 + tcol : 0);
+    const cpx<T>* src = Y + c * ws + (size_t)i * tp_plane(g) + tp_col(g, lok ? l0 + tcol : 0);  // tile-major Y
 #pragma unroll
     for (int s = 0; s < PPT; s++) {
       const int p = lt + s * NTHR;
@@ -732,7 +762,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
         const int col = p % TL, n0 = p / TL;
         cpx<T> v[R0];
 #pragma unroll
-        for (int q = 0; q < R0; q++) v[q] = src[(size_t)(n0 + NB1 * q) * nz];
+        for (int q = 0; q < R0; q++) v[q] = src[(size_t)(n0 + NB1 * q) * 64];
         if (!lok) {
 #pragma unroll
           for (int q = 0; q < R0; q++) v[q] = cpx<T>{T(0), T(0)};

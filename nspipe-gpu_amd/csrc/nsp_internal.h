@@ -54,6 +54,19 @@ struct Geom {
 
 __device__ __forceinline__ bool step_done(const Geom& g) { return g.skip != nullptr && *g.skip != 0; }
 
+// Transform intermediates W (theta-inverse output) and Y (z-pass output): per
+// field and level a plane of NTP azimuthal rows by NZ axial columns, stored
+// tile-major: tiles of 64 columns, each NTP rows of 64 contiguous values.
+// A theta kernel's column tile then moves whole tile rows (512 bytes in FP32)
+// instead of 64-byte pieces of rows 12 KB apart, and a z-pass row is NZ/64
+// runs of 64 values.  Element (th, l) of a plane: th * 64 + tp_col(g, l).
+__host__ __device__ __forceinline__ size_t tp_plane(const Geom& g) {
+  return (size_t)g.ntp * (size_t)(((g.nz + 63) / 64) * 64);
+}
+__host__ __device__ __forceinline__ size_t tp_col(const Geom& g, int l) {
+  return (size_t)(l >> 6) * ((size_t)g.ntp * 64) + (size_t)(l & 63);
+}
+
 // Workgroups are dealt to the 8 XCDs round robin; remap so that each XCD
 // walks one contiguous range of logical blocks and neighbouring tiles and
 // radial levels meet in the same L2.

@@ -400,7 +400,8 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
     h->b.invc = share->b.invc;
   else
     TRY(dalloc(h, &h->b.invc, 4 * size_t(g.modes)));
-  TRY(dalloc(h, &h->nb.dR, nf));
+  // d_r u: three tile-major components (dr_cs), FP32 pairs in the mixed form
+  TRY(dalloc(h, &h->nb.dR, h->nb.mixed ? (3 * dr_cs(g) + 1) / 2 : 3 * dr_cs(g)));
   TRY(dalloc(h, &h->norm, 2));
   TRY(dalloc(h, &h->part, 4 * size_t(g.modes)));
   HTRY(hipHostMalloc((void**)&h->norm_h, 2 * sizeof(unsigned long long), hipHostMallocDefault));
@@ -1416,7 +1417,7 @@ struct nsp_ensemble {
   struct Cls {
     std::vector<int> idx;          // members (caller indices), table order
     int off = 0;                   // first table slot
-    int max_nr = 0, max_nz = 0, max_modes = 0;
+    int max_nr = 0, max_nz = 0, max_modes = 0, max_nt = 0;
   };
   std::vector<Cls> cls;
   std::vector<int> order;          // table slot -> member
@@ -1516,7 +1517,7 @@ void ens_stage(nsp_ensemble* e) {
 bool ens_nonlinear(nsp_ensemble* e, int role, hipStream_t s) {
   const int M = (int)e->m.size();
   NlMember* nl = e->nl + role * M;
-  Batch all{nl, M, e->max_nr, e->max_nz, e->max_modes};
+  Batch all{nl, M, e->max_nr, e->max_nz, e->max_modes, e->max_nt};
   nsp_handle* h0 = e->m[e->order[0]];
   launch_dr3(h0->g, nullptr, nullptr, nullptr, false, s, &all);
   bool ok = true;
@@ -1524,7 +1525,7 @@ bool ens_nonlinear(nsp_ensemble* e, int role, hipStream_t s) {
     nsp_handle* h = e->m[c.idx[0]];
     const NlBufs& nb = h->nb;
     const Geom& g = h->g;
-    Batch bt{nl + c.off, (int)c.idx.size(), c.max_nr, c.max_nz, c.max_modes};
+    Batch bt{nl + c.off, (int)c.idx.size(), c.max_nr, c.max_nz, c.max_modes, c.max_nt};
     if (nb.mixed) {
       ok &= ct_theta_inv<float>(nb.var, g, (const cpx<float>*)nb.ct_tw_t32, nullptr, nullptr, nullptr, nullptr, 0, s,
                                 &bt);
@@ -1637,6 +1638,7 @@ nsp_status nsp_ensemble_create(const nsp_params* params, int count, int device, 
     C.max_nr = std::max(C.max_nr, g.nr);
     C.max_nz = std::max(C.max_nz, g.nz);
     C.max_modes = std::max(C.max_modes, g.modes);
+    C.max_nt = std::max(C.max_nt, g.nt);
     e->max_nr = std::max(e->max_nr, g.nr);
     e->max_nt = std::max(e->max_nt, g.nt);
     e->max_nz = std::max(e->max_nz, g.nz);

@@ -48,62 +48,55 @@ __global__ __launch_bounds__(256) void dr3_kernel(Geom g, const double* __restri
   if (step_done(g)) return;
   constexpr int NB = 12, PF = 5;  // ring slots; levels i-3 .. i+3+PF live at step i
   static_assert(STEN + PF <= NB, "ring too small");
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  // One wave per (azimuthal row, chunk of 64 axial modes): its reads of u are
+  // 1 KB runs of one row, its writes whole 64-value rows of a tile of the
+  // tile-major d_r u, and the parity of m is the same on every lane.
+  const int nch = (g.nz + 63) >> 6;
+  const int gc = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+  const int it_k = gc / nch, l_k = (gc - it_k * nch) * 64 + (threadIdx.x & 63);
   const int c = blockIdx.y;
-  if (k >= g.modes) return;
+  if (it_k >= g.nt || l_k >= g.nz) return;
+  const int k = it_k * g.nz + l_k;
   const int nr = g.nr;
-  const int m = m_of(k / g.nz, g.nt);
-  // Parity table of this mode, read through the scalar cache.  A wave whose
-  // 64 modes span two azimuthal rows (of opposite parity) reads both tables
-  // and selects per lane; every other wave reads one table at uniform addresses.
-  const bool odd = abs((abs(m) % 2) - (c < 2 ? 1 : 0)) != 0;
-  const unsigned long long odd_lanes = __ballot(odd), lanes = __ballot(1);
-  const ctab D0 = ctab(D), D1 = ctab(D) + STEN * nr;
+  const int m = m_of(it_k, g.nt);
+  // parity table of the wave's row, read through the scalar cache
+  const int odd = __builtin_amdgcn_readfirstlane(abs((abs(m) % 2) - (c < 2 ? 1 : 0)));
+  const ctab Dw = ctab(D) + odd * STEN * nr;
   const double2* src = u + (size_t)c * g.n + k;
-  const size_t base = (size_t)c * g.n + k;
   const size_t ld = g.modes;
-  auto walk = [&](auto split_tag) {
-    constexpr bool SPLIT = decltype(split_tag)::value;
-    const ctab Dw = odd_lanes != 0 ? D1 : D0;  // uniform table (not SPLIT)
-    double2 w[NB];  // level L in slot L % NB
+  // output: tile-major d_r u (dr_col), levels dr_plane apart
+  const size_t dbase = (size_t)c * dr_cs(g) + (size_t)it_k * 64 + dr_col(g, l_k);
+  const size_t dld = dr_plane(g);
+  double2 w[NB];  // level L in slot L % NB
 #pragma unroll
-    for (int L = 0; L < IW + PF; L++) w[L] = L < nr ? src[L * ld] : make_double2(0.0, 0.0);
-    for (int ib = 0; ib < nr; ib += NB) {
+  for (int L = 0; L < IW + PF; L++) w[L] = L < nr ? src[L * ld] : make_double2(0.0, 0.0);
+  for (int ib = 0; ib < nr; ib += NB) {
 #pragma unroll
-      for (int s = 0; s < NB; s++) {
-        const int i = ib + s;
-        if (i < nr) {
-          const int L = i + IW + PF;
-          if (L < nr) w[(s + IW + PF) % NB] = src[L * ld];
-          const int left = max(0, i - IW), right = min(i + IW, nr - 1);
-          const int oi = STEN * i + (i + IW - right) - left;  // D[ip0 + oi + lev] = D[ip0 + 7i + j + i0]
-          double px = 0.0, py = 0.0;
+    for (int s = 0; s < NB; s++) {
+      const int i = ib + s;
+      if (i < nr) {
+        const int L = i + IW + PF;
+        if (L < nr) w[(s + IW + PF) % NB] = src[L * ld];
+        const int left = max(0, i - IW), right = min(i + IW, nr - 1);
+        const int oi = STEN * i + (i + IW - right) - left;  // D[ip0 + oi + lev] = D[ip0 + 7i + j + i0]
+        double px = 0.0, py = 0.0;
 #pragma unroll
-          for (int q = 0; q < STEN; q++) {
-            const int lev = i - IW + q;
-            if (lev >= 0 && lev < nr) {
-              double de;
-              if constexpr (SPLIT) {
-                const double d0 = D0[oi + lev], d1 = D1[oi + lev];
-                de = odd ? d1 : d0;
-              } else {
-                de = Dw[oi + lev];
-              }
-              const double2 a = w[(s - IW + q + NB) % NB];
-              px += de * a.x;
-              py += de * a.y;
-            }
+        for (int q = 0; q < STEN; q++) {
+          const int lev = i - IW + q;
+          if (lev >= 0 && lev < nr) {
+            const double de = Dw[oi + lev];
+            const double2 a = w[(s - IW + q + NB) % NB];
+            px += de * a.x;
+            py += de * a.y;
           }
-          if (f32)  // mixed form: d_r u only feeds the FP32 theta transforms
-            reinterpret_cast<float2*>(dRv)[base + i * ld] = make_float2(float(px), float(py));
-          else
-            reinterpret_cast<double2*>(dRv)[base + i * ld] = make_double2(px, py);
         }
+        if (f32)  // mixed form: d_r u only feeds the FP32 theta transforms
+          reinterpret_cast<float2*>(dRv)[dbase + i * dld] = make_float2(float(px), float(py));
+        else
+          reinterpret_cast<double2*>(dRv)[dbase + i * dld] = make_double2(px, py);
       }
     }
-  };
-  if (odd_lanes != 0 && odd_lanes != lanes) walk(std::true_type{});
-  else walk(std::false_type{});
+  }
 }
 
 // ------------------------------------------------------------------ theta inverse
@@ -123,13 +116,14 @@ __global__ __launch_bounds__(256) void theta_inv_kernel(Geom g, FftPlan P, const
   const int half = g.nt / 2;
   for (int f = 0; f < 9; f++) {
     const int c = f % 3, kind = f / 3;  // kind 0: u, 1: d_r u, 2: d_theta u / r
-    const double2* src = (kind == 1 ? dR : u) + c * g.n + (size_t)i * g.modes;
+    const double2* src = u + c * g.n + (size_t)i * g.modes;
+    const double2* srd = dR + c * dr_cs(g) + (size_t)i * dr_plane(g);  // tile-major d_r u
     for (int e = tid; e < g.ntp * TL; e += nthr) buf[e] = C{0.0, 0.0};
     __syncthreads();
     for (int e = tid; e < g.nt * TL; e += nthr) {
       const int it = e / TL, tl = e - it * TL, l = l0 + tl;
       if (l < g.nz) {
-        const double2 v = src[(size_t)it * g.nz + l];
+        const double2 v = kind == 1 ? srd[(size_t)it * 64 + dr_col(g, l)] : src[(size_t)it * g.nz + l];
         C w{v.x, v.y};
         if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
           const double kt = double(m_of(it, g.nt));
@@ -364,11 +358,14 @@ const char* kernel_name(int id) {
 
 void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, bool f32, hipStream_t s,
                 const Batch* bt) {
+  // four (row, 64-mode chunk) waves per workgroup
   if (bt) {
-    dim3 gd((bt->max_modes + 255) / 256, 3, bt->count);
+    const int chunks = bt->max_nt * ((bt->max_nz + 63) / 64);
+    dim3 gd((chunks + 3) / 4, 3, bt->count);
     dr3_kernel<1><<<gd, 256, 0, s>>>(g, D, u, dR, f32 ? 1 : 0, bt->tab);
   } else {
-    dim3 gd((g.modes + 255) / 256, 3);
+    const int chunks = g.nt * ((g.nz + 63) / 64);
+    dim3 gd((chunks + 3) / 4, 3);
     dr3_kernel<0><<<gd, 256, 0, s>>>(g, D, u, dR, f32 ? 1 : 0, nullptr);
   }
 }

@@ -129,6 +129,9 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
   const bool lok = l0 + tcol < nz;
   const size_t o0 = (size_t)i * g.modes + (size_t)(tid / TL) * nz + (lok ? l0 + tcol : 0);
   const size_t ostep = (size_t)(NTHR / TL) * nz;
+  // the same elements of the tile-major d_r u
+  const size_t d0 = (size_t)i * dr_plane(g) + (size_t)(tid / TL) * 64 + dr_col(g, lok ? l0 + tcol : 0);
+  constexpr int dstep = (NTHR / TL) * 64;
   for (int c = 0; c < 3; c++) {
     const double2* src = u + c * g.n;
     double2 U[EPT];
@@ -138,7 +141,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
       const bool in = q < FULL || tid + q * NTHR < NT * TL;
       const size_t off = in ? o0 + q * ostep : o0;
       U[q] = src[off];
-      DR[q] = dR[c * g.n + off];
+      DR[q] = dR[c * dr_cs(g) + (in ? d0 + q * dstep : d0)];
       if (q >= FULL && !in) {
         U[q] = make_double2(0.0, 0.0);
         DR[q] = cpx<T>{T(0), T(0)};
@@ -664,7 +667,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
     const int lt = opaque(tid);
     {
       const double2* src = u + c * g.n + (size_t)i * g.modes;
-      const cpx<T>* srd = dR + c * g.n + (size_t)i * g.modes;
+      const cpx<T>* srd = dR + c * dr_cs(g) + (size_t)i * dr_plane(g);  // tile-major d_r u
 #pragma unroll
       for (int s = 0; s < PPT; s++) {
         const int p = lt + s * NTHR;
@@ -678,7 +681,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
             v[q] = cpx<T>{T(0), T(0)};
             if (it >= 0 && l < nz) {
               if (kind == 1) {
-                v[q] = srd[(size_t)it * nz + l];
+                v[q] = srd[(size_t)it * 64 + dr_col(g, l)];
               } else {
                 const double2 a = src[(size_t)it * nz + l];
                 if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)

@@ -66,6 +66,16 @@ __host__ __device__ __forceinline__ size_t tp_plane(const Geom& g) {
 __host__ __device__ __forceinline__ size_t tp_col(const Geom& g, int l) {
   return (size_t)(l >> 6) * ((size_t)g.ntp * 64) + (size_t)(l & 63);
 }
+// d_r u (dR, written by dr3, read by the theta inverse) is tile-major the same
+// way, over planes of NT spectral rows: element (it, l) of a plane at
+// it * 64 + dr_col(g, l); components dr_cs(g) values apart.
+__host__ __device__ __forceinline__ size_t dr_plane(const Geom& g) {
+  return (size_t)g.nt * (size_t)(((g.nz + 63) / 64) * 64);
+}
+__host__ __device__ __forceinline__ size_t dr_cs(const Geom& g) { return (size_t)g.nr * dr_plane(g); }
+__host__ __device__ __forceinline__ size_t dr_col(const Geom& g, int l) {
+  return (size_t)(l >> 6) * ((size_t)g.nt * 64) + (size_t)(l & 63);
+}
 
 // Workgroups are dealt to the 8 XCDs round robin; remap so that each XCD
 // walks one contiguous range of logical blocks and neighbouring tiles and
@@ -166,7 +176,7 @@ struct NlMember {
 // extents among them (grid sizes).
 struct Batch {
   const NlMember* tab;
-  int count, max_nr, max_nz, max_modes;
+  int count, max_nr, max_nz, max_modes, max_nt;
 };
 
 // Streaming linear sweep (linear2.hip) argument block.

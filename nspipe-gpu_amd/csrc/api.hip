@@ -380,8 +380,8 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
     h->graphs = !(ng && ng[0] && ng[0] != '0');
   }
   // N(u) and d_r u are FP32 pairs in the mixed form
-  const size_t nf = h->nb.mixed ? (3 * n + 1) / 2 : 3 * n;
-  TRY(dalloc(h, &h->Nl, nf));
+  // N(u): three tile-major components (dr_cs), FP32 pairs in the mixed form
+  TRY(dalloc(h, &h->Nl, h->nb.mixed ? (3 * dr_cs(g) + 1) / 2 : 3 * dr_cs(g)));
   // sweep-side layout (E, SR, SP, F, PH): per-workgroup columns, padded to
   // whole chunks of 64 axial modes; SR also serves as 3N scratch elsewhere
   TRY(dalloc(h, &h->E, 3 * sw_field(g)));
@@ -493,12 +493,11 @@ nsp_status nsp_nonlinear(nsp_handle* h, const double* ur, const double* ut, cons
   if (st) return st;
   launch_nonlinear(h->g, h->t, h->nb, h->ui, h->Nl, h->stream, nullptr);
   HIPCHK(hipGetLastError());
-  if (h->nb.mixed) {  // N(u) is stored as FP32 pairs: widen into the sweep scratch
-    launch_widen(reinterpret_cast<const float2*>(h->Nl), h->SR, 3 * h->g.n, h->stream);
-    HIPCHK(hipGetLastError());
-    return fetch_fields(h, h->SR, dr, dt, dz, where);
-  }
-  return fetch_fields(h, h->Nl, dr, dt, dz, where);
+  // N(u) is stored tile-major (FP32 pairs in the mixed form): into the
+  // reference layout in the sweep scratch (3N capacity)
+  launch_untile(h->g, h->Nl, h->nb.mixed, h->SR, h->stream);
+  HIPCHK(hipGetLastError());
+  return fetch_fields(h, h->SR, dr, dt, dz, where);
 }
 
 nsp_status nsp_profile_enable(nsp_handle* h, int enable) {

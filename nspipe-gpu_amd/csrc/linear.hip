@@ -186,6 +186,25 @@ __global__ void widen_kernel(const float2* __restrict__ in, double2* __restrict_
   }
 }
 
+// Tile-major spectral field to the reference layout (nsp_nonlinear's N(u)):
+// one thread per value of the output.
+__global__ void untile_kernel(Geom g, const void* __restrict__ in, int f32, double2* __restrict__ out) {
+  const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (q >= 3 * g.n) return;
+  const int c = (int)(q / g.n);
+  const size_t h = q - (size_t)c * g.n;
+  const int i = (int)(h / g.modes);
+  const int k = (int)(h - (size_t)i * g.modes);
+  const int it = k / g.nz, l = k - it * g.nz;
+  const size_t s = (size_t)c * dr_cs(g) + (size_t)i * dr_plane(g) + (size_t)it * 64 + dr_col(g, l);
+  if (f32) {
+    const float2 v = reinterpret_cast<const float2*>(in)[s];
+    out[q] = make_double2(v.x, v.y);
+  } else {
+    out[q] = reinterpret_cast<const double2*>(in)[s];
+  }
+}
+
 // u += f over `count` complex values (additive seeded perturbations).
 __global__ void add_kernel(double2* __restrict__ u, const double2* __restrict__ f, size_t count) {
   const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -425,6 +444,9 @@ void launch_ens_norm_reset(unsigned long long* const* norms, const StepRec* rec,
 void launch_ens_converge(unsigned long long* const* norms, StepRec* rec, int count, double tol, int it, int maxit,
                          hipStream_t s) {
   ens_converge_kernel<<<(count + 63) / 64, 64, 0, s>>>(norms, rec, count, tol, it, maxit);
+}
+void launch_untile(const Geom& g, const void* in, bool f32, double2* out, hipStream_t s) {
+  untile_kernel<<<(unsigned)((3 * g.n + 255) / 256), 256, 0, s>>>(g, in, f32 ? 1 : 0, out);
 }
 void launch_widen(const float2* in, double2* out, size_t count, hipStream_t s) {
   widen_kernel<<<(unsigned)((count + 255) / 256), 256, 0, s>>>(in, out, count);

@@ -280,6 +280,9 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
   const size_t phb = ab * nr * 128 + lane;              // PH row i: phb + 128 i (+64: p_h)
   const size_t sb3 = (size_t)lb * nr * 384 + threadIdx.x;  // SR, E row i, component c: sb3 + 384 i + 128 c
   const size_t sb1 = (size_t)lb * nr * 128 + threadIdx.x;  // SP row i: sb1 + 128 i
+  // N(u) is tile-major (dr_col): this mode's value of level j at nb0 + j npl,
+  // components ncs apart; a wave's 64 modes are one tile row
+  const size_t nb0 = (size_t)it * 64 + dr_col(g, l), npl = dr_plane(g), ncs = dr_cs(g);
   const size_t n = g.n;
   const int MS = g.modes;
   const double d_im = g.d_im;
@@ -336,17 +339,18 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       double2 R0 = z2, R1 = z2, R2 = z2;
       if (j >= 0 && j < nr) {
         const size_t h = (size_t)j * MS + k;
+        const size_t hn = nb0 + (size_t)j * npl;  // tile-major N(u)
         double2 n0, n1, n2;
         if (A.n32) {
           const float2* N32 = reinterpret_cast<const float2*>(A.N);
-          const float2 f0 = N32[h], f1 = N32[n + h], f2 = N32[2 * n + h];
+          const float2 f0 = N32[hn], f1 = N32[ncs + hn], f2 = N32[2 * ncs + hn];
           n0 = make_double2(f0.x, f0.y);
           n1 = make_double2(f1.x, f1.y);
           n2 = make_double2(f2.x, f2.y);
         } else {
-          n0 = A.N[h];
-          n1 = A.N[n + h];
-          n2 = A.N[2 * n + h];
+          n0 = A.N[hn];
+          n1 = A.N[ncs + hn];
+          n2 = A.N[2 * ncs + hn];
         }
         pm_of(n0, n1);
         double2 rh[3];

@@ -294,7 +294,7 @@ __global__ __launch_bounds__(256) void theta_fwd_kernel(Geom g, FftPlan P, const
             du += D[STEN * (g.nr - 1) + j2 + IW] * u[2 * g.n + (size_t)(g.nr - 1 - IW + j2) * g.modes].x;
           v.x -= (2.0 / g.re) * du;
         }
-        N[c * g.n + ((size_t)i * g.nt + it) * g.nz + l] = make_double2(v.x, v.y);
+        N[c * dr_cs(g) + (size_t)i * dr_plane(g) + (size_t)it * 64 + dr_col(g, l)] = make_double2(v.x, v.y);  // tile-major N(u)
       }
     }
     __syncthreads();

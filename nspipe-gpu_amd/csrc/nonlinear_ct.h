@@ -605,7 +605,8 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
           vx -= (2.0 / g.re) * du;
         }
         // N(u) is stored at the transform precision (FP32 in the mixed form)
-        reinterpret_cast<cpx<T>*>(N)[c * g.n + ((size_t)i * nt + it) * nz + l] = cpx<T>{T(vx), T(vy)};
+        reinterpret_cast<cpx<T>*>(N)[c * dr_cs(g) + (size_t)i * dr_plane(g) + (size_t)it * 64 + dr_col(g, l)] =
+            cpx<T>{T(vx), T(vy)};  // tile-major N(u)
       }
     }
     __syncthreads();
@@ -775,7 +776,8 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
     }
     __syncthreads();
     // N(u) is stored at the transform precision (FP32 in the mixed form)
-    cpx<T>* dst = reinterpret_cast<cpx<T>*>(N) + c * g.n + (size_t)i * g.modes;
+    // tile-major N(u): the TL columns from l0 lie in one 64-column tile
+    cpx<T>* dst = reinterpret_cast<cpx<T>*>(N) + c * dr_cs(g) + (size_t)i * dr_plane(g) + dr_col(g, l0);
     fft_rest_emit<T, TL, NTHR, -1, LY>(PL{}, buf, tw, lt, [&](int col, int j, cpx<T> w) {
       const int l = l0 + col;
       if (l < nz && (j < half || j >= NT)) {
@@ -788,7 +790,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
             du += D[STEN * (g.nr - 1) + j2 + IW] * u[2 * g.n + (size_t)(g.nr - 1 - IW + j2) * g.modes].x;
           vx -= (2.0 / g.re) * du;
         }
-        dst[(unsigned)it * (unsigned)nz + (unsigned)l] = cpx<T>{T(vx), T(vy)};  // < NT*NZ: 32-bit offset
+        dst[(unsigned)it * 64u + (unsigned)col] = cpx<T>{T(vx), T(vy)};
       }
     });
     __syncthreads();

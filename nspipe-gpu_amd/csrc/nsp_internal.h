@@ -66,9 +66,11 @@ __host__ __device__ __forceinline__ size_t tp_plane(const Geom& g) {
 __host__ __device__ __forceinline__ size_t tp_col(const Geom& g, int l) {
   return (size_t)(l >> 6) * ((size_t)g.ntp * 64) + (size_t)(l & 63);
 }
-// d_r u (dR, written by dr3, read by the theta inverse) is tile-major the same
+// d_r u (dR, written by dr3, read by the theta inverse) and the stored N(u)
+// (written by the theta forward, read by the sweep) are tile-major the same
 // way, over planes of NT spectral rows: element (it, l) of a plane at
-// it * 64 + dr_col(g, l); components dr_cs(g) values apart.
+// it * 64 + dr_col(g, l); components dr_cs(g) values apart.  A sweep wave's
+// 64 modes (one azimuthal row, one 64-mode chunk) are one tile row.
 __host__ __device__ __forceinline__ size_t dr_plane(const Geom& g) {
   return (size_t)g.nt * (size_t)(((g.nz + 63) / 64) * 64);
 }
@@ -357,6 +359,9 @@ void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const
                       double2* N, hipStream_t s, Profiler* pf);
 void launch_norm_reset(unsigned long long* norm, const int* skip, hipStream_t s);
 void launch_widen(const float2* in, double2* out, size_t count, hipStream_t s);
+// tile-major spectral field (three components, dr_cs apart; FP32 pairs when
+// f32) to the reference layout [3][NR][NT][NZ] in FP64
+void launch_untile(const Geom& g, const void* in, bool f32, double2* out, hipStream_t s);
 void launch_add(double2* u, const double2* f, size_t count, hipStream_t s);
 // new state = remesh of the old (both on this device); RM and band [nrn] of
 // (first, last) nonzero columns in device memory

@@ -43,6 +43,7 @@ struct nsp_handle {
   double2* SR = nullptr;    // sweep scratch (3 fields), sweep-side layout
   double2* SP = nullptr;    // sweep scratch (1 field), sweep-side layout
   double* F = nullptr;      // banded LU factors (u+ all modes, u- row NT/2, uz and p rows <= NT/2)
+  float* F32 = nullptr;     // mixed form: FP32 copies of the u+, u-, uz sets (read for |m| >= kM32)
   double* PH = nullptr;     // homogeneous pressure data (rows it <= NT/2), sweep-side layout
   double* Dc = nullptr;     // centred D table
   double* Luc = nullptr;    // centred Lu table
@@ -162,6 +163,11 @@ nsp_status fetch_fields(nsp_handle* h, const double2* src, double* a, double* b,
   return NSP_OK;
 }
 
+// The mixed form's sweep reads the u+, u-, uz LU factors rounded to FP32 for
+// |m| >= kM32; the low-|m| sets (near-axis stiffness), the pressure factors
+// and all arithmetic stay FP64.
+constexpr int kM32 = 4;
+
 Lin2Args lin_args(nsp_handle* h, const Geom& g, const double2* N, const double2* u, double2* out, bool predictor) {
   Lin2Args a;
   a.g = g;
@@ -174,6 +180,8 @@ Lin2Args lin_args(nsp_handle* h, const Geom& g, const double2* N, const double2*
   a.Lu = h->t.Lu;
   a.Lp = h->t.Lp;
   a.F = h->F;
+  a.F32 = h->F32;
+  a.m32 = kM32;
   a.PH = h->PH;
   a.invc = h->b.invc;
   a.N = N;
@@ -389,9 +397,11 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   TRY(dalloc(h, &h->SP, sw_field(g)));
   if (share) {
     h->F = share->F;
+    h->F32 = h->nb.mixed ? share->F32 : nullptr;  // an FP64 member never reads FP32 factors
     h->PH = share->PH;
   } else {
     TRY(dalloc(h, &h->F, factor_count(g)));
+    if (h->nb.mixed) TRY(dalloc(h, &h->F32, factor32_count(g)));
     TRY(dalloc(h, &h->PH, sw_ph_count(g)));
   }
   TRY(upload(h, &h->Dc, centred_table(g.nr, h->host.D)));
@@ -426,7 +436,7 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
     h->b.uh = uh;
     h->b.uhP = uhP;
     launch_setup_influence(g, h->t, h->b, P, U, h->PH, h->stream);
-    launch_factor(g, h->t.r, h->t.Lu, h->t.Lp, h->F, h->stream);
+    launch_factor(g, h->t.r, h->t.Lu, h->t.Lp, h->F, h->F32, h->stream);
     hipError_t e5 = hipGetLastError();
     hipError_t e6 = hipStreamSynchronize(h->stream);
     (void)hipFree(uh); (void)hipFree(uhP); (void)hipFree(P); (void)hipFree(U);

@@ -196,7 +196,8 @@ __device__ __forceinline__ void shift_u(double (&Uw)[IW][IW + 1], const double (
 // ------------------------------------------------------------------ factors
 __global__ __launch_bounds__(256) void factor_kernel(Geom g, const double* __restrict__ r,
                                                      const double* __restrict__ Lu,
-                                                     const double* __restrict__ Lp, double* __restrict__ F) {
+                                                     const double* __restrict__ Lp, double* __restrict__ F,
+                                                     float* __restrict__ F32) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int kind = blockIdx.y;
   if (k >= g.modes) return;
@@ -212,11 +213,17 @@ __global__ __launch_bounds__(256) void factor_kernel(Geom g, const double* __res
   for (int i = 0; i < g.nr; i++) {
     double Lc[IW], Ui[IW + 1];
     factor_row(o, g.nr, i, r[i], Uw, Lc, Ui);
+    double fv[7];
 #pragma unroll
-    for (int c = 0; c < IW; c++) F[fidx(g, kind, i, c, k)] = Lc[c];
-    F[fidx(g, kind, i, 3, k)] = 1.0 / Ui[0];
+    for (int c = 0; c < IW; c++) fv[c] = Lc[c];
+    fv[3] = 1.0 / Ui[0];
 #pragma unroll
-    for (int j = 1; j <= IW; j++) F[fidx(g, kind, i, 3 + j, k)] = Ui[j];
+    for (int j = 1; j <= IW; j++) fv[3 + j] = Ui[j];
+#pragma unroll
+    for (int c = 0; c < 7; c++) {
+      F[fidx(g, kind, i, c, k)] = fv[c];
+      if (F32 && kind <= 2) F32[fidx(g, kind, i, c, k)] = float(fv[c]);
+    }
     shift_u(Uw, Ui);
   }
 }
@@ -241,6 +248,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     A.Lu = gmem(tab, A.Lu);
     A.Lp = gmem(tab, A.Lp);
     A.F = gmem(tab, A.F);
+    if (A.F32) A.F32 = gmem(tab, A.F32);  // nullable: tested before re-deriving
     A.PH = gmem(tab, A.PH);
     A.invc = gmem(tab, A.invc);
     A.N = gmem(tab, A.N);
@@ -277,6 +285,10 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
   const size_t ab = nyq ? sw_nblk(g) + ch : (size_t)lb;
   const size_t fb_uz = f_abs(g) + ab * nr * FR_ABS + lane;
   const size_t fb_p = fb_uz + f_absn(g);
+  // the mixed form reads the u+, u-, uz factors as FP32 for |m| >= m32 (the
+  // velocity Helmholtz operators, diagonally dominant through 1/dt); one |m|
+  // per wave: rows pair and NT - pair, or the Nyquist row
+  const bool f32w = __builtin_amdgcn_readfirstlane(A.F32 != nullptr && (nyq ? nth : pair) >= A.m32 ? 1 : 0) != 0;
   const size_t phb = ab * nr * 128 + lane;              // PH row i: phb + 128 i (+64: p_h)
   const size_t sb3 = (size_t)lb * nr * 384 + threadIdx.x;  // SR, E row i, component c: sb3 + 384 i + 128 c
   const size_t sb1 = (size_t)lb * nr * 128 + threadIdx.x;  // SP row i: sb1 + 128 i
@@ -509,12 +521,17 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
 #pragma unroll
       for (int c = 0; c < 3; c++) {  // forward elimination
         // u+, u-: rows of FR_UP, slots of 128; uz: rows of FR_ABS, slots of 64
-        const double* fr = c == 0 ? A.F + fb_up + (size_t)j * FR_UP
-                                  : (c == 1 ? A.F + fb_um + (size_t)j * FR_UP : A.F + fb_uz + (size_t)j * FR_ABS);
+        const size_t fo = c == 0 ? fb_up + (size_t)j * FR_UP
+                                 : (c == 1 ? fb_um + (size_t)j * FR_UP : fb_uz + (size_t)j * FR_ABS);
         const int fs = c == 2 ? 64 : 128;
         double Lc[IW];
+        if (f32w) {
 #pragma unroll
-        for (int w = 0; w < IW; w++) Lc[w] = fr[w * fs];
+          for (int w = 0; w < IW; w++) Lc[w] = A.F32[fo + w * fs];
+        } else {
+#pragma unroll
+          for (int w = 0; w < IW; w++) Lc[w] = A.F[fo + w * fs];
+        }
         double2 y = rh[c];
         y.x -= Lc[0] * yw[c][0].x;
         y.y -= Lc[0] * yw[c][0].y;
@@ -546,11 +563,21 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     auto backsub = [&](int i, double2* xo, double* ho) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
-        const double* fr = c == 0 ? A.F + fb_up + (size_t)i * FR_UP
-                                  : (c == 1 ? A.F + fb_um + (size_t)i * FR_UP : A.F + fb_uz + (size_t)i * FR_ABS);
+        const size_t fo = c == 0 ? fb_up + (size_t)i * FR_UP
+                                 : (c == 1 ? fb_um + (size_t)i * FR_UP : fb_uz + (size_t)i * FR_ABS);
         const int fs = c == 2 ? 64 : 128;
-        const double ru = fr[3 * fs];
-        const double u1 = fr[4 * fs], u2 = fr[5 * fs], u3 = fr[6 * fs];
+        double ru, u1, u2, u3;
+        if (f32w) {
+          ru = A.F32[fo + 3 * fs];
+          u1 = A.F32[fo + 4 * fs];
+          u2 = A.F32[fo + 5 * fs];
+          u3 = A.F32[fo + 6 * fs];
+        } else {
+          ru = A.F[fo + 3 * fs];
+          u1 = A.F[fo + 4 * fs];
+          u2 = A.F[fo + 5 * fs];
+          u3 = A.F[fo + 6 * fs];
+        }
         double2 x = A.SR[sb3 + (size_t)i * 384 + c * 128];
         x.x -= u1 * xw[c][0].x;
         x.y -= u1 * xw[c][0].y;
@@ -684,10 +711,12 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
 }
 
 size_t factor_count(const Geom& g) { return f_abs(g) + 2 * f_absn(g); }
+size_t factor32_count(const Geom& g) { return f_abs(g) + f_absn(g); }
 
-void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, hipStream_t s) {
+void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, float* F32,
+                   hipStream_t s) {
   dim3 grid((g.modes + 255) / 256, 4);
-  factor_kernel<<<grid, 256, 0, s>>>(g, r, Lu, Lp, F);
+  factor_kernel<<<grid, 256, 0, s>>>(g, r, Lu, Lp, F, F32);
 }
 
 template <int PRED, int B>

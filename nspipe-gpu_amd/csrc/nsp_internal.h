@@ -193,6 +193,8 @@ struct Lin2Args {
   const double* Lu;     // [2][NR][7] operator tables (factor recurrence on the fly)
   const double* Lp;     // [2][NR][7]
   const double* F;      // banded LU factors (sweep-side layout: fidx in linear2.hip)
+  const float* F32;     // mixed form: the u+, u-, uz factors rounded to FP32 (same indices), or null
+  int m32;              // |m| from which the sweep reads F32 instead of F
   const double* PH;     // dr p_h, p_h of the homogeneous pressure solution (sw_abs_idx, 2 per row)
   const double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
   const double2* N;     // [3] nonlinear term (r/theta); float2 storage when n32
@@ -206,8 +208,11 @@ struct Lin2Args {
   int predictor;
   int ord;              // block order (NlVariant::s_ord)
 };
-void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, hipStream_t s);
-size_t factor_count(const Geom& g);  // doubles of the stored LU factors
+// F32 (may be null): FP32 copies of the u+, u-, uz factor sets, same indices
+void launch_factor(const Geom& g, const double* r, const double* Lu, const double* Lp, double* F, float* F32,
+                   hipStream_t s);
+size_t factor_count(const Geom& g);    // doubles of the stored LU factors
+size_t factor32_count(const Geom& g);  // floats of the FP32 u+, u-, uz sets
 void launch_sweep2(const Lin2Args& a, hipStream_t s);
 // one launch per pass for `count` pipes (tab in device memory); max_nt/max_nz
 // are the largest extents among them

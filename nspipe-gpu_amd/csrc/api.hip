@@ -182,6 +182,7 @@ Lin2Args lin_args(nsp_handle* h, const Geom& g, const double2* N, const double2*
   a.F = h->F;
   a.F32 = h->F32;
   a.m32 = kM32;
+  a.s32 = h->nb.mixed ? 1 : 0;
   a.PH = h->PH;
   a.invc = h->b.invc;
   a.N = N;

@@ -228,6 +228,30 @@ __global__ __launch_bounds__(256) void factor_kernel(Geom g, const double* __res
   }
 }
 
+// Column scratch of one workgroup at FP64, or (mixed form, |m| >= m32) as
+// FP32 pairs.  Both are indexed with the FP64 sweep-side indices; the FP32
+// view is offset by the workgroup region's first FP64 index, so its values
+// fill the first half of the workgroup's own region.
+template <bool S32>
+struct Scr;
+template <>
+struct Scr<false> {
+  double2* p;
+  __device__ static Scr at(double2* base, size_t) { return Scr{base}; }
+  __device__ double2 ld(size_t i) const { return p[i]; }
+  __device__ void st(size_t i, double2 v) const { p[i] = v; }
+};
+template <>
+struct Scr<true> {
+  float2* p;
+  __device__ static Scr at(double2* base, size_t region0) { return Scr{reinterpret_cast<float2*>(base) + region0}; }
+  __device__ double2 ld(size_t i) const {
+    const float2 v = p[i];
+    return make_double2(v.x, v.y);
+  }
+  __device__ void st(size_t i, double2 v) const { p[i] = make_float2(float(v.x), float(v.y)); }
+};
+
 // ------------------------------------------------------------------ sweep
 // Workgroup = 2 waves: azimuthal rows it and NT - it (0 with NT/2) over one
 // chunk of 64 axial modes.  One kernel per pass (PASS 0..3 = A..D): the
@@ -307,6 +331,16 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
   const ctab Dcp = ctab(A.Dc) + par_p * STEN * nr;
   const ctab rt = ctab(A.r);
   const double2 z2 = make_double2(0.0, 0.0);
+  // Column scratch precision of this workgroup: the mixed form holds SR, SP
+  // and E of |m| >= m32 as FP32 pairs (the rhs, eliminated rows and pressure
+  // of the fluctuation modes; the mean flow and low |m| stay FP64).  Both
+  // waves have the same |m| (pair 0, rows 0 and NT/2, stays FP64).
+  const bool s32b = A.s32 != 0 && pair >= A.m32;
+  auto body = [&](auto s32_tag) {
+  constexpr bool S32 = decltype(s32_tag)::value;
+  const Scr<S32> SR = Scr<S32>::at(A.SR, (size_t)lb * nr * 384);
+  const Scr<S32> SP = Scr<S32>::at(A.SP, (size_t)lb * nr * 128);
+  const Scr<S32> E = Scr<S32>::at(A.E, (size_t)lb * nr * 384);
 
   // ============================== pass A (up)
   if constexpr (PASS == 0) {
@@ -350,7 +384,6 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       const int j = PRED ? t - 3 : t;
       double2 R0 = z2, R1 = z2, R2 = z2;
       if (j >= 0 && j < nr) {
-        const size_t h = (size_t)j * MS + k;
         const size_t hn = nb0 + (size_t)j * npl;  // tile-major N(u)
         double2 n0, n1, n2;
         if (A.n32) {
@@ -390,12 +423,12 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
             px += idt * aux.x;
             py += idt * aux.y;
             rh[c] = make_double2(nn[c].x + px, nn[c].y + py);
-            A.E[sb3 + (size_t)j * 384 + c * 128] =
-                make_double2(px + (1.0 - d_im) * nn[c].x, py + (1.0 - d_im) * nn[c].y);
+            E.st(sb3 + (size_t)j * 384 + c * 128,
+                 make_double2(px + (1.0 - d_im) * nn[c].x, py + (1.0 - d_im) * nn[c].y));
           }
         } else {
           const size_t he = sb3 + (size_t)j * 384;
-          const double2 e0 = A.E[he], e1 = A.E[he + 128], e2 = A.E[he + 256];
+          const double2 e0 = E.ld(he), e1 = E.ld(he + 128), e2 = E.ld(he + 256);
           rh[0] = make_double2(e0.x + d_im * n0.x, e0.y + d_im * n0.y);
           rh[1] = make_double2(e1.x + d_im * n1.x, e1.y + d_im * n1.y);
           rh[2] = make_double2(e2.x + d_im * n2.x, e2.y + d_im * n2.y);
@@ -405,9 +438,9 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         R1 = rh[1];
         R2 = rh[2];
         const size_t hs = sb3 + (size_t)j * 384;
-        A.SR[hs] = R0;
-        A.SR[hs + 128] = R1;
-        A.SR[hs + 256] = R2;
+        SR.st(hs, R0);
+        SR.st(hs + 128, R1);
+        SR.st(hs + 256, R2);
       }
 #pragma unroll
       for (int w = 0; w < 6; w++) Rr[w] = Rr[w + 1];
@@ -449,7 +482,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         dv.y -= l1 * yp[1].y;
         dv.x -= l2 * yp[2].x;
         dv.y -= l2 * yp[2].y;
-        A.SP[sb1 + (size_t)i * 128] = dv;
+        SP.st(sb1 + (size_t)i * 128, dv);
         yp[0] = yp[1];
         yp[1] = yp[2];
         yp[2] = dv;
@@ -467,7 +500,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       const double ru = fp[3 * 64];
       const double u1 = fp[4 * 64], u2 = fp[5 * 64], u3 = fp[6 * 64];
       const size_t h = sb1 + (size_t)t * 128;
-      double2 x = A.SP[h];
+      double2 x = SP.ld(h);
       x.x -= u1 * pw[0].x;
       x.y -= u1 * pw[0].y;
       x.x -= u2 * pw[1].x;
@@ -475,7 +508,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       x.x -= u3 * pw[2].x;
       x.y -= u3 * pw[2].y;
       const double2 pn = make_double2(x.x * ru, x.y * ru);
-      A.SP[h] = pn;
+      SP.st(h, pn);
       pw[2] = pw[1];
       pw[1] = pw[0];
       pw[0] = pn;
@@ -495,7 +528,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     for (int t = 0; t < nr + IW; t++) {
 #pragma unroll
       for (int w = 0; w < 6; w++) pw[w] = pw[w + 1];
-      pw[6] = (t < nr) ? A.SP[sb1 + (size_t)t * 128] : z2;
+      pw[6] = (t < nr) ? SP.ld(sb1 + (size_t)t * 128) : z2;
       const int j = t - IW;
       if (j < 0) continue;
       // addGrad_k (mod_radialFD.cu:115-143)
@@ -509,7 +542,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       }
       const double rK = rt[j];
       const double2 ph = pw[3];
-      double2 rh[3] = {A.SR[h], A.SR[h + 128], A.SR[h + 256]};
+      double2 rh[3] = {SR.ld(h), SR.ld(h + 128), SR.ld(h + 256)};
       rh[0].x -= px;
       rh[0].y -= py;
       rh[1].x -= -q.kt * ph.y / rK;
@@ -539,7 +572,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         y.y -= Lc[1] * yw[c][1].y;
         y.x -= Lc[2] * yw[c][2].x;
         y.y -= Lc[2] * yw[c][2].y;
-        A.SR[h + c * 128] = y;
+        SR.st(h + c * 128, y);
         yw[c][0] = yw[c][1];
         yw[c][1] = yw[c][2];
         yw[c][2] = y;
@@ -578,7 +611,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
           u2 = A.F[fo + 5 * fs];
           u3 = A.F[fo + 6 * fs];
         }
-        double2 x = A.SR[sb3 + (size_t)i * 384 + c * 128];
+        double2 x = SR.ld(sb3 + (size_t)i * 384 + c * 128);
         x.x -= u1 * xw[c][0].x;
         x.y -= u1 * xw[c][0].y;
         x.x -= u2 * xw[c][1].x;
@@ -708,6 +741,9 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       atomicMax(&A.norm[1], (unsigned long long)__double_as_longlong(den));
     }
   }
+  };
+  if (s32b) body(std::true_type{});
+  else body(std::false_type{});
 }
 
 size_t factor_count(const Geom& g) { return f_abs(g) + 2 * f_absn(g); }

@@ -195,6 +195,7 @@ struct Lin2Args {
   const double* F;      // banded LU factors (sweep-side layout: fidx in linear2.hip)
   const float* F32;     // mixed form: the u+, u-, uz factors rounded to FP32 (same indices), or null
   int m32;              // |m| from which the sweep reads F32 instead of F
+  int s32;              // mixed form: column scratch SR, SP and E of |m| >= m32 held as FP32 pairs
   const double* PH;     // dr p_h, p_h of the homogeneous pressure solution (sw_abs_idx, 2 per row)
   const double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
   const double2* N;     // [3] nonlinear term (r/theta); float2 storage when n32

@@ -36,7 +36,9 @@ typedef enum {
 
 typedef enum {
   NSP_FP64 = 0,   /* every stage in FP64 (parity form) */
-  NSP_MIXED = 1   /* FP32 transforms/products and non-stiff solves, FP64 state */
+  NSP_MIXED = 1   /* FP32 transforms/products; the velocity solves of |m| >= 4 read FP32 LU factors
+                     and keep their column scratch in FP32 (FP64 arithmetic); FP64 state, pressure
+                     factors and low-|m| solves */
 } nsp_precision;
 
 typedef enum { NSP_HOST = 0, NSP_DEVICE = 1 } nsp_where;

@@ -1432,6 +1432,7 @@ struct nsp_ensemble {
   std::vector<Cls> cls;
   std::vector<int> order;          // table slot -> member
   int max_nr = 0, max_nt = 0, max_nz = 0, max_modes = 0, maxit = 0;
+  bool fp64 = false, mixed = false;  // some member is of that form
   double tol = 0.0;
   // device: NL tables per input role [3][M], sweep tables [3][M], records [M], norm pointers [M]
   NlMember* nl = nullptr;
@@ -1566,7 +1567,7 @@ nsp_status ens_capture(nsp_ensemble* e) {
     const int role = (it % 2) ? 1 : 2;
     ok &= ens_nonlinear(e, role, s);
     launch_ens_norm_reset(e->norms, e->rec, M, s);
-    launch_sweep_batch(e->sw + role * M, M, 0, e->max_nt, e->max_nz, s);
+    launch_sweep_batch(e->sw + role * M, M, 0, e->max_nt, e->max_nz, e->fp64, e->mixed, s);
     launch_ens_converge(e->norms, e->rec, M, e->tol, it, e->maxit, s);
   };
   auto finish = [&](hipGraphExec_t* out) -> nsp_status {
@@ -1590,7 +1591,7 @@ nsp_status ens_capture(nsp_ensemble* e) {
   (void)hipMemcpyAsync(e->nl, e->stage_h, e->stage_bytes, hipMemcpyHostToDevice, s);
   launch_ens_begin(e->rec, M, s);
   ok &= ens_nonlinear(e, 0, s);
-  launch_sweep_batch(e->sw, M, 1, e->max_nt, e->max_nz, s);
+  launch_sweep_batch(e->sw, M, 1, e->max_nt, e->max_nz, e->fp64, e->mixed, s);
   if (e->maxit >= 1) corrector(1);
   nsp_status st = finish(&e->gexec);
   if (st != NSP_OK || e->maxit < 2) return st;
@@ -1653,6 +1654,7 @@ nsp_status nsp_ensemble_create(const nsp_params* params, int count, int device, 
     e->max_nt = std::max(e->max_nt, g.nt);
     e->max_nz = std::max(e->max_nz, g.nz);
     e->max_modes = std::max(e->max_modes, g.modes);
+    (e->m[j]->nb.mixed ? e->mixed : e->fp64) = true;
   }
   for (auto& C : e->cls) {
     C.off = (int)e->order.size();

@@ -8,11 +8,11 @@
 //   C (up)   rhs -= grad p three rows behind (addGrad_k); to +/-; zero wall
 //            rows (write_bcon_k); forward elimination of the u+, u-, uz systems
 //   D (down) back substitution of u+, u-, uz together with the homogeneous
-//            wall-unit solutions; after the four wall rows the wall
-//            divergence fixes the influence coefficients (corBCon), which
-//            are then applied row by row; to r/theta; convergence measure
-//            (mcor_k) and output.  Mode (0,0) then takes modeZZ_k and the
-//            flux correction (adj_flux).
+//            wall-unit solutions; the wall divergence of the four wall rows
+//            fixes the influence coefficients (corBCon); a second walk from
+//            the wall applies them row by row; to r/theta; convergence
+//            measure (mcor_k) and output.  Mode (0,0) then takes modeZZ_k
+//            and the flux correction (adj_flux).
 //
 // The banded LU factors of the four operators depend only on (m, kz), dt and
 // Re, so they are computed once at setup with the reference's recurrence
@@ -259,7 +259,12 @@ struct Scr<true> {
 // gets its own register allocation (and occupancy).
 // B = 1: batched launch, one pipe per blockIdx.y with its arguments from tab
 // (the single-pipe form B = 0 reads only its own argument block).
-template <int PASS, int PRED, int B>
+// HOLD (pass D): 1 keeps the four wall rows in registers across the
+// coefficient solve, 0 walks them twice (fewer registers, one more wave per
+// SIMD). The FP64 form is faster with the first, the mixed form with the
+// second. The two compile to different instruction sequences (contraction),
+// so a pipe's form fixes which one it runs, batched or not.
+template <int PASS, int PRED, int B, int HOLD = 0>
 __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* tab) {
   if constexpr (B != 0) {
     A = tab[blockIdx.y];
@@ -282,6 +287,15 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     A.SR = gmem(tab, A.SR);
     A.SP = gmem(tab, A.SP);
     A.norm = gmem(tab, A.norm);
+    if constexpr (PASS == 3) {  // members of the other form are the other launch's
+      // The form is tested on a copy the compiler cannot see through: knowing
+      // F32 and s32 past this test would let it drop the other form's paths
+      // and then contract the remaining arithmetic differently from the
+      // single-pipe kernel, whose results the batched one must match bit for bit.
+      int fp64 = __builtin_amdgcn_readfirstlane((A.F32 == nullptr && A.s32 == 0) ? 1 : 0);
+      asm volatile("" : "+s"(fp64));
+      if (HOLD != fp64) return;
+    }
   }
   const Geom& g = A.g;
   if (step_done(g)) return;
@@ -591,8 +605,6 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         xw[c][w] = z2;
         xh[c][w] = 0.0;
       }
-    double2 xs[4][3];   // wall rows NR-1..NR-4
-    double xhs[4][3];
     auto backsub = [&](int i, double2* xo, double* ho) {
 #pragma unroll
       for (int c = 0; c < 3; c++) {
@@ -634,16 +646,41 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         ho[c] = hh;
       }
     };
-#pragma unroll
-    for (int s = 0; s < 4; s++) backsub(nr - 1 - s, xs[s], xhs[s]);
-    // wall divergence of the provisional solution (divWall_k on r/theta form)
-    double2 urw[4], utw = z2, uzw = xs[0][2];
+    // wall divergence of the provisional solution (divWall_k on r/theta
+    // form) from the four wall rows. Without HOLD the walk that emits the
+    // solution repeats their back substitution (same arithmetic, same
+    // values), which keeps the pass at 3 waves per SIMD instead of 2.
+    double2 urw[4], utw = z2, uzw = z2;
+    double2 xs[HOLD ? 4 : 1][3];  // HOLD: wall rows NR-1..NR-4
+    double xhs[HOLD ? 4 : 1][3];
 #pragma unroll
     for (int s = 0; s < 4; s++) {
-      double2 a = xs[s][0], b = xs[s][1];
+      double2 xo[3];
+      double ho[3];
+      backsub(nr - 1 - s, xo, ho);
+      if constexpr (HOLD) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          xs[s][c] = xo[c];
+          xhs[s][c] = ho[c];
+        }
+      }
+      double2 a = xo[0], b = xo[1];
       rt_of(a, b);
       urw[s] = a;
-      if (s == 0) utw = b;
+      if (s == 0) {
+        utw = b;
+        uzw = xo[2];
+      }
+    }
+    if constexpr (!HOLD) {
+#pragma unroll
+      for (int c = 0; c < 3; c++)
+#pragma unroll
+        for (int w = 0; w < 3; w++) {
+          xw[c][w] = z2;
+          xh[c][w] = 0.0;
+        }
     }
     double2 dv;
     {
@@ -703,9 +740,11 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       A.out[n + h] = o[1];
       A.out[2 * n + h] = o[2];
     };
+    if constexpr (HOLD) {
 #pragma unroll
-    for (int s = 0; s < 4; s++) emit(nr - 1 - s, xs[s], xhs[s]);
-    for (int i = nr - 5; i >= 0; i--) {
+      for (int s = 0; s < 4; s++) emit(nr - 1 - s, xs[s], xhs[s]);
+    }
+    for (int i = HOLD ? nr - 5 : nr - 1; i >= 0; i--) {
       double2 xo[3];
       double ho[3];
       backsub(i, xo, ho);
@@ -737,8 +776,26 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       }
     }
     if (!PRED) {
-      atomicMax(&A.norm[0], (unsigned long long)__double_as_longlong(num));
-      atomicMax(&A.norm[1], (unsigned long long)__double_as_longlong(den));
+      // num, den are >= 0 or NaN, whose bit patterns order as the values do
+      // (NaN above every finite value and +inf). A full wave combines its
+      // lanes' maxima and issues one atomic per value; a partly filled wave
+      // (the last axial chunk) keeps one per lane.
+      unsigned long long a = __double_as_longlong(num), b = __double_as_longlong(den);
+      if (__builtin_amdgcn_read_exec() == ~0ull) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+          const unsigned long long ao = __shfl_xor(a, o), bo = __shfl_xor(b, o);
+          a = ao > a ? ao : a;
+          b = bo > b ? bo : b;
+        }
+        if (lane == 0) {
+          atomicMax(&A.norm[0], a);
+          atomicMax(&A.norm[1], b);
+        }
+      } else {
+        atomicMax(&A.norm[0], a);
+        atomicMax(&A.norm[1], b);
+      }
     }
   }
   };
@@ -755,25 +812,30 @@ void launch_factor(const Geom& g, const double* r, const double* Lu, const doubl
   factor_kernel<<<grid, 256, 0, s>>>(g, r, Lu, Lp, F, F32);
 }
 
+// pass D runs as HOLD = 1 for the FP64-form pipes and as HOLD = 0 for the
+// mixed-form ones (a batched launch of both forms makes both launches)
 template <int PRED, int B>
-static void sweep_passes(const Lin2Args& a, const Lin2Args* tab, dim3 grid, hipStream_t s) {
+static void sweep_passes(const Lin2Args& a, const Lin2Args* tab, dim3 grid, bool fp64, bool mixed, hipStream_t s) {
   sweep_kernel<0, PRED, B><<<grid, 128, 0, s>>>(a, tab);
   sweep_kernel<1, PRED, B><<<grid, 128, 0, s>>>(a, tab);
   sweep_kernel<2, PRED, B><<<grid, 128, 0, s>>>(a, tab);
-  sweep_kernel<3, PRED, B><<<grid, 128, 0, s>>>(a, tab);
+  if (fp64) sweep_kernel<3, PRED, B, 1><<<grid, 128, 0, s>>>(a, tab);
+  if (mixed) sweep_kernel<3, PRED, B, 0><<<grid, 128, 0, s>>>(a, tab);
 }
 
 void launch_sweep2(const Lin2Args& a, hipStream_t s) {
   const dim3 grid((a.g.nt / 2) * ((a.g.nz + 63) / 64));
-  if (a.predictor) sweep_passes<1, 0>(a, nullptr, grid, s);
-  else sweep_passes<0, 0>(a, nullptr, grid, s);
+  const bool fp64 = a.F32 == nullptr && a.s32 == 0;
+  if (a.predictor) sweep_passes<1, 0>(a, nullptr, grid, fp64, !fp64, s);
+  else sweep_passes<0, 0>(a, nullptr, grid, fp64, !fp64, s);
 }
 
-void launch_sweep_batch(const Lin2Args* tab, int count, int predictor, int max_nt, int max_nz, hipStream_t s) {
+void launch_sweep_batch(const Lin2Args* tab, int count, int predictor, int max_nt, int max_nz, bool fp64,
+                        bool mixed, hipStream_t s) {
   const dim3 grid((max_nt / 2) * ((max_nz + 63) / 64), count);
   Lin2Args dummy{};
-  if (predictor) sweep_passes<1, 1>(dummy, tab, grid, s);
-  else sweep_passes<0, 1>(dummy, tab, grid, s);
+  if (predictor) sweep_passes<1, 1>(dummy, tab, grid, fp64, mixed, s);
+  else sweep_passes<0, 1>(dummy, tab, grid, fp64, mixed, s);
 }
 
 }  // namespace nsp

@@ -216,8 +216,9 @@ size_t factor_count(const Geom& g);    // doubles of the stored LU factors
 size_t factor32_count(const Geom& g);  // floats of the FP32 u+, u-, uz sets
 void launch_sweep2(const Lin2Args& a, hipStream_t s);
 // one launch per pass for `count` pipes (tab in device memory); max_nt/max_nz
-// are the largest extents among them
-void launch_sweep_batch(const Lin2Args* tab, int count, int predictor, int max_nt, int max_nz, hipStream_t s);
+// are the largest extents among them; fp64/mixed: some pipe is of that form
+void launch_sweep_batch(const Lin2Args* tab, int count, int predictor, int max_nt, int max_nz, bool fp64,
+                        bool mixed, hipStream_t s);
 // centred stencil table: w-th weight of row i multiplies column i-3+w
 std::vector<double> centred_table(int nr, const std::vector<double>& T);
 

@@ -755,26 +755,37 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
   static_assert(NTHR % TL == 0, "a thread keeps one column");
   const int tcol = tid % TL;
   const bool lok = l0 + tcol < nz - 1;  // l = NZ-1 and columns past NZ enter as zero
-#pragma unroll 1
-  for (int c = 0; c < 3; c++) {
+  // first-pass inputs of component c (tile-major Y); the next component's
+  // are loaded while the current one's LDS passes run
+  cpx<T> v[PPT][R0];
+  auto load = [&](int c) {
     const int lt = opaque(tid);
-    const cpx<T>* src = Y + c * ws + (size_t)i * tp_plane(g) + tp_col(g, lok ? l0 + tcol : 0);  // tile-major Y
+    const cpx<T>* src = Y + c * ws + (size_t)i * tp_plane(g) + tp_col(g, lok ? l0 + tcol : 0);
 #pragma unroll
     for (int s = 0; s < PPT; s++) {
       const int p = lt + s * NTHR;
       if ((PAIRS % NTHR) == 0 || p < PAIRS) {
-        const int col = p % TL, n0 = p / TL;
-        cpx<T> v[R0];
+        const int n0 = p / TL;
 #pragma unroll
-        for (int q = 0; q < R0; q++) v[q] = src[(size_t)(n0 + NB1 * q) * 64];
+        for (int q = 0; q < R0; q++) v[s][q] = src[(size_t)(n0 + NB1 * q) * 64];
         if (!lok) {
 #pragma unroll
-          for (int q = 0; q < R0; q++) v[q] = cpx<T>{T(0), T(0)};
+          for (int q = 0; q < R0; q++) v[s][q] = cpx<T>{T(0), T(0)};
         }
-        first_pass_store<T, -1, LY, TL>(PL{}, v, buf, n0, col);
       }
     }
+  };
+  load(0);
+#pragma unroll 1
+  for (int c = 0; c < 3; c++) {
+    const int lt = opaque(tid);
+#pragma unroll
+    for (int s = 0; s < PPT; s++) {
+      const int p = lt + s * NTHR;
+      if ((PAIRS % NTHR) == 0 || p < PAIRS) first_pass_store<T, -1, LY, TL>(PL{}, v[s], buf, p / TL, p % TL);
+    }
     __syncthreads();
+    if (c + 1 < 3) load(c + 1);
     // N(u) is stored at the transform precision (FP32 in the mixed form)
     // tile-major N(u): the TL columns from l0 lie in one 64-column tile
     cpx<T>* dst = reinterpret_cast<cpx<T>*>(N) + c * dr_cs(g) + (size_t)i * dr_plane(g) + dr_col(g, l0);

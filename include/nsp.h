@@ -50,10 +50,10 @@ typedef struct {
   int nz;          /* NZ  = m_z+1 axial modes, odd (head.h:18) */
   double lz;       /* LZ  pipe length in R (head.h:22) */
   double re;       /* Re  (head.h:25) */
-  double dt;       /* dt  constant time step (head.h:46) */
-  double d_im;     /* implicit weight, 0.51 in the reference (head.h:47) */
-  int maxit;       /* corrector iterations cap, 10 (head.h:49) */
-  double tol;      /* corrector tolerance, 5e-8 (head.h:50) */
+  double dt;       /* dt  constant time step (head.h:48) */
+  double d_im;     /* implicit weight, 0.51 in the reference (head.h:49) */
+  int maxit;       /* corrector iterations cap, 10 (head.h:51) */
+  double tol;      /* corrector tolerance, 5e-8 (head.h:52) */
   double qdes;     /* target bulk flux, 0.5 (mod_cublasflux.cu:92) */
   int precision;   /* nsp_precision */
 } nsp_params;
@@ -63,7 +63,7 @@ typedef struct nsp_handle nsp_handle;
 /* Per-step report (the reference prints err every dt_frc, mod_integrate.cu:136). */
 typedef struct {
   int n_it;        /* corrector iterations used (mod_integrate.cu:120) */
-  double err;      /* final convergence measure err[0] (mod_cublasflux.cu:356) */
+  double err;      /* final convergence measure err[0] (mod_cublasflux.cu:115) */
   int flags;       /* bit0: err was NaN; bit1: stopped at maxit without err <= tol */
 } nsp_step_info;
 
@@ -110,8 +110,8 @@ nsp_status nsp_nonlinear(nsp_handle* h, const double* ur, const double* ut, cons
  * out[5] max |wall divergence|, out[6] bulk flux sum rdr Re uz00. */
 nsp_status nsp_diagnostics(nsp_handle* h, double out[7]);
 
-/* Radial tables (init_fd, mod_radialFD.cu:256-369; get_dr0 :373-407;
- * get_dr1 :412-415): r[NR], D[2][NR][7], Lu[2][NR][7], Lp[2][NR][7],
+/* Radial tables (init_fd, mod_radialFD.cu:199-312; get_dr0 :316-350;
+ * get_dr1 :355-358): r[NR], D[2][NR][7], Lu[2][NR][7], Lp[2][NR][7],
  * rdr[NR], dr0[4], dr1[7].  Any pointer may be NULL. */
 nsp_status nsp_tables(nsp_handle* h, double* r, double* D, double* Lu, double* Lp, double* rdr,
                       double* dr0, double* dr1);
@@ -141,8 +141,8 @@ nsp_status nsp_write_state_files(nsp_handle* h, const char* ur_path, const char*
 nsp_status nsp_write_restart(nsp_handle* h, const char* dir, double t, long long step);
 nsp_status nsp_read_restart(nsp_handle* h, const char* dir, double* t, long long* step);
 
-/* Remesh (tools/Re_Mesh/src/mod_remesh.cu: re_mesh :50-124, rmsh_k :8-30,
- * modeZZ_k :35-41): the state of src, on its (NR, NT, NZ), interpolated onto
+/* Remesh (tools/Re_Mesh/src/mod_remesh.cu: re_mesh :50-124, rmsh_k :8-29,
+ * modeZZ_k :34-40): the state of src, on its (NR, NT, NZ), interpolated onto
  * the grid of dst and made dst's state.  Radially each new point is the
  * Lagrange interpolant of 2*3 old points of the grid extended through the
  * axis (mirrored values carry the parity of the field: ur, ut odd in r for
@@ -152,7 +152,7 @@ nsp_status nsp_read_restart(nsp_handle* h, const char* dir, double* t, long long
  * to zero.  No projection follows, as in the reference.  flags:
  * NSP_REMESH_REFERENCE keeps the reference's column assignment of the
  * mirrored points (exact only for rows whose stencil starts at the outermost
- * mirrored point, :92-103); NSP_REMESH_MIRROR gives every mirrored point its
+ * mirrored point, :97-107); NSP_REMESH_MIRROR gives every mirrored point its
  * own column.  src and dst are different handles on one device. */
 #define NSP_REMESH_REFERENCE 0
 #define NSP_REMESH_MIRROR 1
@@ -163,7 +163,7 @@ nsp_status nsp_remesh(nsp_handle* dst, nsp_handle* src, int flags);
 nsp_status nsp_remesh_weights(int nr_old, int nr_new, int flags, double* RM);
 
 /* Physical-space velocity component comp (0 ur, 1 ut, 2 uz) of the current
- * state: padForward + fftBackward of the reference (mod_io_nonlinear.cu:173-195,
+ * state: padForward + fftBackward of the reference (padForward_k, mod_io_nonlinear.cu:173-192;
  * mod_fft.cu:43-49), the field its commented HDF5 writer stores (:486-521).
  * out[NR][NTP][NZF+2] doubles, NTP = 3NT/2, NZF = 3(NZ-1): row (i, j) holds
  * u(r_i, theta_j = 2 pi j/NTP, z_n = n LZ/NZF) for n < NZF, then the two
@@ -172,7 +172,7 @@ nsp_status nsp_remesh_weights(int nr_old, int nr_new, int flags, double* RM);
 nsp_status nsp_physical_field(nsp_handle* h, int comp, double* out, int where);
 
 /* Cross-section integrals at the NZF axial points (wrt_qcr, wrt_ucr,
- * mod_io_nonlinear.cu:576-654: q0kernel_k/u0kernel_k :55-79, the periodic
+ * mod_io_nonlinear.cu:576-654: q0kernel_k/u0kernel_k :55-78, the periodic
  * three-point rule of q1kernel_k :81-103, intq mod_cublasflux.cu:119-130):
  * q[n] = sum_i rdr_i sum_j 0.5/(3 NTP) (f_{j-1} + 4 f_j + f_{j+1}) with
  * f = ur^2 + ut^2, u2[n] the same with f = uz^2.  Host arrays; either may be
@@ -183,7 +183,7 @@ nsp_status nsp_cross_sections(nsp_handle* h, double* q, double* u2);
  * mod_io_nonlinear.cu:119-130). */
 nsp_status nsp_mean_profile(nsp_handle* h, double* prof);
 
-/* Diagnostics files in the reference's formats (setIO, mod_io_nonlinear.cu:255-312;
+/* Diagnostics files in the reference's formats (setIO, mod_io_nonlinear.cu:261-317;
  * wrt_frc :460-479; wrt_qcr/wrt_ucr :576-654; wrt_mpr :663-687).
  * nsp_io_open creates (truncates) io_friction.txt, io_qcross.txt,
  * io_ucross.txt and io_meanpr.txt in dir and writes the header rows (axial

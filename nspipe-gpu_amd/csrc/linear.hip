@@ -3,7 +3,7 @@
 //   * setup_influence_kernel: the homogeneous wall-unit and pressure
 //     solutions and the inverse influence matrix (setBCon,
 //     mod_boundary.cu:205-255) and the flux profile (setCublasFlux,
-//     mod_cublasflux.cu:297-321), one thread per Fourier mode (it, l) walking
+//     mod_cublasflux.cu:56-80), one thread per Fourier mode (it, l) walking
 //     its radial column with the reference's banded LU (LUsolve_*_k);
 //   * the step record and convergence-test kernels of a captured step
 //     (single pipe and ensemble), the mixed form's N(u) widening, additive
@@ -274,7 +274,7 @@ __global__ void converge_kernel(const unsigned long long* norm, int* st, double*
 }
 
 // ------------------------------------------------------------------ setup
-// setBCon (mod_boundary.cu:205-255) and setCublasFlux (mod_cublasflux.cu:297-321),
+// setBCon (mod_boundary.cu:205-255) and setCublasFlux (mod_cublasflux.cu:56-80),
 // one thread per mode.
 __global__ __launch_bounds__(256) void setup_influence_kernel(Geom g, DevTables t, DevInfluence b,
                                                               double2* P, double* Us, double* PH) {

@@ -750,7 +750,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       backsub(i, xo, ho);
       emit(i, xo, ho);
     }
-    if (k == 0) {  // modeZZ_k, adj_flux (mod_integrate.cu:63-68, mod_cublasflux.cu:331-344)
+    if (k == 0) {  // modeZZ_k, adj_flux (mod_integrate.cu:63-68, mod_cublasflux.cu:90-103)
       double qa = 0.0;
       for (int i = 0; i < nr; i++) {
         const size_t h = (size_t)i * MS;

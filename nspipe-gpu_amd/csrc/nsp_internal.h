@@ -13,12 +13,12 @@
 namespace nsp {
 
 constexpr double PI2 = 6.283185307179586;  // head.h:12
-constexpr int STEN = 7;                    // head.h:42
-constexpr int IW = 3;                      // head.h:43
+constexpr int STEN = 7;                    // head.h:44
+constexpr int IW = 3;                      // head.h:45
 constexpr int MAXRAD = 16;
 
 // ----------------------------------------------------------------- tables
-// Host FP64 radial tables (mod_radialFD.cu:256-415).
+// Host FP64 radial tables (init_fd, get_dr0, get_dr1: mod_radialFD.cu:199-358).
 struct RadialTables {
   std::vector<double> r, D, Lu, Lp, rdr, dr0, dr1;
 };
@@ -141,7 +141,7 @@ struct DevTables {
   double* rdr;   // [NR]
   double* dr0;   // [4]
   double* dr1;   // [7]
-  double* uc;    // [NR] flux profile (mod_cublasflux.cu:310-317)
+  double* uc;    // [NR] flux profile (mod_cublasflux.cu:69-76)
   double* qc;    // [1]  Qc = rdr . uc
 };
 

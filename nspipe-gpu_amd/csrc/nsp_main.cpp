@@ -12,7 +12,7 @@
 // (mod_io_nonlinear.cu:426-429); --restart_dir reads a restart directory
 // (state plus time and step).  With --io_dir it writes io_friction.txt,
 // io_qcross.txt, io_ucross.txt and io_meanpr.txt there, each when the time
-// since its last row reaches its --dt_* interval (mod_integrate.cu:130-151,
+// since its last row reaches its --dt_* interval (mod_integrate.cu:131-151,
 // head.h:33-40).  Every --out_every steps it prints step, err and the
 // friction values (t, u_cl, u_tau).  At the end it writes ur.bin, ut.bin,
 // uz.bin (main.cu:42-44), or a restart directory with --save_dir, and the

@@ -8,8 +8,8 @@ namespace nsp {
 
 using C = cpx<double>;
 
-// Remesh gather (rmsh_k, tools/Re_Mesh/src/mod_remesh.cu:8-30, then
-// modeZZ_k :35-41).  One thread per (new mode, component): a mode present
+// Remesh gather (rmsh_k, tools/Re_Mesh/src/mod_remesh.cu:8-29, then
+// modeZZ_k :34-40).  One thread per (new mode, component): a mode present
 // on the old grid gets, at every new radial row, the remesh row's weights
 // times the old column, summed over the row's nonzero band in increasing
 // column order; other modes are zero.  ur, ut take the odd-in-r half for even
@@ -54,7 +54,7 @@ void launch_remesh(const Geom& gn, const Geom& go, const double* RM, const int2*
 }
 
 // ------------------------------------------------------------------ physical space
-// padForward (mod_io_nonlinear.cu:173-195) and the theta pass of the 2-D
+// padForward_k (mod_io_nonlinear.cu:173-192) and the theta pass of the 2-D
 // inverse transform (mod_fft.cu:43-49) for one spectral component: block
 // (l-tile, i) transforms TL axial columns of level i; rows it < NT/2 go to
 // padded row it, the others to it + NT/2, rows NT/2..NT-1 stay zero.

@@ -8,7 +8,7 @@
 namespace nsp {
 
 // Fornberg weights of derivatives 0..m at rau[i0+IW] from the points
-// rau[i0 .. i0+right-left] (mod_radialFD.cu:422-452).  c[j + n*k] holds the
+// rau[i0 .. i0+right-left] (mod_radialFD.cu:365-395).  c[j + n*k] holds the
 // weight of point j for derivative k, n = number of points.
 static void fornberg(double* c, const double* rau, int i0, int left, int right, int m) {
   const int n = 1 + right - left;
@@ -54,7 +54,7 @@ void radial_grid(int nr, double* r) {
 }
 
 // Interpolation weights at x0 from the points x[0..n-1]: the derivative-free
-// case of Fornberg's recurrence (tools/Re_Mesh/src/mod_remesh.cu:147-178).
+// case of Fornberg's recurrence (tools/Re_Mesh/src/mod_remesh.cu:148-178).
 static void lagrange_weights(double* c, const double* x, double x0, int n) {
   c[0] = 1.0;
   double c1 = 1.0, c4 = x[0] - x0;
@@ -120,10 +120,10 @@ std::vector<double> remesh_matrix(const std::vector<double>& ro, const std::vect
 
 // Radial grid, parity-folded derivative tables, quadrature weights and the
 // Laplacians of the velocity (Dirichlet wall row) and pressure (Neumann wall
-// row): mod_radialFD.cu:256-369.  Table index: [parity][row][7], parity 0
+// row): mod_radialFD.cu:199-312.  Table index: [parity][row][7], parity 0
 // for even functions of r (ghost points folded with +), 1 for odd (-).
 // The reference's quadrature loop runs one derivative order past the weights
-// it computed (mod_radialFD.cu:329 reads c[ir+len*len], beyond its 49-double
+// it computed (mod_radialFD.cu:272-277 reads c[ir+len*len], beyond its 49-double
 // buffer); that term is of size dr^8/8! and is omitted here.
 void build_radial_tables(int nr, RadialTables& T) {
   const int hD = STEN * nr;
@@ -137,13 +137,13 @@ void build_radial_tables(int nr, RadialTables& T) {
   double* r = T.r.data();
 
   // Cosine grid clustered to the wall, then stretched so the axis sits
-  // midway to its mirror point (mod_radialFD.cu:268-273).
+  // midway to its mirror point (mod_radialFD.cu:211-216).
   radial_grid(nr, r);
   ri[0] = 0.0;
   for (int i = 0; i < nr; i++) ri[i + 1] = r[i];
   for (int i = 0; i < nr + IW; i++) rau[i] = (i < IW) ? -r[IW - i - 1] : r[i - IW];
 
-  // First and second derivative rows (mod_radialFD.cu:278-309).
+  // First and second derivative rows (mod_radialFD.cu:220-252).
   int ghosts = IW;
   for (int i = 0; i < nr; i++) {
     const int left = i, right = std::min(2 * IW + i, nr + IW - 1), len = 1 + right - left;
@@ -179,7 +179,7 @@ void build_radial_tables(int nr, RadialTables& T) {
 
   // Quadrature weights: rdr_j such that sum_j rdr_j f_j ~ 2 int_0^1 f r dr,
   // from Taylor expansions about each r_i over [r_{i-1}, r_{i+1}] with
-  // r_{-1} = 0 and r_{NR} = r_{NR-1} (mod_radialFD.cu:312-337).
+  // r_{-1} = 0 and r_{NR} = r_{NR-1} (mod_radialFD.cu:253-280).
   ghosts = IW;
   for (int i = 0; i < nr; i++) {
     const int left = i, right = std::min(2 * IW + i, nr + IW - 1), len = 1 + right - left;
@@ -202,7 +202,7 @@ void build_radial_tables(int nr, RadialTables& T) {
     }
   }
 
-  // Radial Laplacians (mod_radialFD.cu:339-359).
+  // Radial Laplacians (mod_radialFD.cu:281-302).
   for (int i = 0; i < nr; i++) {
     for (int j = 0; j < STEN; j++) {
       const int q = j + STEN * i;
@@ -220,7 +220,7 @@ void build_radial_tables(int nr, RadialTables& T) {
     }
   }
 
-  // Interpolation weights to r = 0 from r[0..IW] (get_dr0, mod_radialFD.cu:373-407):
+  // Interpolation weights to r = 0 from r[0..IW] (get_dr0, mod_radialFD.cu:316-350):
   // Lagrange weights l_j(0) = prod_{q != j} (0 - r_q)/(r_j - r_q), in the
   // reference's Fornberg recurrence order.
   {
@@ -243,7 +243,7 @@ void build_radial_tables(int nr, RadialTables& T) {
     }
     T.dr0.assign(cc, cc + n);
   }
-  // Wall row of the even derivative table (get_dr1, mod_radialFD.cu:412-415).
+  // Wall row of the even derivative table (get_dr1, mod_radialFD.cu:355-358).
   T.dr1.assign(T.D.begin() + (nr - 1) * STEN, T.D.begin() + nr * STEN);
 }
 

@@ -47,8 +47,8 @@ struct cplx { double x, y; };
 
 constexpr double PI2 = 6.283185307179586;   // head.h:12
 constexpr double LT  = PI2;                 // head.h:21
-constexpr int STEN = 7;                     // head.h:42
-constexpr int IW   = STEN / 2;              // head.h:43
+constexpr int STEN = 7;                     // head.h:44
+constexpr int IW   = STEN / 2;              // head.h:45
 
 // ---------------------------------------------------------------------------
 // Mixed-radix complex FFT (decimation in time), cuFFT sign convention:
@@ -123,7 +123,7 @@ struct Oracle {
   std::vector<double> r, D, Lu, Lp, rdr, dr0, dr1;
   // influence-matrix data (mod_boundary.cu:200)
   std::vector<cplx> uh[3], uhP[3], invM, coef, div1;
-  std::vector<double> uc;  // mod_cublasflux.cu:269
+  std::vector<double> uc;  // mod_cublasflux.cu:69-76
   double Qc = 0.0;
   // state and work fields (main.cu:30-33, mod_integrate.cu:100-107)
   std::vector<cplx> u[3], rhs[3], uw[3], rhsw[3], ui[3], dtmp[3];
@@ -137,7 +137,7 @@ struct Oracle {
   inline double kz_of(int l) const { return (PI2 / P.LZ) * double(l); }
   inline double kt_of(int m) const { return (PI2 / LT) * double(m); }
 
-  // ------------------------------------------------- mod_radialFD.cu:422-452
+  // ------------------------------------------------- mod_radialFD.cu:365-395
   void fd_weights(double* c, const double* rau, int i0, int left, int right, int m) {
     int n = 1 + right - left;
     double c1, c2, c3, c4, c5, x0 = rau[i0 + IW];
@@ -163,9 +163,9 @@ struct Oracle {
     }
   }
 
-  // ------------------------------------------------- mod_radialFD.cu:256-369
+  // ------------------------------------------------- mod_radialFD.cu:199-312
   // Grid, parity-folded D/D2 tables, rdr quadrature, Lu/Lp.  The rdr loop of
-  // the reference runs k = 0..len inclusive (mod_radialFD.cu:329) and for
+  // the reference runs k = 0..len inclusive (mod_radialFD.cu:272) and for
   // k = len reads c[ir+len*len], past the 49 doubles it allocates; that term
   // carries dr^(len+1)/(len+1)! and is taken as zero here (loop stops at len-1).
   void init_fd() {
@@ -637,7 +637,7 @@ struct Oracle {
       }
   }
 
-  // ------------------------------------------------- mod_cublasflux.cu:297-321, 331-344
+  // ------------------------------------------------- mod_cublasflux.cu:56-80, 90-103
   // cublasDdot is stated as the sequential sum (its order is unpinned).
   double dot_rdr(const double* v) const {
     double s = 0.0;
@@ -822,7 +822,7 @@ struct Oracle {
     cplx* U3[3] = {u[0].data(), u[1].data(), u[2].data()};
     solve_common(U3, U3);
   }
-  // mod_integrate.cu:9-40 + mod_cublasflux.cu:347-357.  cublasIdamax returns
+  // mod_integrate.cu:9-40 + mod_cublasflux.cu:106-116.  cublasIdamax returns
   // the first maximum of |x|; for the non-negative entries of d1/d2 this is
   // the maximum value.  A NaN anywhere is reported as err = NaN (which ends
   // the corrector loop, as a NaN err does in the reference).
@@ -933,7 +933,7 @@ std::vector<double> grid(int nr) {
   return r;
 }
 
-// mod_remesh.cu:147-178 with m = 0 (the derivative loops are empty)
+// mod_remesh.cu:148-178 with m = 0 (the derivative loops are empty)
 void fd_weights(double* c, const double* x, double x0, int n) {
   int m = 0;
   double c1, c2, c3, c4, c5;
@@ -1175,7 +1175,7 @@ void nspo_step(void* h, int nsteps, int* n_it, double* err) {
 }
 
 // Physical-space component comp (0 ur, 1 ut, 2 uz) of the current state:
-// padForward + fftBackward (mod_io_nonlinear.cu:173-195, mod_fft.cu:43-49)
+// padForward + fftBackward (padForward_k, mod_io_nonlinear.cu:173-192; fftBackward, mod_fft.cu:45-49)
 // per radial level, as the reference leaves it in u_pad: out[NR][NTP][2*NZP]
 // doubles, NZF samples per row then the two padding doubles (zero here).
 void nspo_physical(void* h, int comp, double* out) {

@@ -22,9 +22,17 @@
 //        influence matrix, no-slip and wall divergence at round-off
 //   adj_flux                              : bulk flux == Qdes to round-off
 //   step                                  : rescaled Poiseuille is a discrete
-//        fixed point; m=0 swirl eigenvector decays by the closed-form factor
+//        fixed point; m=0 swirl eigenvector decays by the closed-form factor;
+//        at m != 0 a perturbed Poiseuille step equals a dense numpy one-mode
+//        step map at round-off, whose least-stable eigenvalue matches an
+//        independent Chebyshev-collocation eigen-solve of linearised pipe flow
 //   nonlinear                             : brute-force numpy pseudo-spectral
 //        evaluation on a tiny grid
+//   diagnostics                           : closed forms on alpha(1-r^2)
+//        (u_cl, u_tau, flux), Parseval sum of the physical samples (KE),
+//        analytic divergence and wall values of polynomial fields
+//   reference_ic                          : numpy rfft2 of the init_per
+//        samples, truncated as padBackward_k does
 // Parity unpinned (no reference run is possible): cuFFT rounding and its
 // handling of non-Hermitian kz=0 input, cublasDdot summation order, nvcc FMA
 // contraction.  All are below the 1e-12 parity bar.

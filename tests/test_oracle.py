@@ -570,3 +570,341 @@ def test_cross_sections_closed_form():
     q, u2 = o.cross()
     np.testing.assert_allclose(q, np.sum(rdr * 2 * (np.abs(a) ** 2 + np.abs(b) ** 2)), rtol=1e-13)
     np.testing.assert_allclose(u2, np.sum(rdr * 2 * np.abs(c) ** 2), rtol=1e-13)
+
+
+# ----------------------------------------------------------------- diagnostics
+def test_diagnostics_on_poiseuille_closed_form():
+    """On uz00 = alpha (1 - r^2) the centre-line velocity is alpha (dr0 is
+    exact to degree 3), u_tau = sqrt(2 alpha / Re) (the wall row of the even D
+    table is exact to degree 6), the bulk flux is alpha sum rdr (1 - r^2), the
+    field is divergence-free and no-slip (wrt_frc_k, mod_io_nonlinear.cu:40-52)."""
+    o = Oracle(**CFG1)
+    T = o.tables()
+    r, rdr = T["r"], T["rdr"]
+    alpha = 1.7
+    ur, ut, uz = o.zeros(), o.zeros(), o.zeros()
+    uz[:, 0, 0] = alpha * (1 - r ** 2)
+    o.set_state(ur, ut, uz)
+    d = o.diagnostics()
+    assert abs(d["u_cl"] - alpha) < 1e-14 * alpha
+    assert abs(d["u_tau"] - np.sqrt(2 * alpha / o.re)) < 1e-12 * np.sqrt(2 * alpha / o.re)
+    assert abs(d["flux"] - alpha * np.dot(rdr, 1 - r ** 2)) < 1e-15
+    assert d["div_l2"] < 1e-12 and d["wall_div_max"] < 1e-12
+    assert d["wall_max"] < 1e-15
+    # KE of a single radial profile: 1/2 sum rdr uz^2 (mode (0,0) has weight 1)
+    assert abs(d["ke"] - 0.5 * np.dot(rdr, (alpha * (1 - r ** 2)) ** 2)) < 1e-14
+
+
+def hermitian_field(o, rng, scale=1.0):
+    """Random spectral field whose l = 0 column is the spectrum of a real
+    function of theta (u(-m, 0) = conj u(m, 0), u(0, 0) real, no m = -NT/2),
+    so that the stored modes are exactly those of a real physical field."""
+    f = scale * (rng.standard_normal(o.shape) + 1j * rng.standard_normal(o.shape))
+    nt = o.nt
+    f[:, 0, 0] = f[:, 0, 0].real
+    f[:, nt // 2, 0] = 0.0
+    for it in range(1, nt // 2):
+        f[:, nt - it, 0] = np.conj(f[:, it, 0])
+    return f
+
+
+def test_diagnostics_kinetic_energy_is_the_volume_mean_of_physical_samples():
+    """KE = 1/2 sum_i rdr_i <u.u>(r_i), <.> the mean over the 3/2-padded
+    (theta, z) grid of the physical samples (nspo_physical, pinned by brute
+    force): Parseval gives the mode sum with weight 1 for l = 0, 2 otherwise."""
+    o = Oracle(**SMALL)
+    rng = np.random.default_rng(21)
+    f = [hermitian_field(o, rng) for _ in range(3)]
+    o.set_state(*f)
+    rdr = o.tables()["rdr"]
+    ms = 0.0
+    for c in range(3):
+        ph = o.physical(c)[:, :, :o.nzf]
+        ms = ms + (ph ** 2).mean(axis=(1, 2))
+    ke_phys = 0.5 * np.dot(rdr, ms)
+    d = o.diagnostics()
+    assert abs(d["ke"] - ke_phys) < 1e-13 * ke_phys
+
+
+def test_diagnostics_divergence_and_wall_norms_polynomial():
+    """Single-mode polynomial field with an analytic divergence: div_l2 =
+    sqrt(sum_i w_l |rdr_i| |div_i|^2) (w_l = 2 for l > 0), the wall norms are
+    the analytic wall values."""
+    o = Oracle(**CFG1)
+    T = o.tables()
+    r, rdr = T["r"], T["rdr"]
+    m, l = 1, 2
+    it, _ = mode_index(o, m, l)
+    kz = 2 * np.pi / o.lz * l
+    ur, ut, uz = o.zeros(), o.zeros(), o.zeros()
+    ur[:, it, l] = 1 - r ** 2 + 0.25 * r ** 2     # even (odd m): wall value 0.25
+    ut[:, it, l] = 0.5 * (1 - r ** 4)             # even, wall value 0
+    uz[:, it, l] = 2 * r * (1 - r ** 2)           # odd, wall value 0
+    o.set_state(ur, ut, uz)
+    d = o.diagnostics()
+    fr, dfr = 1 - 0.75 * r ** 2, -1.5 * r
+    div = dfr + fr / r + 1j * m * 0.5 * (1 - r ** 4) / r + 1j * kz * 2 * r * (1 - r ** 2)
+    want = np.sqrt(np.sum(2.0 * np.abs(rdr) * np.abs(div) ** 2))
+    assert abs(d["div_l2"] - want) < 1e-10 * want
+    assert abs(d["wall_max"] - 0.25) < 1e-14
+    assert abs(d["wall_div_max"] - abs(dfr[-1] + fr[-1])) < 1e-10
+
+
+def test_reference_ic_matches_numpy_transform_of_init_per():
+    """initField with restart == 0 (mod_io_nonlinear.cu:424-449): u_z00 = 1 - r^2
+    (base_k) plus ur, ut sampled by init_per on the NTP x NZF grid (z scaled to
+    [0, 1)), transformed by numpy's rfft2, normalised by NTP*NZF and truncated
+    as padBackward_k does (m = -NT/2 and l = NZ-1 zeroed)."""
+    o = Oracle(**SMALL)
+    A_P = 0.3
+    o.reference_ic(A_P)
+    got = o.get_state()
+    r = o.tables()["r"]
+    nt, nz, ntp, nzf = o.nt, o.nz, o.ntp, o.nzf
+    th = np.arange(ntp) / ntp * 2 * np.pi
+    z = np.arange(nzf) / nzf
+    ux = np.exp(-10.0 * np.sin(np.pi * z) ** 2)
+    rows = [it if it < nt // 2 else ntp + (it - nt) for it in range(nt)]
+    for i in range(o.nr):
+        a = (1 - r[i] ** 2) ** 2
+        b = a - 4 * r[i] ** 2 * (1 - r[i] ** 2)
+        for c, prof in ((0, a * np.outer(np.sin(th), ux)), (1, b * np.outer(np.cos(th), ux))):
+            S = np.fft.rfft2(A_P * prof) / (ntp * nzf)
+            want = S[rows, :nz]
+            want[nt // 2, :] = 0.0
+            want[:, nz - 1] = 0.0
+            assert np.abs(got[c][i] - want).max() < 1e-15, (c, i)
+    want_z = o.zeros()
+    want_z[:, 0, 0] = 1 - r ** 2
+    assert np.array_equal(got[2], want_z)
+
+
+# --------------------------------------------- whole step at m != 0: dense one-mode map
+def dense_step_map(o, T, m, l, alpha, n_corr):
+    """The time step of mod_integrate.cu:164-228, linearised about the
+    Poiseuille profile U = alpha (1 - r^2) and restricted to one Fourier mode
+    (m, l), assembled as a dense 3NR x 3NR matrix from the radial tables.
+
+    Built from the operators' definitions, not from the oracle's code: the
+    linearised convective term -(U d_z u + u_r U' e_z) (products of mode (0,0)
+    with mode (m, l) land in (m, l) exactly on the 3/2 grid; U' = D U),
+    addLap_k's explicit part, div_k, the Neumann pressure solve, addGrad_k,
+    the +/- decoupling, the Dirichlet Helmholtz solves (numpy.linalg.solve),
+    and the wall correction as the 4x4 linear system (numpy) that makes the
+    wall velocity and the wall divergence zero with the homogeneous solutions
+    of setBCon (mod_boundary.cu:205-255): unit-wall Helmholtz solutions and the
+    gradient of the unit-flux pressure solution."""
+    nr = o.nr
+    r, D, Lu = T["r"], T["D"], T["Lu"]
+    d_im, re, dt = 0.51, o.re, o.dt
+    kz = 2 * np.pi / o.lz * l
+    I = np.eye(nr)
+
+    def band(tab, p):
+        A = np.zeros((nr, nr))
+        for i in range(nr):
+            left, right, i0 = stencil(nr, i)
+            for j in range(right + 1 - left):
+                A[i, left + j] = tab[p, i, j + i0]
+        return A
+
+    pu = abs(abs(m) % 2 - 1)   # parity table of ur, ut, u+-
+    pp = abs(abs(m) % 2)       # parity table of uz, p
+    Dr_u, Dr_p = band(D, pu), band(D, pp)
+    U = alpha * (1 - r ** 2)
+    dU = band(D, 0) @ U
+    Z = np.zeros((nr, nr))
+    # linearised N(u) on (ur, ut, uz) stacked
+    NL = np.block([[np.diag(-1j * kz * U), Z, Z],
+                   [Z, np.diag(-1j * kz * U), Z],
+                   [np.diag(-dU), Z, np.diag(-1j * kz * U)]])
+    # r/theta <-> +/- (decoupleForward_k / decoupleBackward_k)
+    P = np.block([[I, 1j * I, Z], [I, -1j * I, Z], [Z, Z, I]])
+    Pi = np.linalg.inv(P)
+
+    def diag_term(kind):
+        de = -kz * kz - m * m / r ** 2
+        if kind == 0:
+            de = de - 1 / r ** 2 - 2 * m / r ** 2
+        elif kind == 1:
+            de = de - 1 / r ** 2 + 2 * m / r ** 2
+        return de
+
+    def lap(kind):
+        return band(Lu, pu if kind < 2 else pp) + np.diag(diag_term(kind))
+
+    Lex = [I / dt + (1 - d_im) / re * lap(c) for c in range(3)]   # addLap_k, every row
+    H = []
+    for c in range(3):   # LUsolve_u*_k: (1/dt) I - (d_im/Re)(Lu + diag), wall row identity
+        A = I / dt - d_im / re * lap(c)
+        A[-1] = 0.0
+        A[-1, -1] = 1.0
+        H.append(A)
+    Lp = band(T["Lp"], pp).astype(complex)
+    Lp[:-1] += np.diag(-kz * kz - m * m / r ** 2)[:-1]   # Neumann wall row D p = rhs
+    Div = np.hstack([Dr_u + np.diag(1 / r), np.diag(1j * m / r), 1j * kz * I])   # div_k
+    Grad = np.vstack([Dr_p, np.diag(1j * m / r), 1j * kz * I])                    # addGrad_k
+    e = np.zeros(nr); e[-1] = 1.0
+    # homogeneous solutions (setBCon)
+    hp = np.linalg.solve(H[0], e)
+    hm = np.linalg.solve(H[1], e)
+    hz = np.linalg.solve(H[2], 1j * e)
+    ph = np.linalg.solve(Lp, -e)
+    hP = P @ (Grad @ ph)
+    h = [np.concatenate([hp, 0 * e, 0 * e]), np.concatenate([0 * e, hm, 0 * e]),
+         np.concatenate([0 * e, 0 * e, hz]), hP]
+
+    def divwall(x):  # divWall_k on +/- stacked input
+        y = Pi @ x
+        urw, utw, uzw = y[:nr], y[nr:2 * nr], y[2 * nr:]
+        return Dr_u[-1] @ urw + urw[-1] + 1j * m * utw[-1] + 1j * kz * uzw[-1]
+
+    def solve(rhs_pm):  # steps 4-10 of predictor()/corrector(), +/- in and out
+        rt = Pi @ rhs_pm
+        dv = Div @ rt
+        dv[-1] = 0.0
+        p = np.linalg.solve(Lp, dv)
+        rt = rt - Grad @ p
+        b = P @ rt
+        for c in range(3):
+            b[(c + 1) * nr - 1] = 0.0
+        x = np.concatenate([np.linalg.solve(H[c], b[c * nr:(c + 1) * nr]) for c in range(3)])
+        # a0 h+ + a1 h- + a2 hz + a3 hP: zero wall velocity, zero wall divergence
+        # (x holds one column per input vector)
+        M = np.zeros((4, 4), complex)
+        rhs4 = np.zeros((4,) + x.shape[1:], complex)
+        for c in range(3):
+            M[c] = [hj[(c + 1) * nr - 1] for hj in h]
+            rhs4[c] = -x[(c + 1) * nr - 1]
+        M[3] = [divwall(hj) for hj in h]
+        rhs4[3] = -divwall(x)
+        a = np.linalg.solve(M, rhs4)
+        return x + sum(np.multiply.outer(h[j], a[j]) for j in range(4))
+
+    def explicit(u_pm):
+        return np.concatenate([Lex[c] @ u_pm[c * nr:(c + 1) * nr] for c in range(3)])
+
+    # G maps u^n (r/theta/z stacked) to u^{n+1}
+    Nn = P @ NL                                     # N(u^n), +/-
+    G = Pi @ solve(Nn + explicit(P))                # predictor
+    for _ in range(n_corr):
+        G = Pi @ solve(d_im * (P @ NL @ G) + (1 - d_im) * Nn + explicit(P))
+    return G
+
+
+@pytest.mark.parametrize("m,l", [(1, 1), (2, 3), (-3, 2)])
+def test_step_at_nonzero_mode_matches_dense_linear_map(m, l):
+    """One oracle step of Poiseuille plus a 1e-6 perturbation in mode (m, l):
+    the mode's coefficients equal the dense linearised step map applied to the
+    perturbation, to round-off (the quadratic terms are 1e-12 of it and fall
+    in other modes)."""
+    o = Oracle(nr=24, nt=8, nz=9, lz=2 * np.pi, re=600.0, dt=0.01)
+    T = o.tables()
+    r, rdr = T["r"], T["rdr"]
+    alpha = 0.5 / np.dot(rdr, 1 - r ** 2)
+    rng = np.random.default_rng(7 + l)
+    it, _ = mode_index(o, m, l)
+    # a divergence-free-ish no-slip perturbation of the mode's parities
+    pu = abs(abs(m) % 2 - 1)
+    v = [(rng.standard_normal(6) + 1j * rng.standard_normal(6)) for _ in range(3)]
+    prof = lambda c, par: (1 - r ** 2) * sum(c[q] * r ** (2 * q + par) for q in range(6))
+    eps = 1e-6
+    ur, ut, uz = o.zeros(), o.zeros(), o.zeros()
+    ur[:, it, l] = eps * prof(v[0], pu)
+    ut[:, it, l] = eps * prof(v[1], pu)
+    uz[:, it, l] = eps * prof(v[2], 1 - pu)
+    uz[:, 0, 0] = alpha * (1 - r ** 2)
+    o.set_state(ur, ut, uz)
+    x0 = np.concatenate([ur[:, it, l], ut[:, it, l], uz[:, it, l]])
+    n_it, _ = o.step(1)
+    assert n_it[0] == 1
+    a, b, c = o.get_state()
+    got = np.concatenate([a[:, it, l], b[:, it, l], c[:, it, l]])
+    G = dense_step_map(o, T, m, l, alpha, int(n_it[0]))
+    want = G @ x0
+    assert np.linalg.norm(got - want) < 1e-11 * np.linalg.norm(want)
+    # every other fluctuation mode stays at the quadratic level
+    for f in (a, b, c):
+        g = f.copy()
+        g[:, it, l] = 0.0
+        g[:, 0, 0] = 0.0
+        assert np.abs(g).max() < 10 * eps ** 2
+
+
+def chebyshev_pipe_eigenvalues(m, k, re, alpha, n):
+    """Eigenvalues of pipe flow linearised about U = alpha (1 - r^2), mode
+    exp(i(m theta + k z) + lambda t), by Chebyshev collocation in primitive
+    variables (u_r, u_theta, u_z, p) on the n+1 positive points of an odd
+    2n+1-point Chebyshev grid on [-1, 1], folded with each field's parity
+    (u_r, u_theta of parity m+1, u_z and p of parity m).  Momentum at the
+    interior points, no-slip at r = 1, continuity at every point; a generalised
+    eigenproblem solved with scipy.  Independent of the finite-difference
+    tables."""
+    import scipy.linalg
+    N = 2 * n + 1
+    x = np.cos(np.pi * np.arange(N + 1) / N)
+    c = np.ones(N + 1); c[0] = c[-1] = 2.0
+    c = c * (-1.0) ** np.arange(N + 1)
+    X = np.tile(x, (N + 1, 1)).T
+    dX = X - X.T
+    Dm = np.outer(c, 1 / c) / (dX + np.eye(N + 1))
+    Dm = Dm - np.diag(Dm.sum(axis=1))
+    D2 = Dm @ Dm
+    h = n + 1
+    idx, mir = np.arange(h), N - np.arange(h)   # x[N - j] = -x[j]
+
+    def fold(A, par):
+        s = 1.0 if par == 0 else -1.0
+        return A[np.ix_(idx, idx)] + s * A[np.ix_(idx, mir)]
+
+    r = x[:h]
+    pv, ps = (m + 1) % 2, m % 2
+    D1v, D2v, D1s, D2s = fold(Dm, pv), fold(D2, pv), fold(Dm, ps), fold(D2, ps)
+    R, Ri, Ri2 = np.diag(r), np.diag(1 / r), np.diag(1 / r ** 2)
+    U = np.diag(alpha * (1 - r ** 2))
+    dU = np.diag(-2 * alpha * r)
+    I, Z = np.eye(h), np.zeros((h, h))
+    lapv = D2v + Ri @ D1v - (m * m) * Ri2 - k * k * I
+    laps = D2s + Ri @ D1s - (m * m) * Ri2 - k * k * I
+    nu = 1.0 / re
+    Arr = -1j * k * U + nu * (lapv - Ri2)
+    Art = nu * (-2j * m * Ri2)
+    Atr = nu * (2j * m * Ri2)
+    Azz = -1j * k * U + nu * laps
+    A = np.block([[Arr, Art, Z, -D1s],
+                  [Atr, Arr, Z, -1j * m * Ri],
+                  [-dU, Z, Azz, -1j * k * I],
+                  [D1v + Ri, 1j * m * Ri, 1j * k * I, Z]]).astype(complex)
+    B = np.block([[I, Z, Z, Z], [Z, I, Z, Z], [Z, Z, I, Z], [Z, Z, Z, Z]]).astype(complex)
+    for q in range(3):   # no-slip rows at r = 1 (point 0)
+        row = q * h
+        A[row] = 0.0
+        B[row] = 0.0
+        A[row, q * h] = 1.0
+    lam = scipy.linalg.eig(A, B, right=False)
+    return lam[np.isfinite(lam) & (np.abs(lam) < 1e3)]
+
+
+def test_step_map_least_stable_mode_matches_chebyshev_eigenvalue():
+    """The dense step map (pinned above to the oracle's step at round-off)
+    amplifies its least-stable mode by g per step; log(g)/dt matches the
+    least-stable eigenvalue of the continuous linearised problem from an
+    independent Chebyshev-collocation solve, to the discretisation accuracy of
+    the finite differences (NR = 48) and of the time scheme."""
+    o = Oracle(nr=48, nt=8, nz=9, lz=2 * np.pi, re=500.0, dt=0.002)
+    T = o.tables()
+    r, rdr = T["r"], T["rdr"]
+    alpha = 0.5 / np.dot(rdr, 1 - r ** 2)
+    m, l = 1, 1
+    G = dense_step_map(o, T, m, l, alpha, 1)
+    g = np.linalg.eigvals(G)
+    g = g[np.argmax(np.abs(g))]
+    lam_fd = np.log(g) / o.dt
+    k = 2 * np.pi / o.lz * l
+    lam = chebyshev_pipe_eigenvalues(m, k, o.re, alpha, 40)
+    lam2 = chebyshev_pipe_eigenvalues(m, k, o.re, alpha, 56)
+    lead = lam[np.argmax(lam.real)]
+    lead2 = lam2[np.argmax(lam2.real)]
+    assert abs(lead - lead2) < 1e-8 * abs(lead)        # collocation converged
+    assert lead.real < 0                                 # linearly stable pipe flow
+    assert abs(lam_fd - lead) < 1e-4 * abs(lead), (lam_fd, lead)

@@ -136,8 +136,9 @@ nsp_status nsp_write_state_files(nsp_handle* h, const char* ur_path, const char*
  * time, step, nr, nt, nz, lz, re, dt, d_im, maxit, tol, qdes, precision).
  * nsp_read_restart reads them back into the state and returns time and step;
  * it fails with NSP_ERR_INVALID if the metadata's grid differs from the
- * handle's.  A directory holding only the three raw files (a reference
- * restart) reads with time 0 and step 0.  t and step may be NULL. */
+ * handle's and with NSP_ERR_IO if restart.txt is missing (the raw files alone
+ * carry no grid: read a reference restart with nsp_read_state_files).  t and
+ * step may be NULL. */
 nsp_status nsp_write_restart(nsp_handle* h, const char* dir, double t, long long step);
 nsp_status nsp_read_restart(nsp_handle* h, const char* dir, double* t, long long* step);
 

@@ -1177,7 +1177,12 @@ nsp_status nsp_read_restart(nsp_handle* h, const char* dir, double* t, long long
   double tt = 0.0;
   long long ss = 0;
   const std::string path = d + "/" + kRestartMeta;
-  if (FILE* f = std::fopen(path.c_str(), "r")) {
+  // The metadata is required: the raw files alone carry no grid, and another
+  // grid with the same NR*NT*NZ would read without error.  Headerless
+  // reference restarts go through nsp_read_state_files.
+  FILE* f = std::fopen(path.c_str(), "r");
+  if (!f) return fail(NSP_ERR_IO, "missing restart metadata " + path + " (raw reference files: nsp_read_state_files)");
+  {
     char key[64];
     char val[128];
     int nr = -1, nt = -1, nz = -1;

@@ -150,6 +150,7 @@ def test_restart_directory_round_trip(tmp_path):
     p.seeded_ic(9, 3, 3, nsp.IC_FLAT, 0.05)
     p.step(3)
     p.write_restart(tmp_path, t=0.03, step=3)
+    saved = p.get_state()
     meta = dict(line.split() for line in open(tmp_path / "restart.txt"))
     assert meta["format"] == "nspipe-restart-1" and int(meta["nr"]) == shape["nr"] and float(meta["time"]) == 0.03
     q = nsp.Pipe(**shape)
@@ -161,9 +162,14 @@ def test_restart_directory_round_trip(tmp_path):
     q.step(2)
     for a, b in zip(p.get_state(), q.get_state()):
         assert np.array_equal(a, b)
-    # the raw reference files alone read with time 0, step 0
+    # without the metadata the directory is refused (the raw files carry no
+    # grid); they still read as plain reference state files
     (tmp_path / "restart.txt").unlink()
-    assert q.read_restart(tmp_path) == (0.0, 0)
+    with pytest.raises(nsp.NspError):
+        q.read_restart(tmp_path)
+    q.read_state_files(*(str(tmp_path / f"u{c}.bin") for c in "rtz"))
+    for a, b in zip(saved, q.get_state()):
+        assert np.array_equal(a, b)
     # another grid is refused
     p.write_restart(tmp_path, t=1.0, step=100)
     r = nsp.Pipe(shape["nr"] + 2, shape["nt"], shape["nz"], shape["lz"], shape["re"], shape["dt"])

@@ -278,18 +278,21 @@ def profiles(pipe):
 
 def test_config2_mixed_against_fp64_long_run():
     """Config 2 (NR=64, NT=192, NZ=385, Re 5300, dt 0.002, k^-5/3 noise of rms
-    0.1 U_bulk): the mixed form against the FP64 form on the GPU over 500
-    steps.  Mean profile, rms profiles, kinetic energy and wall shear stay
-    within 1e-5 (the mixed form's error budget)."""
+    0.1 U_bulk): the mixed form against the FP64 form on the GPU over the
+    config's 2000 steps.  Mean profile, rms profiles, kinetic energy,
+    fluctuation kinetic energy and wall shear stay within 1e-5 (the mixed
+    form's error budget)."""
     cfg = dict(nr=64, nt=192, nz=385, lz=2 * np.pi, re=5300.0, dt=0.002)
-    print("[gpu] config 2, 500 steps, fp64 and mixed", flush=True)
+    print("[gpu] config 2, 2000 steps, fp64 and mixed", flush=True)
     out = {}
     for prec in (nsp.FP64, nsp.MIXED):
         p = nsp.Pipe(**cfg, precision=prec)
         p.seeded_ic(20261202 + 2 * 1000, 4, 4, nsp.IC_KOLMOGOROV, 0.05)
-        n_it, _, flags = p.step(500)
+        n_it, _, flags = p.step(2000)
         assert np.all(flags == 0)
-        out[prec] = (profiles(p), p.diagnostics(), n_it)
+        d = p.diagnostics()
+        d["fke"] = fluct_ke(p)
+        out[prec] = (profiles(p), d, n_it)
         p.close()
     (m64, r64), d64, n64 = out[nsp.FP64]
     (m32, r32), d32, n32 = out[nsp.MIXED]
@@ -297,9 +300,10 @@ def test_config2_mixed_against_fp64_long_run():
     e_rms = np.abs(r32 - r64).max() / np.abs(r64).max()
     e_ke = abs(d32["ke"] - d64["ke"]) / d64["ke"]
     e_tau = abs(d32["u_tau"] - d64["u_tau"]) / d64["u_tau"]
-    print(f"[gpu] config 2 mixed vs fp64 after 500 steps: mean {e_mean:.3e} rms {e_rms:.3e} "
-          f"ke {e_ke:.3e} u_tau {e_tau:.3e}; correctors {n64.sum()} / {n32.sum()}", flush=True)
-    assert max(e_mean, e_rms, e_ke, e_tau) <= 1e-5
+    e_fke = abs(d32["fke"] - d64["fke"]) / d64["fke"]
+    print(f"[gpu] config 2 mixed vs fp64 after 2000 steps: mean {e_mean:.3e} rms {e_rms:.3e} "
+          f"ke {e_ke:.3e} fke {e_fke:.3e} u_tau {e_tau:.3e}; correctors {n64.sum()} / {n32.sum()}", flush=True)
+    assert max(e_mean, e_rms, e_ke, e_fke, e_tau) <= 1e-5
 
 
 def test_config5_mixed_steps():
@@ -434,7 +438,9 @@ def test_mixed_hundred_steps_cfg1_error_budget():
     n_p, _, f_p = p.step(100)
     n_o, _ = o.step(100)
     dp, do = p.diagnostics(), o.diagnostics()
-    for key in ("ke", "u_tau", "u_cl"):
+    dp["fke"] = fluct_ke(p)
+    do["fke"] = fluct_ke(o)
+    for key in ("ke", "fke", "u_tau", "u_cl"):
         e = abs(dp[key] - do[key]) / abs(do[key])
         print(f"[gpu] mixed 100-step {key} rel err {e:.3e}", flush=True)
         assert e <= 1e-5, key
@@ -507,3 +513,48 @@ def test_maxit_is_flagged():
     n_it, err, flags = p.step(1)
     assert n_it[0] == 3 and flags[0] & nsp.FLAG_MAXIT
     p.close()
+
+
+def fluct_ke(pipe):
+    """Kinetic energy of the fluctuations (every mode but (0, 0)):
+    1/2 sum w_l sum_r rdr |u|^2 with w_0 = 1, w_l = 2 otherwise."""
+    rdr = pipe.tables()["rdr"]
+    w = np.full(pipe.nz, 2.0)
+    w[0] = 1.0
+    tot = 0.0
+    for x in pipe.get_state():
+        a = (np.abs(x) ** 2 * w).sum(axis=2)
+        a[:, 0] -= np.abs(x[:, 0, 0]) ** 2
+        tot += 0.5 * np.dot(rdr, a.sum(axis=1))
+    return tot
+
+
+def test_config3_mixed_against_fp64_hundred_steps():
+    """Headline configuration (NR=96, NT=384, NZ=1537, Re 11700, L 10 pi R,
+    dt 0.001, seeded noise of rms 0.15 U_bulk): the mixed form against the FP64
+    form on the GPU after 100 steps.  Kinetic energy, fluctuation kinetic
+    energy, wall shear (u_tau), centre-line velocity within 1e-5 relative;
+    divergence and wall residuals at the FP64 form's level; identical
+    corrector counts."""
+    cfg = dict(nr=96, nt=384, nz=1537, lz=10 * np.pi, re=11700.0, dt=0.001)
+    out = {}
+    for prec in (nsp.FP64, nsp.MIXED):
+        print(f"[gpu] config 3, 100 steps, precision {prec}", flush=True)
+        p = nsp.Pipe(**cfg, precision=prec)
+        p.seeded_ic(20261202 + 3 * 1000, 4, 4, 0, 0.15 * 0.5)
+        n_it, _, flags = p.step(100)
+        assert np.all(flags == 0)
+        d = p.diagnostics()
+        d["fke"] = fluct_ke(p)
+        out[prec] = (d, n_it)
+        p.close()
+    (d64, n64), (d32, n32) = out[nsp.FP64], out[nsp.MIXED]
+    errs = {k: abs(d32[k] - d64[k]) / abs(d64[k]) for k in ("ke", "fke", "u_tau", "u_cl")}
+    print("[gpu] config 3 mixed vs fp64 after 100 steps: " +
+          " ".join(f"{k} {v:.3e}" for k, v in errs.items()) +
+          f"; div {d32['div_l2']:.3e} / {d64['div_l2']:.3e}; wall {d32['wall_max']:.1e} / {d64['wall_max']:.1e}",
+          flush=True)
+    assert np.array_equal(n64, n32)
+    assert max(errs.values()) <= 1e-5, errs
+    assert d32["wall_max"] < 1e-12 and d64["wall_max"] < 1e-12
+    assert abs(d32["div_l2"] - d64["div_l2"]) <= 1e-3 * d64["div_l2"] + 1e-14

@@ -138,6 +138,9 @@ struct Oracle {
   std::vector<double> d1, d2;
   FFT fft_t, fft_z;
   int nthreads = 1;
+  // > 0: nonlinear() evaluates only levels 0..sample_levels-1 (the others are
+  // left untouched).  Used only to time bounded samples of the CPU baseline.
+  int sample_levels = 0;
 
   // ------------------------------------------------------------------ index
   inline int mode_m(int k) const { int it = k / NZ; return it < NT / 2 ? it : it - NT; }
@@ -747,8 +750,9 @@ struct Oracle {
     {
       std::vector<cplx> lvl(MODES), pad((size_t)NTP * NZP), tmp, col, full;
       std::vector<double> ph(12 * (size_t)NTP * NZF);
+      const int nlev = sample_levels > 0 ? std::min(sample_levels, NR) : NR;
 #pragma omp for schedule(dynamic, 1)
-      for (int i = 0; i < NR; i++) {
+      for (int i = 0; i < nlev; i++) {
         double rad = r[i];
         // fields: 0-2 u, 3-5 dr u, 6-8 dt u, 9-11 dz u
         for (int f = 0; f < 12; f++) {
@@ -1057,6 +1061,9 @@ void nspo_set_threads(int n) {
 #endif
   (void)n;
 }
+// Timing samples only: nonlinear() evaluates levels 0..n-1 (n <= 0: all).
+void nspo_set_sample_levels(void* h, int n) { ((Oracle*)h)->sample_levels = n; }
+
 int nspo_max_threads(void) {
 #ifdef _OPENMP
   return omp_get_max_threads();

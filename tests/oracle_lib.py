@@ -29,6 +29,7 @@ def lib():
         L.nspo_destroy.argtypes = [vp]
         L.nspo_set_threads.argtypes = [ip]
         L.nspo_max_threads.restype = ip
+        L.nspo_set_sample_levels.argtypes = [vp, ip]
         L.nspo_tables.argtypes = [vp, P, P, P, P, P, P, P]
         L.nspo_set_state.argtypes = [vp, P, P, P]
         L.nspo_get_state.argtypes = [vp, P, P, P]
@@ -103,6 +104,10 @@ class Oracle:
         a = [self.zeros() for _ in range(3)]
         lib().nspo_get_state(self.h, *[_p(x) for x in a])
         return a
+
+    def set_sample_levels(self, n):
+        """Timing samples only: nonlinear() evaluates levels 0..n-1 (0: all)."""
+        lib().nspo_set_sample_levels(self.h, n)
 
     def reference_ic(self, a_p=0.3):
         lib().nspo_reference_ic(self.h, a_p)

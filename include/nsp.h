@@ -110,6 +110,21 @@ nsp_status nsp_nonlinear(nsp_handle* h, const double* ur, const double* ut, cons
  * out[5] max |wall divergence|, out[6] bulk flux sum rdr Re uz00. */
 nsp_status nsp_diagnostics(nsp_handle* h, double out[7]);
 
+/* Allocation-free diagnostics.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same seven
+ * values as nsp_diagnostics, written to out: 7 doubles in DEVICE memory.  The
+ * call only enqueues two kernels on the handle's stream (nsp_stream): it
+ * allocates nothing, does not synchronise (wait on the stream before reading
+ * out) and uses no memset.  The reduction's block partials live in the
+ * library's static per-device scratch array (512 KiB, held until the process
+ * ends), which limits NT*NZ to 4,194,304 modes: larger grids fail with
+ * NSP_ERR_INVALID.  Every call on a device shares that device's array, so
+ * calls that overlap on one device (several streams, threads or handles at
+ * once) race.  Sums are combined in another order than nsp_diagnostics', so
+ * KE and the divergence norm agree to rounding, not bit for bit. */
+nsp_status nsp_diagnostics_noalloc(nsp_handle* h, double* out);
+
 /* Radial tables (init_fd, mod_radialFD.cu:199-312; get_dr0 :316-350;
  * get_dr1 :355-358): r[NR], D[2][NR][7], Lu[2][NR][7], Lp[2][NR][7],
  * rdr[NR], dr0[4], dr1[7].  Any pointer may be NULL. */

@@ -280,6 +280,9 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(NSP_ERR_HIP, "no HIP device");
   if (device < 0 || device >= ndev) return fail(NSP_ERR_INVALID, "device index out of range");
   HIPCHK(hipSetDevice(device));
+  // the device's address of the allocation-free diagnostics' scratch array
+  // (once per device, with the device current)
+  HIPCHK(diag_scratch_init(device));
 
   nsp_handle* h = new nsp_handle();
   h->P = *p;
@@ -1113,6 +1116,24 @@ nsp_status nsp_diagnostics(nsp_handle* h, double out[7]) {
   out[4] = wall;
   out[5] = wdiv;
   out[6] = q;
+  return NSP_OK;
+}
+
+// Same values as nsp_diagnostics, reduced entirely on the device into out
+// (device memory): no allocation, no host synchronisation, no memset (every
+// block writes its own partial set in the static scratch array).
+nsp_status nsp_diagnostics_noalloc(nsp_handle* h, double* out) {
+  if (!h || !out) return fail(NSP_ERR_INVALID, "null argument");
+  const Geom& g = h->g;
+  if (g.modes > kDiagMaxModes)
+    return fail(NSP_ERR_INVALID, "nsp_diagnostics_noalloc: NT*NZ = " + std::to_string(g.modes) +
+                                     " exceeds its scratch limit of " + std::to_string(kDiagMaxModes) +
+                                     " modes (use nsp_diagnostics)");
+  HIPCHK(hipSetDevice(h->device));
+  double* part = diag_scratch(h->device);
+  if (!part) return fail(NSP_ERR_HIP, "diagnostics scratch not initialised on this device");
+  launch_diagnostics_noalloc(g, h->t, h->u, part, out, h->stream);
+  HIPCHK(hipGetLastError());
   return NSP_OK;
 }
 

@@ -11,6 +11,8 @@
 //   * diag_kernel: per-mode partial sums of the diagnostics.
 // Column vectors of the set-up live in mode-fastest global arrays, so every
 // access of a wavefront is one contiguous 1 KiB (FP64 complex) line.
+#include <mutex>
+
 #include "nsp_internal.h"
 
 namespace nsp {
@@ -391,9 +393,7 @@ __global__ __launch_bounds__(256) void setup_influence_kernel(Geom g, DevTables 
 // ------------------------------------------------------------------ diagnostics
 // Per-mode partial sums: [0] KE, [1] |rdr|-weighted |div|^2, [2] max wall |u|,
 // [3] |wall divergence|.
-__global__ __launch_bounds__(256) void diag_kernel(Geom g, DevTables t, const double2* u, double* part) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.modes) return;
+__device__ void diag_mode(const Geom& g, const DevTables& t, const double2* u, int k, double* out) {
   const Mode q = mode_of(g, k);
   const size_t n = g.n;
   const double w = (q.l == 0) ? 1.0 : 2.0;
@@ -422,10 +422,108 @@ __global__ __launch_bounds__(256) void diag_kernel(Geom g, DevTables t, const do
     wall = fmax(wall, sqrt(a.x * a.x + a.y * a.y));
   }
   const double2 dw = div_wall(g, t, q, u, u + n, u + 2 * n);
-  part[(size_t)k * 4 + 0] = ke;
-  part[(size_t)k * 4 + 1] = d2;
-  part[(size_t)k * 4 + 2] = wall;
-  part[(size_t)k * 4 + 3] = sqrt(dw.x * dw.x + dw.y * dw.y);
+  out[0] = ke;
+  out[1] = d2;
+  out[2] = wall;
+  out[3] = sqrt(dw.x * dw.x + dw.y * dw.y);
+}
+
+__global__ __launch_bounds__(256) void diag_kernel(Geom g, DevTables t, const double2* u, double* part) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.modes) return;
+  diag_mode(g, t, u, k, part + (size_t)k * 4);
+}
+
+// Allocation-free diagnostics: the per-mode partials of a block of 256 modes
+// are combined in the block (sums of [0], [1], maxima of [2], [3]) into one
+// partial set per block, part[4 * blockIdx.x + j]; a single block then
+// combines the block partials and adds the values of the mode (0,0) column
+// (centre-line velocity, friction velocity, bulk flux: wrt_frc_k,
+// mod_io_nonlinear.cu:40-52).  Every block writes its own partial set, so
+// nothing has to start at zero.
+constexpr int kDiagThreads = 256;
+__device__ __forceinline__ void block_combine(double (*sm)[kDiagThreads], int tid) {
+  for (int s = kDiagThreads / 2; s > 0; s >>= 1) {
+    if (tid < s) {
+      sm[0][tid] += sm[0][tid + s];
+      sm[1][tid] += sm[1][tid + s];
+      sm[2][tid] = fmax(sm[2][tid], sm[2][tid + s]);
+      sm[3][tid] = fmax(sm[3][tid], sm[3][tid + s]);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ __launch_bounds__(kDiagThreads) void diag_block_kernel(Geom g, DevTables t, const double2* u,
+                                                                   double* part) {
+  __shared__ double sm[4][kDiagThreads];
+  const int tid = threadIdx.x;
+  const int k = blockIdx.x * kDiagThreads + tid;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  if (k < g.modes) diag_mode(g, t, u, k, v);
+#pragma unroll
+  for (int j = 0; j < 4; j++) sm[j][tid] = v[j];
+  __syncthreads();
+  block_combine(sm, tid);
+  if (tid < 4) part[4 * (size_t)blockIdx.x + tid] = sm[tid][0];
+}
+
+__global__ __launch_bounds__(kDiagThreads) void diag_final_kernel(Geom g, DevTables t, const double2* u,
+                                                                   const double* part, int nblk, double* out) {
+  __shared__ double sm[4][kDiagThreads];
+  const int tid = threadIdx.x;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int b = tid; b < nblk; b += kDiagThreads) {
+    v[0] += part[4 * (size_t)b];
+    v[1] += part[4 * (size_t)b + 1];
+    v[2] = fmax(v[2], part[4 * (size_t)b + 2]);
+    v[3] = fmax(v[3], part[4 * (size_t)b + 3]);
+  }
+#pragma unroll
+  for (int j = 0; j < 4; j++) sm[j][tid] = v[j];
+  __syncthreads();
+  block_combine(sm, tid);
+  if (tid == 0) {
+    const double2* uz = u + 2 * g.n;  // mode (0,0) of uz at level i: uz[i * modes]
+    double uc = 0.0, du = 0.0, q = 0.0;
+    for (int i = 0; i < IW + 1; i++) uc += uz[(size_t)i * g.modes].x * t.dr0[i];
+    for (int i = 0; i < STEN; i++) du += uz[(size_t)(g.nr - STEN + i) * g.modes].x * t.dr1[i];
+    for (int i = 0; i < g.nr; i++) q += t.rdr[i] * uz[(size_t)i * g.modes].x;
+    out[0] = uc;
+    out[1] = sqrt(fabs(du) / g.re);
+    out[2] = sm[0][0];
+    out[3] = sqrt(sm[1][0]);
+    out[4] = sm[2][0];
+    out[5] = sm[3][0];
+    out[6] = q;
+  }
+}
+
+// The block partials of the allocation-free diagnostics live in this
+// file-scope array, one copy per device, held until the process ends.  Host
+// code never passes the array's name to a kernel (in host code it is only a
+// stand-in for the symbol): diag_scratch_init obtains each device's address
+// once, with that device current, and launchers use that address.
+static __device__ __align__(256) double scratch[4 * kDiagMaxBlocks];
+constexpr int kMaxDevices = 64;
+static double* g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+hipError_t diag_scratch_init(int device) {
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return hipSuccess;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess) return e;
+  g_scratch_ptr[device] = static_cast<double*>(ptr);
+  return hipSuccess;
+}
+
+double* diag_scratch(int device) {
+  if (device < 0 || device >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return g_scratch_ptr[device];
 }
 
 // ------------------------------------------------------------------ launchers
@@ -462,6 +560,12 @@ void launch_setup_influence(const Geom& g, const DevTables& t, const DevInfluenc
 }
 void launch_diagnostics(const Geom& g, const DevTables& t, const double2* u, double* part, hipStream_t s) {
   diag_kernel<<<(g.modes + 255) / 256, 256, 0, s>>>(g, t, u, part);
+}
+void launch_diagnostics_noalloc(const Geom& g, const DevTables& t, const double2* u, double* part, double* out,
+                                hipStream_t s) {
+  const int nblk = (g.modes + kDiagThreads - 1) / kDiagThreads;  // <= kDiagMaxBlocks (checked by the caller)
+  diag_block_kernel<<<nblk, kDiagThreads, 0, s>>>(g, t, u, part);
+  diag_final_kernel<<<1, kDiagThreads, 0, s>>>(g, t, u, part, nblk, out);
 }
 
 }  // namespace nsp

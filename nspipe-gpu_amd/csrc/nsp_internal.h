@@ -387,5 +387,13 @@ void launch_setup_influence(const Geom& g, const DevTables& t, const DevInfluenc
                             double2* scratch_p, double* scratch_U, double* PH, hipStream_t s);
 void launch_diagnostics(const Geom& g, const DevTables& t, const double2* u, double* partial,
                         hipStream_t s);
+// Allocation-free diagnostics (nsp_diagnostics_noalloc): block partials of at
+// most kDiagMaxBlocks blocks of 256 modes in a static per-device array.
+constexpr int kDiagMaxBlocks = 16384;
+constexpr int kDiagMaxModes = kDiagMaxBlocks * 256;
+hipError_t diag_scratch_init(int device);  // once per device, that device current
+double* diag_scratch(int device);          // its address on that device (null before init)
+void launch_diagnostics_noalloc(const Geom& g, const DevTables& t, const double2* u, double* part, double* out,
+                                hipStream_t s);
 
 }  // namespace nsp

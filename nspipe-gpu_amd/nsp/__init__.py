@@ -47,7 +47,8 @@ EXPORTS = ["nsp_default_params", "nsp_create", "nsp_destroy", "nsp_set_state", "
            "nsp_last_error", "nsp_ensemble_create", "nsp_ensemble_destroy", "nsp_ensemble_size",
            "nsp_ensemble_member", "nsp_ensemble_autotune", "nsp_ensemble_step", "nsp_remesh",
            "nsp_remesh_weights", "nsp_physical_field", "nsp_cross_sections", "nsp_mean_profile",
-           "nsp_io_open", "nsp_io_write", "nsp_io_close", "nsp_write_restart", "nsp_read_restart"]
+           "nsp_io_open", "nsp_io_write", "nsp_io_close", "nsp_write_restart", "nsp_read_restart",
+           "nsp_diagnostics_noalloc"]
 
 
 class NspError(RuntimeError):
@@ -74,6 +75,7 @@ def lib():
     L.nsp_step.argtypes = [vp, ip, ctypes.POINTER(StepInfo)]
     L.nsp_nonlinear.argtypes = [vp, vp, vp, vp, vp, vp, vp, ip]
     L.nsp_diagnostics.argtypes = [vp, P]
+    L.nsp_diagnostics_noalloc.argtypes = [vp, vp]
     L.nsp_tables.argtypes = [vp, P, P, P, P, P, P, P]
     L.nsp_radial_tables.argtypes = [ip, P, P, P, P, P, P, P]
     L.nsp_read_state_files.argtypes = [vp, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p]

@@ -116,14 +116,35 @@ def run_ensemble(nsp, precision, steps, warmup, per_class=32):
     ens.autotune(1)
     if warmup:
         ens.step(warmup)
+    ens[0].synchronize()
     t0 = time.perf_counter()
     n_it, err, flags = ens.step(steps)
+    ens[0].synchronize()
     t1 = time.perf_counter()
     ens.close()
     secs = t1 - t0
+    fp64 = precision == nsp.FP64
+    # stage byte model summed over the members, per ensemble step
+    step_b = sum(step_bytes(m, fp64, float(n_it[:, j].mean())) for j, m in enumerate(ms))
+    achieved = step_b * steps / secs / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), model_bytes_per_step=round(step_b))
+    stages = None
+    prof = load_profile("ensemble_stages.json")
+    key = "fp64" if fp64 else "mixed"
+    if prof and key in prof:
+        # per stage: time per ensemble step from a rocprofv3 kernel trace of the
+        # same ensemble (committed), against the members' summed stage bytes
+        stages = {}
+        for st, d in prof[key]["stages"].items():
+            per_member = [algorithmic_bytes(st, m, fp64) for m in ms]
+            launches = d["launches_per_step"]
+            b = sum(per_member) * launches
+            stages[st] = dict(us_per_step=round(d["us_per_step"], 1), alg_gbs=round(b / d["us_per_step"] / 1e3, 1),
+                              alg_frac=round(b / d["us_per_step"] / 1e3 / HBM_PEAK_GBS, 4))
     return dict(members=len(ms), member_steps_per_s=round(len(ms) * steps / secs, 2),
                 ensemble_steps_per_s=round(steps / secs, 4), steps=steps, n_it_mean=round(float(n_it.mean()), 3),
-                flags=int(flags.max()),
+                flags=int(flags.max()), roofline=roof, stage_roofline=stages,
                 config="32 x (NR=48, NT=96, NZ=513) + 32 x (NR=64, NT=128, NZ=1025), Re 2000, L 50R, dt 0.005",
                 ic=f"puff seed {SEED4} (window rms 0.5 U_bulk, width 5R) + member j perturbation seed "
                    f"{SEED4}+1+j (rms 1e-6 U_bulk)")
@@ -146,7 +167,7 @@ def roofline(res, cfg, fp64, traffic):
                 traffic_gbs=None if tr is None else round(tr / avg_s / 1e9, 1))
 
 
-def stage_rooflines(res, cfg, fp64, traffic):
+def stage_rooflines(res, cfg, fp64, traffic, valu=None):
     """Per stage: time per launch, algorithmic and measured-traffic rates and
     their fractions of the HBM peak."""
     form = "fp64" if fp64 else "mixed"
@@ -162,12 +183,19 @@ def stage_rooflines(res, cfg, fp64, traffic):
         if tk:
             d["traffic_gbs"] = round(tk["traffic"] / avg_s / 1e9, 1)
             d["traffic_frac"] = round(tk["traffic"] / avg_s / 1e9 / HBM_PEAK_GBS, 4)
+        cs = [c for c in (valu or {}).get(form, {}).get("kernels", {}).values() if c["stage"] == name]
+        if cs and all(c.get("valu_issue") is not None for c in cs):
+            # VALU busy of the stage's kernels (rocprofv3 counters), weighted by their durations
+            tot = sum(c["avg_us"] for c in cs)
+            d["valu_issue_frac"] = round(sum(c["valu_issue"] * c["avg_us"] for c in cs) / tot, 4)
+            d["valu_insts"] = round(sum(c["valu_insts"] for c in cs))
         out[name] = d
     return out
 
 
-def load_traffic():
-    path = os.path.join(ROOT, "profiles", "hbm_traffic.json")
+def load_profile(name):
+    """A committed rocprofv3 reduction under profiles/ (None if absent)."""
+    path = os.path.join(ROOT, "profiles", name)
     if os.path.exists(path):
         with open(path) as f:
             return json.load(f)
@@ -175,32 +203,59 @@ def load_traffic():
 
 
 def cpu_baseline(cfg, n_it_gpu):
-    """Oracle on a bounded sample: the headline workload with NR reduced to
-    8 (same 576 x 4608 transforms per level, same per-mode solves), one
-    time step, scaled linearly in NR and by sweep count to one full step."""
+    """The oracle (host C++ FP64 restatement of the reference's kernels,
+    OpenMP over radial levels and modes) on the same workload: one whole time
+    step of the headline grid on every thread the process may use, measured
+    end to end (after a one-off setup that is not timed).  A single-thread
+    sample of the same work follows: one radial level of one nonlinear
+    evaluation (the oracle's dominant part), timed on one thread, and the
+    same per-level work on all threads for comparison.  The oracle's FFT is a
+    plain recursive mixed-radix transform, so these rates are a floor for a
+    CPU implementation, not a tuned CPU code."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import numpy as np
     import nsp
-    from oracle_lib import Oracle, max_threads
-    nr_s = 8
-    sample = dict(cfg, nr=nr_s)
-    # the same IC recipe on the sample grid, generated by the product's host code
-    p = nsp.Pipe(**sample)
+    from oracle_lib import Oracle, max_threads, set_threads
+    # the headline IC, generated by the product's host code on the GPU
+    p = nsp.Pipe(**cfg)
     p.seeded_ic(SEED3, 4, 4, 0, RMS3)
     state = p.get_state()
     p.close()
-    o = Oracle(**sample)
+    threads = max_threads()
+    o = Oracle(**cfg)
     o.set_state(*state)
     t0 = time.perf_counter()
     n_it, _ = o.step(1)
-    t = time.perf_counter() - t0
-    per_sweep = t / (1 + int(n_it[0])) * (cfg["nr"] / nr_s)
-    full_step = per_sweep * (1 + n_it_gpu)
-    return dict(value=round(1.0 / full_step, 6), unit="time steps/s", cores=max_threads(), kind="port",
-                sample=(f"oracle (host C++ FP64, OpenMP) one time step of the headline workload with NR={nr_s} "
-                        f"(same 576x4608 transforms and per-mode solves), {t:.1f} s for 1+{int(n_it[0])} sweeps, "
-                        f"scaled x{cfg['nr'] // nr_s} in NR and to 1+{n_it_gpu:.2f} sweeps"),
-                sample_seconds=round(t, 2))
+    t_step = time.perf_counter() - t0
+    # single-thread sample: level 0 of one nonlinear evaluation
+    o.set_state(*state)
+    o.set_sample_levels(1)
+    set_threads(1)
+    t0 = time.perf_counter()
+    o.nonlinear(*state)
+    t_lvl1 = time.perf_counter() - t0
+    # the same per-level work on every thread (one level per thread)
+    set_threads(threads)
+    o.set_sample_levels(min(threads, cfg["nr"]))
+    t0 = time.perf_counter()
+    o.nonlinear(*state)
+    t_lvlT = time.perf_counter() - t0
+    o.set_sample_levels(0)
+    del o, state
+    lv = min(threads, cfg["nr"])
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return dict(value=round(1.0 / t_step, 6), unit="time steps/s", cores=threads, kind="port",
+                sample=(f"oracle (host C++ FP64, OpenMP, {threads} threads): one whole time step of the headline "
+                        f"workload (NR=96, NT=384, NZ=1537; 1+{int(n_it[0])} sweeps) in {t_step:.1f} s"),
+                step_seconds=round(t_step, 2), n_it=int(n_it[0]),
+                nproc=os.cpu_count(), usable_cpus=usable, omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                single_core=dict(sample="one radial level of one nonlinear evaluation of the headline workload "
+                                        "(12 inverse and 3 forward 576x4608 transforms plus the products), one thread",
+                                 seconds=round(t_lvl1, 3),
+                                 all_threads_seconds_per_level=round(t_lvlT / lv, 3),
+                                 all_threads_levels=lv))
 
 
 def copy_calibration(nbytes, copies=3):
@@ -234,6 +289,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run 3 device-to-device copies of BYTES (counter calibration)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the config-4 ensemble measurement")
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="only the config-4 ensemble measurement (its own JSON line; for profiling)")
     ap.add_argument("--ensemble-steps", type=int, default=5)
     ap.add_argument("--tuning", metavar="FILE", help="apply saved kernel variants instead of autotuning")
     ap.add_argument("--save-tuning", metavar="FILE", help="write the chosen kernel variants")
@@ -246,8 +303,15 @@ def main():
     nsp.lib()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+    if args.ensemble_only:
+        prec = nsp.FP64 if args.precision == "fp64" else nsp.MIXED
+        ens = run_ensemble(nsp, prec, args.ensemble_steps, 1)
+        ens["form"] = "fp64" if prec == nsp.FP64 else "mixed"
+        print(json.dumps({"ensemble": ens}), flush=True)
+        return
     cfg = CONFIG3
-    traffic = load_traffic()
+    traffic = load_profile("hbm_traffic.json")
+    valu = load_profile("valu_counters.json")
     forms = {"fp64": nsp.FP64, "mixed": nsp.MIXED}
     want = ["fp64", "mixed"] if args.precision == "both" else [args.precision]
     saved = {}
@@ -286,7 +350,7 @@ def main():
                    "nr": cfg["nr"], "nt": cfg["nt"], "nz": cfg["nz"], "re": cfg["re"], "dt": cfg["dt"],
                    "form": head, "n_it_mean": n_it_mean},
         "roofline": roofline(r, cfg, fp64, traffic),
-        "stage_roofline": stage_rooflines(r, cfg, fp64, traffic),
+        "stage_roofline": stage_rooflines(r, cfg, fp64, traffic, valu),
         "step_model_hbm_frac": round(step_bytes(cfg, fp64, n_it_mean) * r["steps_per_s"] / 1e9 / HBM_PEAK_GBS, 4),
     }
     for f in want:

@@ -36,9 +36,16 @@ typedef enum {
 
 typedef enum {
   NSP_FP64 = 0,   /* every stage in FP64 (parity form) */
-  NSP_MIXED = 1   /* FP32 transforms/products; the velocity solves of |m| >= 4 read FP32 LU factors
-                     and keep their column scratch in FP32 (FP64 arithmetic); FP64 state, pressure
-                     factors and low-|m| solves */
+  NSP_MIXED = 1,  /* FP32 transforms/products; the velocity solves of |m| >= 4 read FP32 LU factors
+                     and keep their column scratch (right-hand side, eliminated rows, explicit part,
+                     pressure) in FP32, with FP64 arithmetic: those modes' new state carries the FP32
+                     rounding of the right-hand side (~6e-8 of |u|/dt times dt, i.e. of |u|) each
+                     step.  The state array, pressure factors and the |m| < 4 solves are FP64 */
+  NSP_MIXED_REFINED = 2  /* NSP_MIXED with the velocity solves of every mode made increment
+                     solves: the FP32-stored right-hand side is rhs - H u^n (formed in FP64 from
+                     small terms), the FP32 solve gives delta = u - u^n and u = u^n + delta is formed
+                     in FP64: one FP64 residual-refinement step from u^n, so the FP32 rounding falls
+                     on delta only.  Single pipes only (not in ensembles) */
 } nsp_precision;
 
 typedef enum { NSP_HOST = 0, NSP_DEVICE = 1 } nsp_where;

@@ -42,6 +42,7 @@ struct nsp_handle {
   double2* E = nullptr;     // explicit part of the step (+/-), sweep-side layout
   double2* SR = nullptr;    // sweep scratch (3 fields), sweep-side layout
   double2* SP = nullptr;    // sweep scratch (1 field), sweep-side layout
+  double2* DH = nullptr;    // refined form: div(H u^n) of the step, sweep-side layout
   double* F = nullptr;      // banded LU factors (u+ all modes, u- row NT/2, uz and p rows <= NT/2)
   float* F32 = nullptr;     // mixed form: FP32 copies of the u+, u-, uz sets (read for |m| >= kM32)
   double* PH = nullptr;     // homogeneous pressure data (rows it <= NT/2), sweep-side layout
@@ -116,7 +117,8 @@ nsp_status check_params(const nsp_params* p) {
   if (p->nz < 3 || p->nz % 2 == 0) return fail(NSP_ERR_INVALID, "nz must be odd and >= 3");
   if (!(p->lz > 0) || !(p->re > 0) || !(p->dt > 0)) return fail(NSP_ERR_INVALID, "lz, re, dt must be > 0");
   if (p->maxit < 1) return fail(NSP_ERR_INVALID, "maxit must be >= 1");
-  if (p->precision != NSP_FP64 && p->precision != NSP_MIXED) return fail(NSP_ERR_INVALID, "bad precision");
+  if (p->precision != NSP_FP64 && p->precision != NSP_MIXED && p->precision != NSP_MIXED_REFINED)
+    return fail(NSP_ERR_INVALID, "bad precision");
   const int nzf = 3 * (p->nz - 1);
   if (nzf > 36 * 256) return fail(NSP_ERR_INVALID, "3(nz-1) must be <= 9216");
   return NSP_OK;
@@ -168,7 +170,9 @@ nsp_status fetch_fields(nsp_handle* h, const double2* src, double* a, double* b,
 // and all arithmetic stay FP64.
 constexpr int kM32 = 4;
 
-Lin2Args lin_args(nsp_handle* h, const Geom& g, const double2* N, const double2* u, double2* out, bool predictor) {
+// un: the step's state u^n (the refined form's solves are for u - u^n)
+Lin2Args lin_args(nsp_handle* h, const Geom& g, const double2* N, const double2* u, double2* out, bool predictor,
+                  const double2* un) {
   Lin2Args a;
   a.g = g;
   a.r = h->t.r;
@@ -183,6 +187,10 @@ Lin2Args lin_args(nsp_handle* h, const Geom& g, const double2* N, const double2*
   a.F32 = h->F32;
   a.m32 = kM32;
   a.s32 = h->nb.mixed ? 1 : 0;
+  a.form64 = h->nb.mixed ? 0 : 1;
+  a.inc = h->P.precision == NSP_MIXED_REFINED ? 1 : 0;
+  a.DH = h->DH;
+  a.un = un;
   a.PH = h->PH;
   a.invc = h->b.invc;
   a.N = N;
@@ -349,7 +357,7 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   TRY(upload(h, &h->nb.tw_z, twiddles(g.nzf, g.nh + 1)));
   h->nb.wstride = size_t(g.nr) * tp_plane(g);  // tile-major planes (tp_plane)
   h->nb.use_ct = false;
-  h->nb.mixed = p->precision == NSP_MIXED;
+  h->nb.mixed = p->precision != NSP_FP64;
   h->nb.var = NlVariant{ThetaVar{0, 8, 256, 0, 1}, ThetaVar{0, 8, 256, 0, 1}, 1, 256, 0, 0, 1};
   if (!ct_has_zreg(g.nh)) {  // long axial rows: the 512-thread LDS form
     h->nb.var.z_alg = 0;
@@ -399,6 +407,7 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   TRY(dalloc(h, &h->E, 3 * sw_field(g)));
   TRY(dalloc(h, &h->SR, 3 * sw_field(g)));
   TRY(dalloc(h, &h->SP, sw_field(g)));
+  if (p->precision == NSP_MIXED_REFINED) TRY(dalloc(h, &h->DH, sw_field(g)));
   if (share) {
     h->F = share->F;
     h->F32 = h->nb.mixed ? share->F32 : nullptr;  // an FP64 member never reads FP32 factors
@@ -570,7 +579,7 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
                                              nb.wstride, h->t.D, h->u, h->Nl, s);
     // a predictor sweep of the current state into the free iterate buffer
     // (E and the iterate are recomputed by the next step)
-    Lin2Args a = lin_args(h, g, h->Nl, h->u, h->ui, true);
+    Lin2Args a = lin_args(h, g, h->Nl, h->u, h->ui, true, h->u);
     a.ord = v.s_ord;
     launch_sweep2(a, s);
     return true;
@@ -743,7 +752,7 @@ nsp_status capture_step(nsp_handle* h, int a) {
     double2* dst = (it % 2) ? uc : ub;
     launch_nonlinear(gk, h->t, h->nb, src, h->Nl, s, nullptr);
     launch_norm_reset(h->norm, gk.skip, s);
-    launch_sweep2(lin_args(h, gk, h->Nl, src, dst, false), s);
+    launch_sweep2(lin_args(h, gk, h->Nl, src, dst, false, ua), s);
     launch_converge(h->norm, &h->rec->st[0], &h->rec->err, h->P.tol, it, h->P.maxit, s);
   };
   auto finish = [&](hipGraphExec_t* out, const char* what) -> nsp_status {
@@ -762,7 +771,7 @@ nsp_status capture_step(nsp_handle* h, int a) {
   HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
   launch_step_begin(&h->rec->st[0], &h->rec->err, s);
   launch_nonlinear(g, h->t, h->nb, ua, h->Nl, s, nullptr);
-  launch_sweep2(lin_args(h, g, h->Nl, ua, ub, true), s);
+  launch_sweep2(lin_args(h, g, h->Nl, ua, ub, true, ua), s);
   if (h->P.maxit >= 1) corrector(1);
   nsp_status st = finish(&h->gexec[a], "step graph");
   if (st != NSP_OK || h->P.maxit < 2) return st;
@@ -823,7 +832,7 @@ nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {
       // nonlinear(u,rhs); predictor(u,rhs,uw,rhsw,ui)
       launch_nonlinear(g, h->t, h->nb, h->u, h->Nl, s, pf);
       if (pf) pf->begin(K_LIN_PRED, s);
-      launch_sweep2(lin_args(h, g, h->Nl, h->u, h->ui, true), s);
+      launch_sweep2(lin_args(h, g, h->Nl, h->u, h->ui, true, h->u), s);
       if (pf) pf->end(s);
       HIPCHK(hipGetLastError());
       while (ite < h->P.maxit && err > h->P.tol) {
@@ -831,7 +840,7 @@ nsp_status nsp_step(nsp_handle* h, int nsteps, nsp_step_info* info) {
         launch_nonlinear(g, h->t, h->nb, h->ui, h->Nl, s, pf);
         launch_norm_reset(h->norm, nullptr, s);
         if (pf) pf->begin(K_LIN_CORR, s);
-        launch_sweep2(lin_args(h, g, h->Nl, h->ui, h->un, false), s);
+        launch_sweep2(lin_args(h, g, h->Nl, h->ui, h->un, false, h->u), s);
         if (pf) pf->end(s);
         HIPCHK(hipGetLastError());
         HIPCHK(hipMemcpyAsync(h->norm_h, h->norm, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
@@ -1544,9 +1553,9 @@ void ens_stage(nsp_ensemble* e) {
     nl[0 * M + j] = nl_member(h, g0, ua);
     nl[1 * M + j] = nl_member(h, gk, ub);
     nl[2 * M + j] = nl_member(h, gk, uc);
-    sw[0 * M + j] = lin_args(h, g0, h->Nl, ua, ub, true);
-    sw[1 * M + j] = lin_args(h, gk, h->Nl, ub, uc, false);
-    sw[2 * M + j] = lin_args(h, gk, h->Nl, uc, ub, false);
+    sw[0 * M + j] = lin_args(h, g0, h->Nl, ua, ub, true, ua);
+    sw[1 * M + j] = lin_args(h, gk, h->Nl, ub, uc, false, ua);
+    sw[2 * M + j] = lin_args(h, gk, h->Nl, uc, ub, false, ua);
   }
 }
 
@@ -1633,6 +1642,9 @@ nsp_status ens_capture(nsp_ensemble* e) {
 nsp_status nsp_ensemble_create(const nsp_params* params, int count, int device, nsp_ensemble** out) {
   if (!out || !params || count < 1) return fail(NSP_ERR_INVALID, "bad ensemble arguments");
   *out = nullptr;
+  for (int j = 0; j < count; j++)
+    if (params[j].precision == NSP_MIXED_REFINED)
+      return fail(NSP_ERR_INVALID, "ensemble members take NSP_FP64 or NSP_MIXED (NSP_MIXED_REFINED: single pipes)");
   nsp_ensemble* e = new nsp_ensemble();
   e->device = device;
   auto bail = [&](nsp_status st) {

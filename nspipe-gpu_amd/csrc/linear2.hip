@@ -264,7 +264,18 @@ struct Scr<true> {
 // SIMD). The FP64 form is faster with the first, the mixed form with the
 // second. The two compile to different instruction sequences (contraction),
 // so a pipe's form fixes which one it runs, batched or not.
-template <int PASS, int PRED, int B, int HOLD = 0>
+// INC = 1 (refined mixed form, single pipes): the velocity solves are for the
+// increment delta = u^{k+1} - u^n, whose right-hand side is formed in FP64 from
+// small quantities, so the FP32 column storage of |m| >= m32 rounds delta,
+// not u (one step of iterative refinement from u^n):
+//   A1 (PASS 4, predictor only): E' = (1/Re) L'u^n + (1 - d_im) N(u^n) and
+//        Dh = div(H u^n) with H u^n = u^n/dt - (d_im/Re) L'u^n, L' = Lu + diag
+//        (addLap_k's operator), for every row;
+//   A:   rhs' = E' + d_im N(u^k) = rhs - H u^n; div(rhs) = div(rhs') + Dh;
+//   C:   wall rows -u^n (so that u^n + delta vanishes there);
+//   D:   back substitution gives delta; u^n + delta takes the place of the
+//        solution from there on (wall divergence, correction, output).
+template <int PASS, int PRED, int B, int HOLD = 0, int INC = 0>
 __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* tab) {
   if constexpr (B != 0) {
     A = tab[blockIdx.y];
@@ -289,10 +300,10 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     A.norm = gmem(tab, A.norm);
     if constexpr (PASS == 3) {  // members of the other form are the other launch's
       // The form is tested on a copy the compiler cannot see through: knowing
-      // F32 and s32 past this test would let it drop the other form's paths
+      // the form past this test would let it drop the other form's paths
       // and then contract the remaining arithmetic differently from the
       // single-pipe kernel, whose results the batched one must match bit for bit.
-      int fp64 = __builtin_amdgcn_readfirstlane((A.F32 == nullptr && A.s32 == 0) ? 1 : 0);
+      int fp64 = __builtin_amdgcn_readfirstlane(A.form64 ? 1 : 0);
       asm volatile("" : "+s"(fp64));
       if (HOLD != fp64) return;
     }
@@ -356,9 +367,123 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
   const Scr<S32> SP = Scr<S32>::at(A.SP, (size_t)lb * nr * 128);
   const Scr<S32> E = Scr<S32>::at(A.E, (size_t)lb * nr * 384);
 
+  // ============================== pass A1 (up, INC predictor): E', Dh
+  if constexpr (PASS == 4) {
+    const Scr<S32> DH = Scr<S32>::at(A.DH, (size_t)lb * nr * 128);
+    double2 uw[3][7];      // u^n +/- rows t-6..t
+    double2 Hr[7];         // (H u^n)_r rows i-3..i+3
+    double2 Ht[4], Hz[4];  // (H u^n)_t, (H u^n)_z rows i..i+3
+#pragma unroll
+    for (int w = 0; w < 7; w++) {
+      Hr[w] = z2;
+#pragma unroll
+      for (int c = 0; c < 3; c++) uw[c][w] = z2;
+    }
+#pragma unroll
+    for (int w = 0; w < 4; w++) Ht[w] = Hz[w] = z2;
+    const double ire = 1.0 / g.re, aim = d_im / g.re, idt = 1.0 / g.dt;
+    for (int t = 0; t < nr + 6; t++) {
+      double2 a = z2, b = z2, c = z2;
+      if (t < nr) {
+        const size_t h = (size_t)t * MS + k;
+        a = A.u[h];
+        b = A.u[n + h];
+        c = A.u[2 * n + h];
+        pm_of(a, b);
+      }
+#pragma unroll
+      for (int w = 0; w < 6; w++) {
+        uw[0][w] = uw[0][w + 1];
+        uw[1][w] = uw[1][w + 1];
+        uw[2][w] = uw[2][w + 1];
+      }
+      uw[0][6] = a;
+      uw[1][6] = b;
+      uw[2][6] = c;
+      const int j = t - 3;  // row whose 7-point window is complete
+      double2 H0 = z2, H1 = z2, H2 = z2;
+      if (j >= 0 && j < nr) {
+        const size_t hn = nb0 + (size_t)j * npl;  // tile-major N(u^n)
+        double2 nn[3];
+        if (A.n32) {
+          const float2* N32 = reinterpret_cast<const float2*>(A.N);
+          const float2 f0 = N32[hn], f1 = N32[ncs + hn], f2 = N32[2 * ncs + hn];
+          nn[0] = make_double2(f0.x, f0.y);
+          nn[1] = make_double2(f1.x, f1.y);
+          nn[2] = make_double2(f2.x, f2.y);
+        } else {
+          nn[0] = A.N[hn];
+          nn[1] = A.N[ncs + hn];
+          nn[2] = A.N[2 * ncs + hn];
+        }
+        pm_of(nn[0], nn[1]);
+        const double rj = rt[j], r2 = rj * rj;
+        double2 hu[3];
+#pragma unroll
+        for (int cc = 0; cc < 3; cc++) {  // L'u^n as in addLap_k (mod_linear.cu:12-51)
+          const ctab Lc = ctab(A.Luc) + ((cc < 2) ? par_u : par_p) * STEN * nr + STEN * j;
+          double px = 0.0, py = 0.0;
+#pragma unroll
+          for (int w = 0; w < 7; w++) {
+            px += Lc[w] * uw[cc][w].x;
+            py += Lc[w] * uw[cc][w].y;
+          }
+          const double2 aux = uw[cc][3];
+          double de = -q.kz * q.kz - q.kt * q.kt / r2;
+          if (cc == 0) de += -(1.0 / r2) - (2.0 * q.kt / r2);
+          else if (cc == 1) de += -(1.0 / r2) + (2.0 * q.kt / r2);
+          px += de * aux.x;
+          py += de * aux.y;
+          E.st(sb3 + (size_t)j * 384 + cc * 128,
+               make_double2(ire * px + (1.0 - d_im) * nn[cc].x, ire * py + (1.0 - d_im) * nn[cc].y));
+          hu[cc] = make_double2(idt * aux.x - aim * px, idt * aux.y - aim * py);
+        }
+        rt_of(hu[0], hu[1]);
+        H0 = hu[0];
+        H1 = hu[1];
+        H2 = hu[2];
+      }
+#pragma unroll
+      for (int w = 0; w < 6; w++) Hr[w] = Hr[w + 1];
+      Hr[6] = H0;
+#pragma unroll
+      for (int w = 0; w < 3; w++) {
+        Ht[w] = Ht[w + 1];
+        Hz[w] = Hz[w + 1];
+      }
+      Ht[3] = H1;
+      Hz[3] = H2;
+      const int i = j - 3;
+      if (i >= 0 && i < nr) {  // div_k (mod_radialFD.cu:67-96) of H u^n, wall row zero
+        double2 dv = z2;
+        if (i < nr - 1) {
+          const ctab Dr = Dcu + STEN * i;
+          double px = 0.0, py = 0.0;
+#pragma unroll
+          for (int w = 0; w < 7; w++) {
+            px += Dr[w] * Hr[w].x;
+            py += Dr[w] * Hr[w].y;
+          }
+          const double rK = rt[i];
+          const double2 ur = Hr[3], ut = Ht[0], uz = Hz[0];
+          px += ur.x / rK;
+          py += ur.y / rK;
+          px += -q.kt * ut.y / rK;
+          py += q.kt * ut.x / rK;
+          px += -q.kz * uz.y;
+          py += q.kz * uz.x;
+          dv = make_double2(px, py);
+        }
+        DH.st(sb1 + (size_t)i * 128, dv);
+      }
+    }
+  }
+
   // ============================== pass A (up)
   if constexpr (PASS == 0) {
-    constexpr int LAG = PRED ? 6 : 3;
+    constexpr bool XPRED = PRED && !INC;  // INC: the predictor's explicit part comes from A1
+    constexpr int LAG = XPRED ? 6 : 3;
+    const Scr<S32> DH = Scr<S32>::at(INC ? A.DH : A.SP, (size_t)lb * nr * 128);
     double2 uw[3][7];     // predictor: u^n +/- rows t-6..t
     double2 Rr[7];        // R_r rows i-3..i+3
     double2 Rt[4], Rz[4]; // R_t, R_z rows i..i+3
@@ -375,7 +500,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     for (int w = 0; w < 3; w++) yp[w] = z2;
     const double cn = (1.0 - d_im) / g.re, idt = 1.0 / g.dt;
     for (int t = 0; t < nr + LAG; t++) {
-      if (PRED) {
+      if (XPRED) {
         double2 a = z2, b = z2, c = z2;
         if (t < nr) {
           const size_t h = (size_t)t * MS + k;
@@ -395,7 +520,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         uw[2][6] = c;
       }
       // rhs row j
-      const int j = PRED ? t - 3 : t;
+      const int j = XPRED ? t - 3 : t;
       double2 R0 = z2, R1 = z2, R2 = z2;
       if (j >= 0 && j < nr) {
         const size_t hn = nb0 + (size_t)j * npl;  // tile-major N(u)
@@ -413,7 +538,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         }
         pm_of(n0, n1);
         double2 rh[3];
-        if (PRED) {
+        if (XPRED) {
           // addLap_k (mod_linear.cu:12-51) on u^n, then rhs = N + that
           const double rj = rt[j], r2 = rj * rj;
           const double2 nn[3] = {n0, n1, n2};
@@ -487,6 +612,11 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
           px += -q.kz * uz.y;
           py += q.kz * uz.x;
           dv = make_double2(px, py);
+          if constexpr (INC) {  // div(rhs) = div(rhs - H u^n) + div(H u^n)
+            const double2 dh = DH.ld(sb1 + (size_t)i * 128);
+            dv.x += dh.x;
+            dv.y += dh.y;
+          }
         }
         const double* fp = A.F + fb_p + (size_t)i * FR_ABS;
         const double l0 = fp[0], l1 = fp[64], l2 = fp[128];
@@ -564,7 +694,18 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       rh[2].x -= -q.kz * ph.y;
       rh[2].y -= q.kz * ph.x;
       pm_of(rh[0], rh[1]);
-      if (j == nr - 1) rh[0] = rh[1] = rh[2] = z2;
+      if (j == nr - 1) {  // write_bcon_k: wall rows zero (INC: delta = -u^n there)
+        rh[0] = rh[1] = rh[2] = z2;
+        if constexpr (INC) {
+          const size_t hw = (size_t)j * MS + k;
+          double2 a = A.un[hw], b = A.un[n + hw];
+          pm_of(a, b);
+          const double2 c = A.un[2 * n + hw];
+          rh[0] = make_double2(-a.x, -a.y);
+          rh[1] = make_double2(-b.x, -b.y);
+          rh[2] = make_double2(-c.x, -c.y);
+        }
+      }
 #pragma unroll
       for (int c = 0; c < 3; c++) {  // forward elimination
         // u+, u-: rows of FR_UP, slots of 128; uz: rows of FR_ABS, slots of 64
@@ -644,6 +785,15 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
         xh[c][0] = hh;
         xo[c] = x;
         ho[c] = hh;
+      }
+      if constexpr (INC) {  // the solution is u^n + delta (FP64)
+        const size_t hu = (size_t)i * MS + k;
+        double2 a = A.un[hu], b = A.un[n + hu];
+        pm_of(a, b);
+        const double2 c = A.un[2 * n + hu];
+        xo[0] = make_double2(xo[0].x + a.x, xo[0].y + a.y);
+        xo[1] = make_double2(xo[1].x + b.x, xo[1].y + b.y);
+        xo[2] = make_double2(xo[2].x + c.x, xo[2].y + c.y);
       }
     };
     // wall divergence of the provisional solution (divWall_k on r/theta
@@ -814,6 +964,15 @@ void launch_factor(const Geom& g, const double* r, const double* Lu, const doubl
 
 // pass D runs as HOLD = 1 for the FP64-form pipes and as HOLD = 0 for the
 // mixed-form ones (a batched launch of both forms makes both launches)
+template <int PRED>
+static void sweep_passes_inc(const Lin2Args& a, dim3 grid, hipStream_t s) {
+  if (PRED) sweep_kernel<4, PRED, 0, 0, 1><<<grid, 128, 0, s>>>(a, nullptr);
+  sweep_kernel<0, PRED, 0, 0, 1><<<grid, 128, 0, s>>>(a, nullptr);
+  sweep_kernel<1, PRED, 0, 0, 1><<<grid, 128, 0, s>>>(a, nullptr);
+  sweep_kernel<2, PRED, 0, 0, 1><<<grid, 128, 0, s>>>(a, nullptr);
+  sweep_kernel<3, PRED, 0, 0, 1><<<grid, 128, 0, s>>>(a, nullptr);
+}
+
 template <int PRED, int B>
 static void sweep_passes(const Lin2Args& a, const Lin2Args* tab, dim3 grid, bool fp64, bool mixed, hipStream_t s) {
   sweep_kernel<0, PRED, B><<<grid, 128, 0, s>>>(a, tab);
@@ -825,7 +984,12 @@ static void sweep_passes(const Lin2Args& a, const Lin2Args* tab, dim3 grid, bool
 
 void launch_sweep2(const Lin2Args& a, hipStream_t s) {
   const dim3 grid((a.g.nt / 2) * ((a.g.nz + 63) / 64));
-  const bool fp64 = a.F32 == nullptr && a.s32 == 0;
+  if (a.inc) {
+    if (a.predictor) sweep_passes_inc<1>(a, grid, s);
+    else sweep_passes_inc<0>(a, grid, s);
+    return;
+  }
+  const bool fp64 = a.form64 != 0;
   if (a.predictor) sweep_passes<1, 0>(a, nullptr, grid, fp64, !fp64, s);
   else sweep_passes<0, 0>(a, nullptr, grid, fp64, !fp64, s);
 }

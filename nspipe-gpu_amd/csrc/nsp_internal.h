@@ -196,6 +196,7 @@ struct Lin2Args {
   const float* F32;     // mixed form: the u+, u-, uz factors rounded to FP32 (same indices), or null
   int m32;              // |m| from which the sweep reads F32 instead of F
   int s32;              // mixed form: column scratch SR, SP and E of |m| >= m32 held as FP32 pairs
+  int form64;           // 1: the pipe is of the FP64 form (selects its pass-D kernel, batched or not)
   const double* PH;     // dr p_h, p_h of the homogeneous pressure solution (sw_abs_idx, 2 per row)
   const double2* invc;  // [MODES][4] column 3 of the inverse influence matrix
   const double2* N;     // [3] nonlinear term (r/theta); float2 storage when n32
@@ -205,6 +206,9 @@ struct Lin2Args {
   double2* out;         // [3] new iterate (r/theta)
   double2* SR;          // [3] scratch (sweep-side layout)
   double2* SP;          // [1] scratch (sweep-side layout)
+  double2* DH;          // [1] refined form: div(H u^n) of the step (sweep-side layout), else null
+  const double2* un;    // [3] u^n of the step (r/theta); the refined form's solves are for u - u^n
+  int inc;              // refined form: increment solves (sweep_kernel INC = 1)
   unsigned long long* norm;
   int predictor;
   int ord;              // block order (NlVariant::s_ord)

@@ -19,7 +19,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(os.path.dirname(HERE), "libnsp.so")
 
-FP64, MIXED = 0, 1
+FP64, MIXED, MIXED_REFINED = 0, 1, 2
 IC_FLAT, IC_KOLMOGOROV, IC_PUFF, IC_ADD = 0, 1, 2, 16
 HOST, DEVICE = 0, 1
 FLAG_NAN, FLAG_MAXIT = 1, 2

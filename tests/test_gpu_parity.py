@@ -285,7 +285,7 @@ def test_config2_mixed_against_fp64_long_run():
     cfg = dict(nr=64, nt=192, nz=385, lz=2 * np.pi, re=5300.0, dt=0.002)
     print("[gpu] config 2, 2000 steps, fp64 and mixed", flush=True)
     out = {}
-    for prec in (nsp.FP64, nsp.MIXED):
+    for prec in (nsp.FP64, nsp.MIXED, nsp.MIXED_REFINED):
         p = nsp.Pipe(**cfg, precision=prec)
         p.seeded_ic(20261202 + 2 * 1000, 4, 4, nsp.IC_KOLMOGOROV, 0.05)
         n_it, _, flags = p.step(2000)
@@ -295,15 +295,16 @@ def test_config2_mixed_against_fp64_long_run():
         out[prec] = (profiles(p), d, n_it)
         p.close()
     (m64, r64), d64, n64 = out[nsp.FP64]
-    (m32, r32), d32, n32 = out[nsp.MIXED]
-    e_mean = np.abs(m32 - m64).max() / np.abs(m64).max()
-    e_rms = np.abs(r32 - r64).max() / np.abs(r64).max()
-    e_ke = abs(d32["ke"] - d64["ke"]) / d64["ke"]
-    e_tau = abs(d32["u_tau"] - d64["u_tau"]) / d64["u_tau"]
-    e_fke = abs(d32["fke"] - d64["fke"]) / d64["fke"]
-    print(f"[gpu] config 2 mixed vs fp64 after 2000 steps: mean {e_mean:.3e} rms {e_rms:.3e} "
-          f"ke {e_ke:.3e} fke {e_fke:.3e} u_tau {e_tau:.3e}; correctors {n64.sum()} / {n32.sum()}", flush=True)
-    assert max(e_mean, e_rms, e_ke, e_fke, e_tau) <= 1e-5
+    for prec, name in ((nsp.MIXED, "mixed"), (nsp.MIXED_REFINED, "refined")):
+        (m32, r32), d32, n32 = out[prec]
+        e_mean = np.abs(m32 - m64).max() / np.abs(m64).max()
+        e_rms = np.abs(r32 - r64).max() / np.abs(r64).max()
+        e_ke = abs(d32["ke"] - d64["ke"]) / d64["ke"]
+        e_tau = abs(d32["u_tau"] - d64["u_tau"]) / d64["u_tau"]
+        e_fke = abs(d32["fke"] - d64["fke"]) / d64["fke"]
+        print(f"[gpu] config 2 {name} vs fp64 after 2000 steps: mean {e_mean:.3e} rms {e_rms:.3e} "
+              f"ke {e_ke:.3e} fke {e_fke:.3e} u_tau {e_tau:.3e}; correctors {n64.sum()} / {n32.sum()}", flush=True)
+        assert max(e_mean, e_rms, e_ke, e_fke, e_tau) <= 1e-5
 
 
 def test_config5_mixed_steps():
@@ -337,7 +338,7 @@ def test_set_tuning_rejects_missing_variant():
     p.close()
 
 
-@pytest.mark.parametrize("precision", [nsp.FP64, nsp.MIXED])
+@pytest.mark.parametrize("precision", [nsp.FP64, nsp.MIXED, nsp.MIXED_REFINED])
 def test_graph_and_eager_steps_agree(monkeypatch, precision):
     """The captured step (default) and the launch-by-launch step with a host
     convergence test give identical bits and iteration counts."""
@@ -450,6 +451,34 @@ def test_mixed_hundred_steps_cfg1_error_budget():
     p.close()
 
 
+@pytest.mark.parametrize("precision", [nsp.MIXED, nsp.MIXED_REFINED], ids=["mixed", "refined"])
+def test_mixed_forms_against_oracle_cfg1(precision):
+    """Both mixed forms against the FP64 oracle at config 1: one step (every
+    field), and 100 steps (KE, fluctuation KE, u_tau, u_cl), with identical
+    corrector counts.  The refined form's increment solves keep the FP32
+    column rounding off the state, so its errors are the FP32 transforms'."""
+    name = "refined" if precision == nsp.MIXED_REFINED else "mixed"
+    p = nsp.Pipe(**CFG1, precision=precision)
+    o = Oracle(**CFG1)
+    p.seeded_ic(SEED1, 4, 4, 0, 0.05)
+    o.set_state(*p.get_state())
+    n_p, _, _ = p.step(1)
+    n_o, _ = o.step(1)
+    errs = [rel_l2(a, b) for a, b in zip(p.get_state(), o.get_state())]
+    print(f"[gpu] {name} one-step state rel L2 {errs}", flush=True)
+    assert max(errs) <= 1e-6 and n_p[0] == n_o[0]
+    n_p, _, f_p = p.step(99)
+    n_o, _ = o.step(99)
+    assert np.array_equal(n_p, n_o) and np.all(f_p == 0)
+    dp, do = p.diagnostics(), o.diagnostics()
+    dp["fke"], do["fke"] = fluct_ke(p), fluct_ke(o)
+    e = {k: abs(dp[k] - do[k]) / abs(do[k]) for k in ("ke", "fke", "u_tau", "u_cl")}
+    print(f"[gpu] {name} 100 steps cfg1 vs oracle: " + " ".join(f"{k} {v:.3e}" for k, v in e.items()), flush=True)
+    assert max(e.values()) <= 1e-5
+    assert dp["wall_max"] < 1e-12
+    p.close()
+
+
 def test_poiseuille_fixed_point_gpu():
     shape = dict(nr=16, nt=8, nz=9, lz=2 * np.pi, re=800.0, dt=0.01)
     p = nsp.Pipe(**shape)
@@ -538,7 +567,7 @@ def test_config3_mixed_against_fp64_hundred_steps():
     corrector counts."""
     cfg = dict(nr=96, nt=384, nz=1537, lz=10 * np.pi, re=11700.0, dt=0.001)
     out = {}
-    for prec in (nsp.FP64, nsp.MIXED):
+    for prec in (nsp.FP64, nsp.MIXED, nsp.MIXED_REFINED):
         print(f"[gpu] config 3, 100 steps, precision {prec}", flush=True)
         p = nsp.Pipe(**cfg, precision=prec)
         p.seeded_ic(20261202 + 3 * 1000, 4, 4, 0, 0.15 * 0.5)
@@ -548,13 +577,15 @@ def test_config3_mixed_against_fp64_hundred_steps():
         d["fke"] = fluct_ke(p)
         out[prec] = (d, n_it)
         p.close()
-    (d64, n64), (d32, n32) = out[nsp.FP64], out[nsp.MIXED]
-    errs = {k: abs(d32[k] - d64[k]) / abs(d64[k]) for k in ("ke", "fke", "u_tau", "u_cl")}
-    print("[gpu] config 3 mixed vs fp64 after 100 steps: " +
-          " ".join(f"{k} {v:.3e}" for k, v in errs.items()) +
-          f"; div {d32['div_l2']:.3e} / {d64['div_l2']:.3e}; wall {d32['wall_max']:.1e} / {d64['wall_max']:.1e}",
-          flush=True)
-    assert np.array_equal(n64, n32)
-    assert max(errs.values()) <= 1e-5, errs
-    assert d32["wall_max"] < 1e-12 and d64["wall_max"] < 1e-12
-    assert abs(d32["div_l2"] - d64["div_l2"]) <= 1e-3 * d64["div_l2"] + 1e-14
+    d64, n64 = out[nsp.FP64]
+    for prec, name in ((nsp.MIXED, "mixed"), (nsp.MIXED_REFINED, "refined")):
+        d32, n32 = out[prec]
+        errs = {k: abs(d32[k] - d64[k]) / abs(d64[k]) for k in ("ke", "fke", "u_tau", "u_cl")}
+        print(f"[gpu] config 3 {name} vs fp64 after 100 steps: " +
+              " ".join(f"{k} {v:.3e}" for k, v in errs.items()) +
+              f"; div {d32['div_l2']:.3e} / {d64['div_l2']:.3e}; wall {d32['wall_max']:.1e} / {d64['wall_max']:.1e}",
+              flush=True)
+        assert np.array_equal(n64, n32)
+        assert max(errs.values()) <= 1e-5, errs
+        assert d32["wall_max"] < 1e-12 and d64["wall_max"] < 1e-12
+        assert abs(d32["div_l2"] - d64["div_l2"]) <= 1e-3 * d64["div_l2"] + 1e-14

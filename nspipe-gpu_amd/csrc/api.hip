@@ -648,6 +648,15 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
     const double t = time_it(1, v, lab);
     if (t >= 0 && t < bt) { bt = t; best.z_alg = 1; best.z_pf = pf; }
   }
+  for (int pad = 0; pad < 3; pad++) {
+    NlVariant v = best;
+    v.z_alg = 2;
+    v.z_pad = pad;
+    char lab[64];
+    std::snprintf(lab, sizeof lab, "zpass triples pad=%d", pad);
+    const double t = time_it(1, v, lab);
+    if (t >= 0 && t < bt) { bt = t; best.z_alg = 2; best.z_pad = pad; }
+  }
   // linear sweep: order of its (m, -m, axial chunk) workgroups
   bt = 1e30;
   for (int so = 0; so < 2; so++) {

@@ -359,6 +359,12 @@ template <typename T, int NTHR, int SIGN, typename LY, int R0, int... Rest>
 __device__ __forceinline__ void fft_rest(Plan<R0, Rest...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
   dit_passes<T, Plan<R0, Rest...>::L, 1, NTHR, SIGN, LY, R0, R0 - 1, Rest...>(buf, tw, tid);
 }
+// Remaining passes after first_pass_store of NCOL interleaved columns (and a
+// barrier): the butterflies of all columns are spread over the threads.
+template <typename T, int NCOL, int NTHR, int SIGN, typename LY, int R0, int... Rest>
+__device__ __forceinline__ void fft_rest_cols(Plan<R0, Rest...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
+  dit_passes<T, Plan<R0, Rest...>::L, NCOL, NTHR, SIGN, LY, R0, R0 - 1, Rest...>(buf, tw, tid);
+}
 // Remaining passes after first_pass_store, the last one emitting.
 template <typename T, int NCOL, int NTHR, int SIGN, typename LY, int R0, int... Rest, typename F>
 __device__ __forceinline__ void fft_rest_emit(Plan<R0, Rest...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid,

@@ -553,6 +553,187 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   }
 }
 
+// ------------------------------------------------------------------ z pass, transforms in triples
+// The register form with the twelve inverse transforms of a row taken three
+// at a time through one LDS buffer of three interleaved columns (element n of
+// column c at logical index c + 3n): the velocities (u_r, u_theta, u_z), then
+// for each component c the three fields of its convective product
+// (d_r u_c, d_theta u_c / r, d_z u_c).  A pass over the three columns spreads
+// their 3 L/R butterflies over the threads: the radix-16 passes of 2304
+// (144 butterflies) fill 432 of 512 thread slots in two rounds instead of 144
+// of 256 in one, and a triple takes one barrier per pass instead of three.
+// A component's product needs exactly its triple (no partial sums are kept
+// between transforms).  The forward transform of each product runs alone, in
+// the single-column layout LY of the same buffer.  Arithmetic per transform
+// and per sample is that of zpass2_ct.
+template <typename T, int NTHR, int PAD, int PAD3, typename PL, int BAT>
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 2 : 1))) void zpass3_ct(
+    Geom g, const cpx<T>* __restrict__ tw, const cpx<T>* __restrict__ twz, const double* __restrict__ r,
+    const cpx<T>* __restrict__ W, size_t ws, cpx<T>* __restrict__ Y, const NlMember* tab) {
+  if constexpr (BAT != 0) {
+    const NlMember& m = tab[blockIdx.y];
+    g = m.g;
+    r = gmem(tab, m.r);
+    W = gmem(tab, reinterpret_cast<const cpx<T>*>(m.W));
+    Y = gmem(tab, reinterpret_cast<cpx<T>*>(m.Y));
+    ws = m.ws;
+  }
+  if (step_done(g)) return;
+  constexpr int NH = PL::L;
+  constexpr int R0 = PL::rad[0];
+  constexpr int NB1 = NH / R0;
+  static_assert(NB1 <= NTHR, "one first-pass butterfly per thread");
+  constexpr int RL = NH / PL::rad[PL::P - 1];
+  using LY = Lay<RL, PAD>;       // one column (forward transforms)
+  using LY3 = Lay<3 * RL, PAD3>;  // three interleaved columns (inverse triples)
+  constexpr int NBUF = LY3::size(3 * NH) > LY::size(NH) ? LY3::size(3 * NH) : LY::size(NH);
+  __shared__ cpx<T> buf[NBUF];
+  const int row = blockIdx.x;
+  if (row >= g.nr * g.ntp) return;  // beyond this member's extent
+  const int i = row / g.ntp;
+  const int tid = threadIdx.x;
+  const bool own = NB1 == NTHR || tid < NB1;
+  const int nz = g.nz;
+  const T rad = T(r[i]);
+  const T irad = T(1) / rad;
+  auto over_r = [&](T x) -> T {
+    if constexpr (sizeof(T) == 4) return x * irad;
+    else return x / rad;
+  };
+  const int th = row - i * g.ntp;
+  const size_t roff = (size_t)i * tp_plane(g) + (size_t)th * 64;
+  const int rbytes = (int)((tp_plane(g) - (size_t)th * 64) * sizeof(cpx<T>));
+  const int t64 = g.ntp * 64;
+  const T kz0 = T(g.kz0);
+  const cpx<T> zero{T(0), T(0)};
+  constexpr int CS = (int)sizeof(cpx<T>);
+  constexpr int OOB = 0x7ffffff0;
+  // X[k] and X[NH-k] of the owned samples of field f (zero past NZ: range-checked buffer loads)
+  auto load = [&](int f, cpx<T>* A, cpx<T>* B) {
+    const int lt = opaque(tid);
+    if (NB1 == NTHR || lt < NB1) {
+      const nsp_i4v rs = buf_rsrc(W + f * ws + roff, rbytes);
+      if constexpr (NB1 % 64 == 0 && NH % 64 == 0) {
+        const int ok0 = ((lt >> 6) * t64 + (lt & 63)) * CS;
+        const int oc0 = (((NH >> 6) - ((lt + 63) >> 6)) * t64 + ((-lt) & 63)) * CS;
+        const int dq = (NB1 / 64) * t64 * CS;
+#pragma unroll
+        for (int q = 0; q < R0; q++) {
+          const int k = lt + NB1 * q, kc = NH - k;
+          A[q] = buf_load<T>(rs, k < nz ? ok0 + q * dq : OOB);
+          B[q] = buf_load<T>(rs, kc < nz ? oc0 - q * dq : OOB);
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < R0; q++) {
+          const int k = lt + NB1 * q, kc = NH - k;
+          A[q] = buf_load<T>(rs, k < nz ? (int)tp_col(g, k) * CS : OOB);
+          B[q] = buf_load<T>(rs, kc < nz ? (int)tp_col(g, kc) * CS : OOB);
+        }
+      }
+    }
+  };
+  cpx<T> wz[R0];
+#pragma unroll
+  for (int q = 0; q < R0; q++) wz[q] = own ? twz[tid + NB1 * q] : zero;
+  // C2R pre-processing of one column's owned samples and its first pass
+  auto first = [&](cpx<T>* A, cpx<T>* B, bool dz, int col) {
+    const int lt = opaque(tid);
+    if (own) {
+      cpx<T> v[R0];
+#pragma unroll
+      for (int q = 0; q < R0; q++) {
+        const int k = lt + NB1 * q, kc = NH - k;
+        cpx<T> a = A[q], b = B[q];
+        if (k == 0) { a.y = T(0); b.y = T(0); }
+        if (dz) {  // derivz_k (mod_deriv.cu:24-35)
+          const T ka = kz0 * T(k), kb = kz0 * T(kc);
+          a = cpx<T>{-ka * a.y, ka * a.x};
+          b = cpx<T>{-kb * b.y, kb * b.x};
+        }
+        const cpx<T> E{a.x + b.x, a.y - b.y};
+        const cpx<T> dl{a.x - b.x, a.y + b.y};
+        const cpx<T> O = cx_mulc(dl, wz[q]);
+        v[q] = cpx<T>{E.x - O.y, E.y + O.x};
+      }
+      first_pass_store<T, +1, LY3, 3>(PL{}, v, buf, lt, col);
+    }
+  };
+  // Inverse transforms of fields f0, f1, f2 (the third differentiated in z
+  // when DZ2); their physical values at the owned samples land in s0, s1, s2.
+  // On entry A, B hold the loads of f0; on exit, those of fnext (if >= 0).
+  cpx<T> A[R0], B[R0];
+  auto triple = [&](int f0, int f1, int f2, bool dz2, int fnext, cpx<T>(&s0)[R0], cpx<T>(&s1)[R0],
+                    cpx<T>(&s2)[R0]) {
+    first(A, B, false, 0);
+    load(f1, A, B);
+    first(A, B, false, 1);
+    load(f2, A, B);
+    first(A, B, dz2, 2);
+    __syncthreads();
+    const int lt = opaque(tid);
+    fft_rest_cols<T, 3, NTHR, +1, LY3>(PL{}, buf, tw, lt);
+#pragma unroll
+    for (int q = 0; q < R0; q++) {
+      s0[q] = s1[q] = s2[q] = zero;
+      if (own) {
+        const int e = 3 * (lt + NB1 * q);
+        s0[q] = buf[LY3::pos(e)];
+        s1[q] = buf[LY3::pos(e + 1)];
+        s2[q] = buf[LY3::pos(e + 2)];
+      }
+    }
+    __syncthreads();
+    if (fnext >= 0) load(fnext, A, B);
+  };
+  // field index in W: u_r, u_t, u_z (0-2), d_r u_c (3 + c), d_theta u_c / r (6 + c)
+  cpx<T> ur[R0], ut[R0], uz[R0];
+  load(0, A, B);
+  triple(0, 1, 2, false, 3, ur, ut, uz);
+#pragma unroll 1
+  for (int c = 0; c < 3; c++) {
+    cpx<T> sr[R0], st[R0], sz[R0];
+    triple(3 + c, 6 + c, c, true, c < 2 ? 4 + c : -1, sr, st, sz);
+    // calcNonlinear_k (mod_io_nonlinear.cu:142-170), summed in its order
+    if (own) {
+      cpx<T> v[R0];
+#pragma unroll
+      for (int q = 0; q < R0; q++) {
+        T vx = ur[q].x * sr[q].x, vy = ur[q].y * sr[q].y;
+        vx += ut[q].x * st[q].x;
+        vy += ut[q].y * st[q].y;
+        vx = vx + uz[q].x * sz[q].x;
+        vy = vy + uz[q].y * sz[q].y;
+        if (c == 0) {
+          vx = vx - ut[q].x * over_r(ut[q].x);
+          vy = vy - ut[q].y * over_r(ut[q].y);
+        } else if (c == 1) {
+          vx = vx + ur[q].x * over_r(ut[q].x);
+          vy = vy + ur[q].y * over_r(ut[q].y);
+        }
+        v[q] = cpx<T>{-vx, -vy};
+      }
+      first_pass_store<T, -1, LY, 1>(PL{}, v, buf, opaque(tid));
+    }
+    __syncthreads();
+    const int lt = opaque(tid);
+    fft_rest<T, NTHR, -1, LY>(PL{}, buf, tw, lt);
+    cpx<T>* dst = Y + c * ws + roff;
+    for (int k = lt; k < nz; k += NTHR) {  // half-length R2C post-processing
+      cpx<T> X = zero;
+      if (k < nz - 1) {
+        const cpx<T> zk = buf[LY::pos(k)], zc = buf[LY::pos(k == 0 ? 0 : NH - k)];
+        const cpx<T> E{T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y - zc.y)};
+        const cpx<T> O{T(0.5) * (zk.y + zc.y), T(-0.5) * (zk.x - zc.x)};
+        const cpx<T> wo = cx_mul(twz[k], O);
+        X = cpx<T>{E.x + wo.x, E.y + wo.y};
+      }
+      dst[tp_col(g, k)] = X;
+    }
+    __syncthreads();
+  }
+}
+
 // ------------------------------------------------------------------ theta forward
 template <typename T, int TL, int NTHR, typename LY, typename PL, int BAT>
 __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __restrict__ tw,
@@ -920,6 +1101,22 @@ bool zpass_plan(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T
         zpass2_ct<T, 256, 1, PAD, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
       else
         zpass2_ct<T, 256, 0, PAD, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+      return true;
+    } else {
+      return false;
+    }
+  }
+  if (v.z_alg == 2) {  // triples of inverse transforms, 256 threads; z_pad: 3-column row padding 0, 1, R0
+    if constexpr (LL / PL::rad[0] <= 256 && lds_fits<T, Dense>(3 * LL + PL::rad[PL::P - 1] * PL::rad[0])) {
+      constexpr int PAD = zpass2_pad<PL>();
+      if (v.z_pad == 0)
+        zpass3_ct<T, 256, PAD, 0, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+      else if (v.z_pad == 1)
+        zpass3_ct<T, 256, PAD, 1, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+      else if (v.z_pad == 2)
+        zpass3_ct<T, 256, PAD, PL::rad[0], PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+      else
+        return false;
       return true;
     } else {
       return false;

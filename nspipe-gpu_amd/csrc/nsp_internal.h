@@ -238,9 +238,11 @@ struct ThetaVar {
 // Kernel variant of the compiled-plan path (chosen by the autotuner).
 struct NlVariant {
   ThetaVar inv, fwd;
-  int z_alg;   // z pass: 0 LDS-scatter form, 1 register form (256 threads)
+  int z_alg;   // z pass: 0 LDS-scatter form, 1 register form (256 threads), 2 register form with the
+               // inverse transforms in triples
   int z_nthr;  // scatter form: threads per row (256 or 512)
-  int z_pad;   // scatter form: 1 = one LDS slot per 16 elements, 2 = row padding as the register form
+  int z_pad;   // scatter form: 1 = one LDS slot per 16 elements, 2 = row padding as the register form;
+               // triples: row padding of the three-column layout 0, 1 or R0 slots
   int z_pf;    // register form: prefetch the next transform's inputs
   int s_ord;   // linear sweep: 1 = the chunks of an azimuthal row XCD-contiguous, 0 = dispatch order
 };

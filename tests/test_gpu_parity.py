@@ -199,6 +199,8 @@ def _variants():
             out.append(dict(base, z_alg=0, z_nthr=nthr, z_pad=pad))
     for pf in (0, 1):
         out.append(dict(base, z_alg=1, z_pf=pf))
+    for pad in (0, 1, 2):
+        out.append(dict(base, z_alg=2, z_pad=pad))
     return out
 
 

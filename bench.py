@@ -27,6 +27,9 @@ sys.path.insert(0, os.path.join(ROOT, "nspipe-gpu_amd"))
 METRIC = "time steps/s at 96×384×1536: mixed vs FP64 GPU, % HBM roofline"
 HBM_PEAK_GBS = 8000.0
 CONFIG3 = dict(nr=96, nt=384, nz=1537, lz=10 * math.pi, re=11700.0, dt=0.001)
+# largest configuration: Re_bulk 19000 pipe of length 4 pi R, NR=160, NT=768, m_z=3072
+CONFIG5 = dict(nr=160, nt=768, nz=3073, lz=4 * math.pi, re=19000.0, dt=0.0005)
+SEED5 = 20261202 + 5 * 1000
 SEED3 = 20261202 + 3 * 1000
 RMS3 = 0.15 * 0.5  # 0.15 U_bulk in code units (U_bulk = Qdes = 0.5)
 
@@ -59,9 +62,12 @@ def step_bytes(cfg, fp64, n_it):
     return (1 + n_it) * b_nl + (1 + n_it) * b_sweep
 
 
-def run_gpu(nsp, precision, steps, warmup, cfg, chosen=None):
+def run_gpu(nsp, precision, steps, warmup, cfg, chosen=None, ic=None):
     p = nsp.Pipe(**cfg, precision=precision)
-    p.seeded_ic(SEED3, 4, 4, 0, RMS3)
+    if ic is None:
+        p.seeded_ic(SEED3, 4, 4, 0, RMS3)
+    else:
+        p.seeded_ic(*ic)
     if chosen is not None:
         p.set_tuning(chosen)
         tuning = dict(tuned="set", chosen=chosen)
@@ -289,6 +295,8 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="first run 3 device-to-device copies of BYTES (counter calibration)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the config-4 ensemble measurement")
+    ap.add_argument("--workload", choices=["headline", "config5"], default="headline",
+                    help="config5: only the largest configuration (mixed form), its per-stage rates")
     ap.add_argument("--ensemble-only", action="store_true",
                     help="only the config-4 ensemble measurement (its own JSON line; for profiling)")
     ap.add_argument("--ensemble-steps", type=int, default=5)
@@ -303,6 +311,22 @@ def main():
     nsp.lib()
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+    if args.workload == "config5":
+        cfg = CONFIG5
+        r = run_gpu(nsp, nsp.MIXED, args.steps, args.warmup, cfg, ic=(SEED5, 4, 4, nsp.IC_KOLMOGOROV, RMS3))
+        n_it_mean = sum(r["n_it"]) / len(r["n_it"])
+        out = {"metric": "time steps/s at 160x768x3072 (config 5), mixed form, % HBM roofline per stage",
+               "value": round(r["steps_per_s"], 4), "unit": "time steps/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": round(r["ms_per_step"], 3), "dtype": "f32+f64",
+               "data": "synthetic: seeded (mt19937_64, seed 20266202) divergence-free no-slip noise with a "
+                       "k^-5/3 envelope on modes |m|<=4, l<=4, rms 0.15 U_bulk, on the flux-matched Poiseuille profile",
+               "config": {"workload": "nsPipe time step, Re_bulk=19000 pipe L=4piR, NR=160 NT=768 m_z=3072 "
+                                      "(reference NZ=3073), dt=0.0005", "form": "mixed", "n_it_mean": n_it_mean},
+               "stage_roofline": stage_rooflines(r, cfg, False, None),
+               "step_model_hbm_frac": round(step_bytes(cfg, False, n_it_mean) * r["steps_per_s"] / 1e9 / HBM_PEAK_GBS, 4),
+               "tuning": (r["tuning"] or {}).get("chosen")}
+        print(json.dumps(out), flush=True)
+        return
     if args.ensemble_only:
         prec = nsp.FP64 if args.precision == "fp64" else nsp.MIXED
         ens = run_ensemble(nsp, prec, args.ensemble_steps, 1)
@@ -366,6 +390,19 @@ def main():
             out[f] = rr
     if "steps_per_s" in results.get("mixed", {}) and "steps_per_s" in results.get("fp64", {}):
         out["mixed_over_fp64"] = round(results["mixed"]["steps_per_s"] / results["fp64"]["steps_per_s"], 4)
+    if args.precision == "both" and "steps_per_s" in results.get("mixed", {}):
+        # the refined mixed form (FP64 residual refinement of the FP32 solves),
+        # with the mixed form's kernel variants; reported beside the headline
+        try:
+            rr = run_gpu(nsp, nsp.MIXED_REFINED, args.steps, args.warmup, cfg, results["mixed"]["tuning"]["chosen"])
+            out["mixed_refined"] = dict(steps_per_s=round(rr["steps_per_s"], 4), ms_per_step=round(rr["ms_per_step"], 3),
+                                        n_it=rr["n_it"], kernels_ms={k: round(v[0] / max(v[1], 1), 4)
+                                                                     for k, v in rr["kernels"].items() if v[1]},
+                                        ke=rr["diag"]["ke"], u_tau=rr["diag"]["u_tau"],
+                                        over_fp64=(round(rr["steps_per_s"] / results["fp64"]["steps_per_s"], 4)
+                                                   if "steps_per_s" in results.get("fp64", {}) else None))
+        except nsp.NspError as e:
+            out["mixed_refined"] = dict(error=str(e))
     if not args.no_ensemble:
         try:
             out["ensemble"] = run_ensemble(nsp, nsp.MIXED if head == "mixed" else nsp.FP64, args.ensemble_steps, 1)

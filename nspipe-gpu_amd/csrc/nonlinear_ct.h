@@ -1006,7 +1006,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
   X(768, Plan<3, 16, 16>)     \
   X(1536, Plan<6, 16, 16>)    \
   X(2304, Plan<9, 16, 16>)    \
-  X(4608, Plan<2, 9, 16, 16>)
+  X(4608, Plan<9, 2, 16, 16>)
 
 // ------------------------------------------------------------------ stage launchers
 // A form is compiled for a plan only if its LDS buffer fits the 160 KB of a CU.
@@ -1094,27 +1094,31 @@ bool zpass_plan(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T
   constexpr int LL = PL::L;
   const NlMember* tab = bt ? bt->tab : nullptr;
   const dim3 nb = bt ? dim3(bt->max_nr * g.ntp, bt->count) : dim3(g.nr * g.ntp);
-  if (v.z_alg == 1) {  // register form, 256 threads: one first-pass butterfly per thread
-    if constexpr (LL / PL::rad[0] <= 256) {
+  // register forms: one first-pass butterfly per thread, 256 threads (512
+  // when the first pass has 512 butterflies, as for 4608)
+  constexpr int NB1 = LL / PL::rad[0];
+  constexpr int RT = NB1 <= 256 ? 256 : 512;
+  if (v.z_alg == 1) {  // register form
+    if constexpr (NB1 <= 512) {
       constexpr int PAD = zpass2_pad<PL>();
       if (v.z_pf)
-        zpass2_ct<T, 256, 1, PAD, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+        zpass2_ct<T, RT, 1, PAD, PL, BAT><<<nb, RT, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
       else
-        zpass2_ct<T, 256, 0, PAD, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+        zpass2_ct<T, RT, 0, PAD, PL, BAT><<<nb, RT, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
       return true;
     } else {
       return false;
     }
   }
-  if (v.z_alg == 2) {  // triples of inverse transforms, 256 threads; z_pad: 3-column row padding 0, 1, R0
-    if constexpr (LL / PL::rad[0] <= 256 && lds_fits<T, Dense>(3 * LL + PL::rad[PL::P - 1] * PL::rad[0])) {
+  if (v.z_alg == 2) {  // triples of inverse transforms; z_pad: 3-column row padding 0, 1, R0
+    if constexpr (NB1 <= 512 && lds_fits<T, Dense>(3 * LL + PL::rad[PL::P - 1] * PL::rad[0])) {
       constexpr int PAD = zpass2_pad<PL>();
       if (v.z_pad == 0)
-        zpass3_ct<T, 256, PAD, 0, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+        zpass3_ct<T, RT, PAD, 0, PL, BAT><<<nb, RT, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
       else if (v.z_pad == 1)
-        zpass3_ct<T, 256, PAD, 1, PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+        zpass3_ct<T, RT, PAD, 1, PL, BAT><<<nb, RT, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
       else if (v.z_pad == 2)
-        zpass3_ct<T, 256, PAD, PL::rad[0], PL, BAT><<<nb, 256, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+        zpass3_ct<T, RT, PAD, PL::rad[0], PL, BAT><<<nb, RT, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
       else
         return false;
       return true;

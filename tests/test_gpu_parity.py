@@ -384,6 +384,30 @@ def test_graph_steps_match_eager_any_corrector_count(monkeypatch, tol, maxit):
     p2.close()
 
 
+def test_graph_steps_match_eager_headline_multi_corrector(monkeypatch):
+    """At the headline size (NR=96, NT=384, NZ=1537, mixed form), a tolerance
+    no corrector meets runs the second captured graph (correctors 2..maxit)
+    every step; it gives the bits, counts and flags of the launch-by-launch
+    step."""
+    cfg = dict(nr=96, nt=384, nz=1537, lz=10 * np.pi, re=11700.0, dt=0.001)
+    print("[gpu] headline graph vs eager, maxit 3", flush=True)
+    p1 = nsp.Pipe(**cfg, precision=nsp.MIXED, tol=1e-30, maxit=3)
+    p1.seeded_ic(20261202 + 3 * 1000, 4, 4, 0, 0.075)
+    n1, e1, f1 = p1.step(2)
+    a = p1.get_state()
+    p1.close()
+    monkeypatch.setenv("NSP_NO_GRAPH", "1")
+    p2 = nsp.Pipe(**cfg, precision=nsp.MIXED, tol=1e-30, maxit=3)
+    p2.seeded_ic(20261202 + 3 * 1000, 4, 4, 0, 0.075)
+    n2, e2, f2 = p2.step(2)
+    b = p2.get_state()
+    p2.close()
+    assert np.all(n1 == 3) and np.all(f1 == nsp.FLAG_MAXIT)
+    assert np.array_equal(n1, n2) and np.array_equal(f1, f2) and np.array_equal(e1, e2)
+    for c in range(3):
+        assert np.array_equal(a[c], b[c])
+
+
 def test_profile_counts_executed_launches():
     """Corrector iterations skipped inside a captured step are not counted."""
     p = nsp.Pipe(**CFG1)

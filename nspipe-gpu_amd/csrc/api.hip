@@ -368,7 +368,13 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
     return bail(fail(NSP_ERR_INVALID, "the mixed form needs compiled transform plans for 3NT/2 and 3(NZ-1)/2"));
   if (have_ct && (h->nb.mixed || !getenv("NSP_GENERIC_FFT"))) {
     const auto twt = pass_twiddles(ct_radices(g.ntp, true));
-    const auto twh = pass_twiddles(ct_radices(g.nh, false));
+    // the half-length plan's pass twiddles, then those of the pair form's full-length plan
+    auto twh = pass_twiddles(ct_radices(g.nh, false));
+    const auto rpair = ct_pair_radices(g.nh);
+    if (!rpair.empty()) {
+      const auto twp = pass_twiddles(rpair);
+      twh.insert(twh.end(), twp.begin(), twp.end());
+    }
     if (h->nb.mixed) {
       TRY(upload(h, &h->nb.ct_tw_t32, to_float2(twt)));
       TRY(upload(h, &h->nb.ct_tw_h32, to_float2(twh)));
@@ -657,6 +663,17 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
     const double t = time_it(1, v, lab);
     if (t >= 0 && t < bt) { bt = t; best.z_alg = 2; best.z_pad = pad; }
   }
+  for (int pad = 0; pad < 3; pad++)
+    for (int pf = 0; pf < 2; pf++) {
+      NlVariant v = best;
+      v.z_alg = 3;
+      v.z_pad = pad;
+      v.z_pf = pf;
+      char lab[64];
+      std::snprintf(lab, sizeof lab, "zpass pairs pad=%d pf=%d", pad, pf);
+      const double t = time_it(1, v, lab);
+      if (t >= 0 && t < bt) { bt = t; best.z_alg = 3; best.z_pad = pad; best.z_pf = pf; }
+    }
   // linear sweep: order of its (m, -m, axial chunk) workgroups
   bt = 1e30;
   for (int so = 0; so < 2; so++) {

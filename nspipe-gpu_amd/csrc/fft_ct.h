@@ -6,7 +6,7 @@
 // S_p into one of length S_p*R_p; the result comes out in natural order.
 // All radices, strides and divisors are compile-time constants, so index
 // arithmetic reduces to shifts and multiply-high sequences and the butterfly
-// loops unroll.  Butterflies of radix 6, 8, 9, 12, 16 are composite
+// loops unroll.  Butterflies of radix 6, 8, 9, 12, 16, 18 are composite
 // in-register DFTs (R = R1*R2) with constant twiddles evaluated by the
 // compiler.
 //
@@ -152,10 +152,14 @@ __device__ __forceinline__ void dft_composite(cpx<T>* v) {
 #pragma unroll
     for (int k1 = 1; k1 < R1; ++k1) {
       const int e = (n2 * k1) % R;
-      const T c = T(kTw<R>.c[e]);
-      const T s = T(SIGN) * T(kTw<R>.s[e]);
       const cpx<T> a = y[n2 * R1 + k1];
-      y[n2 * R1 + k1] = {a.x * c - a.y * s, a.x * s + a.y * c};
+      if (4 * e == R) {  // W_R^e = -i: a swap and a sign (no products by 0 and 1; e is a constant after unrolling)
+        y[n2 * R1 + k1] = {T(SIGN) * -a.y, T(SIGN) * a.x};
+      } else {
+        const T c = T(kTw<R>.c[e]);
+        const T s = T(SIGN) * T(kTw<R>.s[e]);
+        y[n2 * R1 + k1] = {a.x * c - a.y * s, a.x * s + a.y * c};
+      }
     }
 #pragma unroll
   for (int k1 = 0; k1 < R1; ++k1) {
@@ -177,6 +181,8 @@ template <typename T, int SIGN> struct DFT<T, 12, SIGN> {
   __device__ __forceinline__ static void run(cpx<T>* v) { dft_composite<T, 4, 3, SIGN>(v); } };
 template <typename T, int SIGN> struct DFT<T, 16, SIGN> {
   __device__ __forceinline__ static void run(cpx<T>* v) { dft_composite<T, 4, 4, SIGN>(v); } };
+template <typename T, int SIGN> struct DFT<T, 18, SIGN> {
+  __device__ __forceinline__ static void run(cpx<T>* v) { dft_composite<T, 9, 2, SIGN>(v); } };
 
 // ---------------------------------------------------------------- plans
 template <int... R>

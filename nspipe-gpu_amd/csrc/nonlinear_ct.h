@@ -4,6 +4,8 @@
 // T = double for the FP64 form, float for the FP32 transforms of the mixed
 // form.  Lengths without a compiled plan use the generic kernels.
 #pragma once
+#include <type_traits>
+
 #include "fft_ct.h"
 
 namespace nsp {
@@ -734,6 +736,208 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   }
 }
 
+// ------------------------------------------------------------------ z pass, pair form
+// The real inverse transforms of two fields a, b of a row run as one complex
+// transform of the full length NZF: z = a + i b, from Z[k] = A[k] + i B[k]
+// with A[NZF - k] = conj A[k] (the stored modes k < NZ, zero up to NZF/2;
+// the l = 0 mode enters with its real part only, as in the C2R forms).  The
+// real forward transforms of two products likewise, separated afterwards:
+// A[k] = (Z[k] + conj Z[NZF - k]) / 2, B[k] = (Z[k] - conj Z[NZF - k]) / 2i.
+// No even/odd split and no split twiddles; the LDS buffer holds one
+// full-length column.  The thread with first-pass butterfly n0 owns the
+// physical samples n0 + (NZF / R0) q, q < R0, of every field (the outputs it
+// reads after an inverse transform are the inputs of its first forward
+// pass).  Six inverse pairs, (u_r, u_t), (u_z, d_z u_r), (d_r u_r,
+// d_theta u_r / r), (d_r u_t, d_theta u_t / r), (d_z u_t, d_z u_z), (d_r u_z,
+// d_theta u_z / r), and two forward pairs, (N_r, N_t) and (N_z, 0).  The
+// plan's pass twiddles follow the half-length plan's in tw (NZF/2 - 1
+// entries).
+template <typename T, int NTHR, int PAD, int PFE, typename PF, int BAT>
+__global__ __launch_bounds__(NTHR) void zpass4_ct(Geom g, const cpx<T>* __restrict__ tw,
+                                                  const cpx<T>* __restrict__ twz, const double* __restrict__ r,
+                                                  const cpx<T>* __restrict__ W, size_t ws,
+                                                  cpx<T>* __restrict__ Y, const NlMember* tab) {
+  if constexpr (BAT != 0) {
+    const NlMember& m = tab[blockIdx.y];
+    g = m.g;
+    r = gmem(tab, m.r);
+    W = gmem(tab, reinterpret_cast<const cpx<T>*>(m.W));
+    Y = gmem(tab, reinterpret_cast<cpx<T>*>(m.Y));
+    ws = m.ws;
+  }
+  (void)twz;
+  if (step_done(g)) return;
+  constexpr int NF = PF::L;
+  constexpr int NH = NF / 2;
+  constexpr int R0 = PF::rad[0];
+  constexpr int NB1 = NF / R0;
+  static_assert(NB1 <= NTHR, "one first-pass butterfly per thread");
+  using LY = Lay<NF / PF::rad[PF::P - 1], PAD>;
+  __shared__ cpx<T> buf[LY::size(NF)];
+  const cpx<T>* __restrict__ twf = tw + (NH - 1);
+  const int row = blockIdx.x;
+  if (row >= g.nr * g.ntp) return;  // beyond this member's extent
+  const int i = row / g.ntp;
+  const int tid = threadIdx.x;
+  const bool own = NB1 == NTHR || tid < NB1;
+  const int nz = g.nz;
+  const T rad = T(r[i]);
+  const T irad = T(1) / rad;
+  auto over_r = [&](T x) -> T {
+    if constexpr (sizeof(T) == 4) return x * irad;
+    else return x / rad;
+  };
+  const int th = row - i * g.ntp;
+  const size_t roff = (size_t)i * tp_plane(g) + (size_t)th * 64;
+  const int rbytes = (int)((tp_plane(g) - (size_t)th * 64) * sizeof(cpx<T>));
+  const T kz0 = T(g.kz0);
+  const cpx<T> zero{T(0), T(0)};
+  constexpr int CS = (int)sizeof(cpx<T>);
+  constexpr int OOB = 0x7ffffff0;
+  // sample n = n0 + NB1 q reads mode m = n (n <= NZF/2) or NZF - n (conjugated)
+  auto conj_of = [](int lt, int q) -> bool {
+    if (NB1 * (q + 1) <= NH + 1) return false;  // every lane: n <= NH
+    if (NB1 * q > NH) return true;              // every lane: n > NH
+    return lt + NB1 * q > NH;
+  };
+  // the stored modes m of fields fa, fb at the owned samples, zero past NZ
+  // (range-checked loads).  With NB1 and NZF multiples of 64, the tile-major
+  // offsets of m = n and m = NZF - n are affine in q.
+  auto load = [&](int fa, int fb, cpx<T>* A, cpx<T>* B) {
+    const int lt = opaque(tid);
+    if (own) {
+      const nsp_i4v ra = buf_rsrc(W + fa * ws + roff, rbytes);
+      const nsp_i4v rb = buf_rsrc(W + fb * ws + roff, rbytes);
+      const int t64 = g.ntp * 64;
+#pragma unroll
+      for (int q = 0; q < R0; q++) {
+        const int n = lt + NB1 * q;
+        int o;
+        if constexpr (NB1 % 64 == 0 && NF % 64 == 0) {
+          const int dq = (NB1 / 64) * t64 * CS;
+          const int ok = ((lt >> 6) * t64 + (lt & 63)) * CS + q * dq;
+          const int oc = (((NF >> 6) - ((lt + 63) >> 6)) * t64 + ((-lt) & 63)) * CS - q * dq;
+          const bool cj = conj_of(lt, q);
+          o = cj ? (NF - n < nz ? oc : OOB) : (n < nz ? ok : OOB);
+        } else {
+          const int m = conj_of(lt, q) ? NF - n : n;
+          o = m < nz ? (int)tp_col(g, m) * CS : OOB;
+        }
+        A[q] = buf_load<T>(ra, o);
+        B[q] = buf_load<T>(rb, o);
+      }
+    }
+  };
+  // z = a + i b at the owned samples (d_z applied first where asked) and the first pass
+  auto first = [&](const cpx<T>* A, const cpx<T>* B, bool dza, bool dzb) {
+    const int lt = opaque(tid);
+    if (own) {
+      cpx<T> v[R0];
+#pragma unroll
+      for (int q = 0; q < R0; q++) {
+        const int n = lt + NB1 * q;
+        const bool cj = conj_of(lt, q);
+        cpx<T> a = A[q], b = B[q];
+        if (q == 0 && n == 0) { a.y = T(0); b.y = T(0); }
+        if (dza || dzb) {  // derivz_k (mod_deriv.cu:24-35) on the stored mode
+          const T km = kz0 * T(cj ? NF - n : n);
+          if (dza) a = cpx<T>{-km * a.y, km * a.x};
+          if (dzb) b = cpx<T>{-km * b.y, km * b.x};
+        }
+        v[q] = cj ? cpx<T>{a.x + b.y, b.x - a.y} : cpx<T>{a.x - b.y, a.y + b.x};
+      }
+      first_pass_store<T, +1, LY, 1>(PF{}, v, buf, lt);
+    }
+  };
+  cpx<T> A[R0], B[R0];
+  cpx<T> s[R0];
+  // inverse pair from A, B, then the loads of the next pair (if any): PFE 1
+  // issues them right after the first pass (in flight during the LDS
+  // passes), PFE 0 after the outputs are read (fewer live registers)
+  auto inverse = [&](bool dza, bool dzb, int fa, int fb) {
+    first(A, B, dza, dzb);
+    if (PFE && fa >= 0) load(fa, fb, A, B);
+    __syncthreads();
+    const int lt = opaque(tid);
+    fft_rest<T, NTHR, +1, LY>(PF{}, buf, twf, lt);
+#pragma unroll
+    for (int q = 0; q < R0; q++) s[q] = own ? buf[LY::pos(lt + NB1 * q)] : zero;
+    __syncthreads();
+    if (!PFE && fa >= 0) load(fa, fb, A, B);
+  };
+  // forward pair of the products pa (+ i pb), separated into Y components c0 (and c0 + 1)
+  auto forward = [&](const T* pa, const T* pb, int c0) {
+    const int lt = opaque(tid);
+    if (own) {
+      cpx<T> v[R0];
+#pragma unroll
+      for (int q = 0; q < R0; q++) v[q] = cpx<T>{pa[q], pb ? pb[q] : T(0)};
+      first_pass_store<T, -1, LY, 1>(PF{}, v, buf, lt);
+    }
+    __syncthreads();
+    fft_rest<T, NTHR, -1, LY>(PF{}, buf, twf, lt);
+    cpx<T>* da = Y + c0 * ws + roff;
+    cpx<T>* db = Y + (c0 + 1) * ws + roff;
+    const int t64 = g.ntp * 64;
+    for (int k = lt; k < nz; k += NTHR) {
+      cpx<T> xa = zero, xb = zero;
+      if (k < nz - 1) {
+        const cpx<T> zk = buf[LY::pos(k)], zc = buf[LY::pos(k == 0 ? 0 : NF - k)];
+        xa = cpx<T>{T(0.5) * (zk.x + zc.x), T(0.5) * (zk.y - zc.y)};
+        xb = cpx<T>{T(0.5) * (zk.y + zc.y), T(-0.5) * (zk.x - zc.x)};
+      }
+      const int o = (k >> 6) * t64 + (k & 63);  // tp_col(g, k)
+      da[o] = xa;
+      if (pb) db[o] = xb;
+    }
+    __syncthreads();
+  };
+  // field index in W: u_r, u_t, u_z (0-2), d_r u_c (3 + c), d_theta u_c / r (6 + c)
+  T ur[R0], ut[R0], uz[R0], hz[R0], n0[R0], n1[R0];
+  load(0, 1, A, B);
+  inverse(false, false, 2, 0);
+#pragma unroll
+  for (int q = 0; q < R0; q++) { ur[q] = s[q].x; ut[q] = s[q].y; }
+  inverse(false, true, 3, 6);
+#pragma unroll
+  for (int q = 0; q < R0; q++) { uz[q] = s[q].x; hz[q] = s[q].y; }
+  // calcNonlinear_k (mod_io_nonlinear.cu:142-170), summed in its order
+  inverse(false, false, 4, 7);
+#pragma unroll
+  for (int q = 0; q < R0; q++) {
+    T v = ur[q] * s[q].x;
+    v += ut[q] * s[q].y;
+    v = v + uz[q] * hz[q];
+    v = v - ut[q] * over_r(ut[q]);
+    n0[q] = -v;
+  }
+  inverse(false, false, 1, 2);
+#pragma unroll
+  for (int q = 0; q < R0; q++) {
+    T v = ur[q] * s[q].x;
+    v += ut[q] * s[q].y;
+    n1[q] = v;
+  }
+  inverse(true, true, 5, 8);
+#pragma unroll
+  for (int q = 0; q < R0; q++) {
+    T v = n1[q] + uz[q] * s[q].x;
+    v = v + ur[q] * over_r(ut[q]);
+    n1[q] = -v;
+    hz[q] = s[q].y;
+  }
+  forward(n0, n1, 0);
+  inverse(false, false, -1, -1);
+#pragma unroll
+  for (int q = 0; q < R0; q++) {
+    T v = ur[q] * s[q].x;
+    v += ut[q] * s[q].y;
+    v = v + uz[q] * hz[q];
+    n0[q] = -v;
+  }
+  forward(n0, nullptr, 2);
+}
+
 // ------------------------------------------------------------------ theta forward
 template <typename T, int TL, int NTHR, typename LY, typename PL, int BAT>
 __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __restrict__ tw,
@@ -1000,6 +1204,28 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
   X(576, Plan<4, 9, 16>)   \
   X(1152, Plan<8, 9, 16>)
 
+// Full-length plans NZF = 2 NH of the pair form, keyed by NH (one first-pass
+// butterfly per thread: NZF / R0 threads).
+#define NSP_ZPAIR_PLANS(X)        \
+  X(96, Plan<3, 8, 8>)            \
+  X(576, Plan<9, 8, 16>)          \
+  X(768, Plan<3, 8, 8, 8>)        \
+  X(1536, Plan<6, 8, 8, 8>)       \
+  X(2304, Plan<9, 8, 8, 8>)       \
+  X(4608, Plan<18, 8, 8, 8>)
+
+template <int NH>
+struct ZPair {
+  using P = void;
+};
+#define X(LL, ...)          \
+  template <>               \
+  struct ZPair<LL> {        \
+    using P = __VA_ARGS__;  \
+  };
+NSP_ZPAIR_PLANS(X)
+#undef X
+
 #define NSP_ZHALF_PLANS(X)    \
   X(96, Plan<6, 16>)          \
   X(576, Plan<4, 9, 16>)      \
@@ -1125,6 +1351,29 @@ bool zpass_plan(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T
     } else {
       return false;
     }
+  }
+  if (v.z_alg == 3) {  // pairs of real transforms as full-length complex ones; z_pad: row padding 0, 1, R0
+    using PP = typename ZPair<LL>::P;
+    if constexpr (!std::is_void_v<PP>) {
+      constexpr int NT4 = PP::L / PP::rad[0];
+      constexpr int RL4 = PP::L / PP::rad[PP::P - 1];
+      if constexpr (NT4 % 64 == 0 && NT4 <= 1024 && lds_fits<T, Lay<RL4, PP::rad[0]>>(PP::L)) {
+        auto go = [&](auto pf) {
+          constexpr int PFE = decltype(pf)::value;
+          if (v.z_pad == 0)
+            zpass4_ct<T, NT4, 0, PFE, PP, BAT><<<nb, NT4, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+          else if (v.z_pad == 1)
+            zpass4_ct<T, NT4, 1, PFE, PP, BAT><<<nb, NT4, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+          else if (v.z_pad == 2)
+            zpass4_ct<T, NT4, PP::rad[0], PFE, PP, BAT><<<nb, NT4, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+          else
+            return false;
+          return true;
+        };
+        return v.z_pf ? go(std::integral_constant<int, 1>{}) : go(std::integral_constant<int, 0>{});
+      }
+    }
+    return false;
   }
   if (v.z_alg != 0) return false;
   using RP = Lay<LL / PL::rad[PL::P - 1], zpass2_pad<PL>()>;

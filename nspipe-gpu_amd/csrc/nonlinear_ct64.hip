@@ -46,4 +46,15 @@ std::vector<int> ct_radices(int L, bool theta) {
 #undef X
   return {};
 }
+// radix list of the pair form's full-length plan for half length NH (empty if none)
+std::vector<int> ct_pair_radices(int nh) {
+#define X(LL, ...)                                                                 \
+  if (nh == LL) {                                                                  \
+    using PP = __VA_ARGS__;                                                        \
+    return std::vector<int>(PP::rad, PP::rad + PP::P);                             \
+  }
+  NSP_ZPAIR_PLANS(X)
+#undef X
+  return {};
+}
 }  // namespace nsp

@@ -239,11 +239,13 @@ struct ThetaVar {
 struct NlVariant {
   ThetaVar inv, fwd;
   int z_alg;   // z pass: 0 LDS-scatter form, 1 register form (256 threads), 2 register form with the
-               // inverse transforms in triples
+               // inverse transforms in triples, 3 pair form (two real transforms as one full-length
+               // complex transform)
   int z_nthr;  // scatter form: threads per row (256 or 512)
   int z_pad;   // scatter form: 1 = one LDS slot per 16 elements, 2 = row padding as the register form;
-               // triples: row padding of the three-column layout 0, 1 or R0 slots
-  int z_pf;    // register form: prefetch the next transform's inputs
+               // triples: row padding of the three-column layout 0, 1 or R0 slots; pairs: row
+               // padding 0, 1 or R0 slots
+  int z_pf;    // register and pair forms: load the next transform's inputs before the LDS passes
   int s_ord;   // linear sweep: 1 = the chunks of an azimuthal row XCD-contiguous, 0 = dispatch order
 };
 // Nonlinear-phase buffers and FFT data.
@@ -261,7 +263,7 @@ struct NlBufs {
   // compile-time plans (when the lengths have one): per-pass twiddles
   bool use_ct;
   double2* ct_tw_t;    // [NTP-1]
-  double2* ct_tw_h;    // [NH-1]
+  double2* ct_tw_h;    // [NH-1] half-length z plan, then [NZF-1] of the pair form's plan if compiled
   // mixed form: FP32 transforms and products
   bool mixed;
   float2* W32;         // [9][NR*NTP*NZ]
@@ -341,6 +343,7 @@ bool ct_has_theta(int L);
 bool ct_has_zhalf(int L);
 bool ct_has_zreg(int L);
 std::vector<int> ct_radices(int L, bool theta);
+std::vector<int> ct_pair_radices(int nh);
 // per-pass twiddles of a plan: for each pass (radix R, sub-length S) S*(R-1)
 // entries exp(-2 pi i q k / (S R)) at [k*(R-1) + q-1]
 std::vector<double2> pass_twiddles(const std::vector<int>& radices);

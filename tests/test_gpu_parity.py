@@ -201,6 +201,9 @@ def _variants():
         out.append(dict(base, z_alg=1, z_pf=pf))
     for pad in (0, 1, 2):
         out.append(dict(base, z_alg=2, z_pad=pad))
+    for pad in (0, 1, 2):
+        for pf in (0, 1):
+            out.append(dict(base, z_alg=3, z_pad=pad, z_pf=pf))
     return out
 
 

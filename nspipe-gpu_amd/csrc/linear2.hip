@@ -361,8 +361,14 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
   // of the fluctuation modes; the mean flow and low |m| stay FP64).  Both
   // waves have the same |m| (pair 0, rows 0 and NT/2, stays FP64).
   const bool s32b = A.s32 != 0 && pair >= A.m32;
-  auto body = [&](auto s32_tag) {
+  // The factor precision of this wave (F32W) and the storage of N(u) (N32,
+  // FP32 in the mixed form) are compile-time branches too: run-time tests
+  // inside the walks made the compiler drain outstanding loads before the
+  // loads of each branch (two or three memory round trips per row).
+  auto body = [&](auto s32_tag, auto f32_tag, auto n32_tag) {
   constexpr bool S32 = decltype(s32_tag)::value;
+  constexpr bool F32W = decltype(f32_tag)::value;
+  constexpr bool N32 = decltype(n32_tag)::value;
   const Scr<S32> SR = Scr<S32>::at(A.SR, (size_t)lb * nr * 384);
   const Scr<S32> SP = Scr<S32>::at(A.SP, (size_t)lb * nr * 128);
   const Scr<S32> E = Scr<S32>::at(A.E, (size_t)lb * nr * 384);
@@ -384,12 +390,15 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     const double ire = 1.0 / g.re, aim = d_im / g.re, idt = 1.0 / g.dt;
     for (int t = 0; t < nr + 6; t++) {
       double2 a = z2, b = z2, c = z2;
-      if (t < nr) {
-        const size_t h = (size_t)t * MS + k;
-        a = A.u[h];
-        b = A.u[n + h];
-        c = A.u[2 * n + h];
-        pm_of(a, b);
+      {  // row t, or zero past the wall: the loads themselves stay in bounds
+        const size_t h = (size_t)(t < nr ? t : nr - 1) * MS + k;
+        const double2 a0 = A.u[h], b0 = A.u[n + h], c0 = A.u[2 * n + h];
+        if (t < nr) {
+          a = a0;
+          b = b0;
+          c = c0;
+          pm_of(a, b);
+        }
       }
 #pragma unroll
       for (int w = 0; w < 6; w++) {
@@ -405,7 +414,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       if (j >= 0 && j < nr) {
         const size_t hn = nb0 + (size_t)j * npl;  // tile-major N(u^n)
         double2 nn[3];
-        if (A.n32) {
+        if constexpr (N32) {
           const float2* N32 = reinterpret_cast<const float2*>(A.N);
           const float2 f0 = N32[hn], f1 = N32[ncs + hn], f2 = N32[2 * ncs + hn];
           nn[0] = make_double2(f0.x, f0.y);
@@ -502,12 +511,15 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     for (int t = 0; t < nr + LAG; t++) {
       if (XPRED) {
         double2 a = z2, b = z2, c = z2;
-        if (t < nr) {
-          const size_t h = (size_t)t * MS + k;
-          a = A.u[h];
-          b = A.u[n + h];
-          c = A.u[2 * n + h];
-          pm_of(a, b);
+        {  // row t, or zero past the wall: the loads themselves stay in bounds
+          const size_t h = (size_t)(t < nr ? t : nr - 1) * MS + k;
+          const double2 a0 = A.u[h], b0 = A.u[n + h], c0 = A.u[2 * n + h];
+          if (t < nr) {
+            a = a0;
+            b = b0;
+            c = c0;
+            pm_of(a, b);
+          }
         }
 #pragma unroll
         for (int w = 0; w < 6; w++) {
@@ -521,11 +533,20 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       }
       // rhs row j
       const int j = XPRED ? t - 3 : t;
+      // pressure elimination row i = j - 3: its factors (and Dh) are loaded
+      // before this iteration's stores, which the compiler would not move
+      // loads across (one memory round trip per row instead of two)
+      const int i = j - 3;
+      const int ic = i < 0 ? 0 : (i < nr ? i : nr - 1);
+      const double* fpi = A.F + fb_p + (size_t)ic * FR_ABS;
+      const double fl0 = fpi[0], fl1 = fpi[64], fl2 = fpi[128];
+      double2 dhi = z2;
+      if constexpr (INC) dhi = DH.ld(sb1 + (size_t)ic * 128);
       double2 R0 = z2, R1 = z2, R2 = z2;
       if (j >= 0 && j < nr) {
         const size_t hn = nb0 + (size_t)j * npl;  // tile-major N(u)
         double2 n0, n1, n2;
-        if (A.n32) {
+        if constexpr (N32) {
           const float2* N32 = reinterpret_cast<const float2*>(A.N);
           const float2 f0 = N32[hn], f1 = N32[ncs + hn], f2 = N32[2 * ncs + hn];
           n0 = make_double2(f0.x, f0.y);
@@ -592,7 +613,6 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       Rt[3] = R1;
       Rz[3] = R2;
       // divergence and pressure elimination at row i = j - 3
-      const int i = j - 3;
       if (i >= 0 && i < nr) {
         double2 dv = z2;
         if (i < nr - 1) {  // div_k (mod_radialFD.cu:67-96), wall row zero
@@ -613,13 +633,11 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
           py += q.kz * uz.x;
           dv = make_double2(px, py);
           if constexpr (INC) {  // div(rhs) = div(rhs - H u^n) + div(H u^n)
-            const double2 dh = DH.ld(sb1 + (size_t)i * 128);
-            dv.x += dh.x;
-            dv.y += dh.y;
+            dv.x += dhi.x;
+            dv.y += dhi.y;
           }
         }
-        const double* fp = A.F + fb_p + (size_t)i * FR_ABS;
-        const double l0 = fp[0], l1 = fp[64], l2 = fp[128];
+        const double l0 = fl0, l1 = fl1, l2 = fl2;
         dv.x -= l0 * yp[0].x;
         dv.y -= l0 * yp[0].y;
         dv.x -= l1 * yp[1].x;
@@ -672,9 +690,29 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     for (int t = 0; t < nr + IW; t++) {
 #pragma unroll
       for (int w = 0; w < 6; w++) pw[w] = pw[w + 1];
-      pw[6] = (t < nr) ? SP.ld(sb1 + (size_t)t * 128) : z2;
+      {  // row t, or zero past the wall: the load itself stays in bounds
+        const double2 pv = SP.ld(sb1 + (size_t)(t < nr ? t : nr - 1) * 128);
+        pw[6] = (t < nr) ? pv : z2;
+      }
       const int j = t - IW;
       if (j < 0) continue;
+      // the L factors of the three solves at row j, loaded before the
+      // row's stores (the compiler does not move loads across them)
+      double Lc3[3][IW];
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        // u+, u-: rows of FR_UP, slots of 128; uz: rows of FR_ABS, slots of 64
+        const size_t fo = c == 0 ? fb_up + (size_t)j * FR_UP
+                                 : (c == 1 ? fb_um + (size_t)j * FR_UP : fb_uz + (size_t)j * FR_ABS);
+        const int fs = c == 2 ? 64 : 128;
+        if constexpr (F32W) {
+#pragma unroll
+          for (int w = 0; w < IW; w++) Lc3[c][w] = A.F32[fo + w * fs];
+        } else {
+#pragma unroll
+          for (int w = 0; w < IW; w++) Lc3[c][w] = A.F[fo + w * fs];
+        }
+      }
       // addGrad_k (mod_radialFD.cu:115-143)
       const size_t h = sb3 + (size_t)j * 384;
       const ctab Dr = Dcp + STEN * j;
@@ -708,18 +746,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       }
 #pragma unroll
       for (int c = 0; c < 3; c++) {  // forward elimination
-        // u+, u-: rows of FR_UP, slots of 128; uz: rows of FR_ABS, slots of 64
-        const size_t fo = c == 0 ? fb_up + (size_t)j * FR_UP
-                                 : (c == 1 ? fb_um + (size_t)j * FR_UP : fb_uz + (size_t)j * FR_ABS);
-        const int fs = c == 2 ? 64 : 128;
-        double Lc[IW];
-        if (f32w) {
-#pragma unroll
-          for (int w = 0; w < IW; w++) Lc[w] = A.F32[fo + w * fs];
-        } else {
-#pragma unroll
-          for (int w = 0; w < IW; w++) Lc[w] = A.F[fo + w * fs];
-        }
+        const double* Lc = Lc3[c];
         double2 y = rh[c];
         y.x -= Lc[0] * yw[c][0].x;
         y.y -= Lc[0] * yw[c][0].y;
@@ -753,7 +780,11 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
                                  : (c == 1 ? fb_um + (size_t)i * FR_UP : fb_uz + (size_t)i * FR_ABS);
         const int fs = c == 2 ? 64 : 128;
         double ru, u1, u2, u3;
-        if (f32w) {
+        // FP32 factors by the body's compile-time flag (a run-time test
+        // here made the scheduler serialise the FP32 loads of the mixed
+        // form); the HOLD kernels (FP64 form) keep the run-time test, with
+        // which they were measured 3-7% faster
+        if (HOLD ? f32w : F32W) {
           ru = A.F32[fo + 3 * fs];
           u1 = A.F32[fo + 4 * fs];
           u2 = A.F32[fo + 5 * fs];
@@ -859,10 +890,14 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       }
     }
     double num = 0.0, den = 0.0;
-    auto emit = [&](int i, const double2* xo, const double* ho) {
+    // emit row i: pd = {dr p_h, p_h} and uc (the corrector's previous
+    // iterate) come from a prefetched row, or are loaded here when null
+    auto emit = [&](int i, const double2* xo, const double* ho, const double* pd = nullptr,
+                    const double2* uc = nullptr) {
       const size_t h = (size_t)i * MS + k;
       // pressure-gradient homogeneous solution: (dr p_h -+ kt p_h / r, 0), (0, kz p_h)
-      const double dph = A.PH[phb + (size_t)i * 128], ph = A.PH[phb + (size_t)i * 128 + 64];
+      const double dph = pd ? pd[0] : A.PH[phb + (size_t)i * 128];
+      const double ph = pd ? pd[1] : A.PH[phb + (size_t)i * 128 + 64];
       const double tq = q.kt * ph / rt[i];
       const double2 hp[3] = {make_double2(dph - tq, 0.0), make_double2(dph + tq, 0.0), make_double2(-0.0, q.kz * ph)};
       const double2 hv[3] = {make_double2(ho[0], 0.0), make_double2(ho[1], 0.0), make_double2(0.0, ho[2])};
@@ -879,7 +914,7 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
       if (!PRED && k != 0) {  // mcor_k; mode (0,0) is measured after its fix-up
 #pragma unroll
         for (int c = 0; c < 3; c++) {
-          const double2 u1 = A.u[c * n + h];
+          const double2 u1 = uc ? uc[c] : A.u[c * n + h];
           const double dx = u1.x - o[c].x, dy = u1.y - o[c].y;
           const double a1 = dx * dx + dy * dy, a2 = u1.x * u1.x + u1.y * u1.y;
           num = (a1 > num || a1 != a1) ? a1 : num;
@@ -893,12 +928,96 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     if constexpr (HOLD) {
 #pragma unroll
       for (int s = 0; s < 4; s++) emit(nr - 1 - s, xs[s], xhs[s]);
-    }
-    for (int i = HOLD ? nr - 5 : nr - 1; i >= 0; i--) {
-      double2 xo[3];
-      double ho[3];
-      backsub(i, xo, ho);
-      emit(i, xo, ho);
+      for (int i = nr - 5; i >= 0; i--) {
+        double2 xo[3];
+        double ho[3];
+        backsub(i, xo, ho);
+        emit(i, xo, ho);
+      }
+    } else {
+      // The walk that emits the solution, software-pipelined: the loads of
+      // row i - 1 (factors, column, p_h, previous iterate) are issued before
+      // row i is solved and stored, so each row's memory latency overlaps
+      // the previous row's arithmetic (the compiler does not move loads
+      // across the stores of a row by itself).  Same arithmetic as backsub.
+      using FT = std::conditional_t<F32W, float, double>;
+      struct RowD {
+        FT f[3][4];      // ru, u1, u2, u3 of the u+, u-, uz solves
+        double2 x[3];    // eliminated column rows
+        double pd[2];    // dr p_h, p_h
+        double2 uc[3];   // corrector: previous iterate
+        double2 un[3];   // INC: u^n (+/- form for the velocity pair)
+      };
+      auto loadrow = [&](int i, RowD& R) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const size_t fo = c == 0 ? fb_up + (size_t)i * FR_UP
+                                   : (c == 1 ? fb_um + (size_t)i * FR_UP : fb_uz + (size_t)i * FR_ABS);
+          const int fs = c == 2 ? 64 : 128;
+#pragma unroll
+          for (int w = 0; w < 4; w++) {
+            if constexpr (F32W) R.f[c][w] = A.F32[fo + (3 + w) * fs];
+            else R.f[c][w] = A.F[fo + (3 + w) * fs];
+          }
+          R.x[c] = SR.ld(sb3 + (size_t)i * 384 + c * 128);
+        }
+        R.pd[0] = A.PH[phb + (size_t)i * 128];
+        R.pd[1] = A.PH[phb + (size_t)i * 128 + 64];
+        const size_t h = (size_t)i * MS + k;
+        if constexpr (!PRED) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) R.uc[c] = A.u[c * n + h];
+        }
+        if constexpr (INC) {
+#pragma unroll
+          for (int c = 0; c < 3; c++) R.un[c] = A.un[c * n + h];
+        }
+      };
+      auto solve = [&](int i, const RowD& R, double2* xo, double* ho) {
+#pragma unroll
+        for (int c = 0; c < 3; c++) {
+          const double ru = R.f[c][0], u1 = R.f[c][1], u2 = R.f[c][2], u3 = R.f[c][3];
+          double2 x = R.x[c];
+          x.x -= u1 * xw[c][0].x;
+          x.y -= u1 * xw[c][0].y;
+          x.x -= u2 * xw[c][1].x;
+          x.y -= u2 * xw[c][1].y;
+          x.x -= u3 * xw[c][2].x;
+          x.y -= u3 * xw[c][2].y;
+          x = make_double2(x.x * ru, x.y * ru);
+          double hh = (i == nr - 1) ? 1.0 : 0.0;
+          hh -= u1 * xh[c][0];
+          hh -= u2 * xh[c][1];
+          hh -= u3 * xh[c][2];
+          hh *= ru;
+          xw[c][2] = xw[c][1];
+          xw[c][1] = xw[c][0];
+          xw[c][0] = x;
+          xh[c][2] = xh[c][1];
+          xh[c][1] = xh[c][0];
+          xh[c][0] = hh;
+          xo[c] = x;
+          ho[c] = hh;
+        }
+        if constexpr (INC) {  // the solution is u^n + delta (FP64)
+          double2 a = R.un[0], b = R.un[1];
+          pm_of(a, b);
+          const double2 c = R.un[2];
+          xo[0] = make_double2(xo[0].x + a.x, xo[0].y + a.y);
+          xo[1] = make_double2(xo[1].x + b.x, xo[1].y + b.y);
+          xo[2] = make_double2(xo[2].x + c.x, xo[2].y + c.y);
+        }
+      };
+      RowD cur, nxt;
+      loadrow(nr - 1, cur);
+      for (int i = nr - 1; i >= 0; i--) {
+        loadrow(i > 0 ? i - 1 : 0, nxt);
+        double2 xo[3];
+        double ho[3];
+        solve(i, cur, xo, ho);
+        emit(i, xo, ho, cur.pd, PRED ? nullptr : cur.uc);
+        cur = nxt;
+      }
     }
     if (k == 0) {  // modeZZ_k, adj_flux (mod_integrate.cu:63-68, mod_cublasflux.cu:90-103)
       double qa = 0.0;
@@ -949,8 +1068,17 @@ __global__ __launch_bounds__(128) void sweep_kernel(Lin2Args A, const Lin2Args* 
     }
   }
   };
-  if (s32b) body(std::true_type{});
-  else body(std::false_type{});
+  // the FP64 form has FP64 N(u), scratch and factors (s32b and f32w false)
+  using T1 = std::true_type;
+  using T0 = std::false_type;
+  if (!A.n32) body(T0{}, T0{}, T0{});
+  else if (s32b) {
+    if (f32w) body(T1{}, T1{}, T1{});
+    else body(T1{}, T0{}, T1{});
+  } else {
+    if (f32w) body(T0{}, T1{}, T1{});
+    else body(T0{}, T0{}, T1{});
+  }
 }
 
 size_t factor_count(const Geom& g) { return f_abs(g) + 2 * f_absn(g); }

@@ -618,9 +618,10 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
     bt = 1e30;
     std::vector<ThetaVar> cands;
     for (int ord = 0; ord < 2; ord++) {
-      for (int pad = 0; pad < 3; pad++)
+      for (int pad = 0; pad < 4; pad++)
         for (int tl : {4, 8}) cands.push_back(ThetaVar{0, tl, 256, pad, ord});
       cands.push_back(ThetaVar{0, 16, 256, 0, ord});
+      cands.push_back(ThetaVar{0, 16, 256, 3, ord});
       for (const auto& tr : treg) cands.push_back(ThetaVar{1, tr[0], tr[1], 0, ord});
     }
     for (const ThetaVar& tv : cands) {
@@ -663,17 +664,18 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
     const double t = time_it(1, v, lab);
     if (t >= 0 && t < bt) { bt = t; best.z_alg = 2; best.z_pad = pad; }
   }
-  for (int pad = 0; pad < 3; pad++)
-    for (int pf = 0; pf < 2; pf++) {
-      NlVariant v = best;
-      v.z_alg = 3;
-      v.z_pad = pad;
-      v.z_pf = pf;
-      char lab[64];
-      std::snprintf(lab, sizeof lab, "zpass pairs pad=%d pf=%d", pad, pf);
-      const double t = time_it(1, v, lab);
-      if (t >= 0 && t < bt) { bt = t; best.z_alg = 3; best.z_pad = pad; best.z_pf = pf; }
-    }
+  for (int alg : {3, 4})
+    for (int pad = 0; pad < 3; pad++)
+      for (int pf = 0; pf < 2; pf++) {
+        NlVariant v = best;
+        v.z_alg = alg;
+        v.z_pad = pad;
+        v.z_pf = pf;
+        char lab[64];
+        std::snprintf(lab, sizeof lab, "zpass pairs%s pad=%d pf=%d", alg == 4 ? " 2 rows" : "", pad, pf);
+        const double t = time_it(1, v, lab);
+        if (t >= 0 && t < bt) { bt = t; best.z_alg = alg; best.z_pad = pad; best.z_pf = pf; }
+      }
   // linear sweep: order of its (m, -m, axial chunk) workgroups
   bt = 1e30;
   for (int so = 0; so < 2; so++) {

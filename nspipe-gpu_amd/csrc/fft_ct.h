@@ -238,6 +238,40 @@ struct Lay {
 };
 using Dense = Lay<1, 0>;
 
+// Swizzled layout of W interleaved columns (W a power of two): logical
+// element e = col + n W sits in row n, stored as row n ^ (((n >> B) ^
+// (n >> 2B)) & (2^B - 1)).  Only the low B bits of the row index change, by
+// a function of the higher bits: a permutation inside each aligned group of
+// 2^B rows (no padding).  With 2^B rows per 256-byte bank period, the rows
+// that the digit-reversed scatter and the pass patterns of the theta plans
+// touch together land in different bank groups.  Not affine.
+template <int W, int B>
+struct Swz {
+  __device__ __forceinline__ static int pos(int e) {
+    if constexpr (B == 0) {
+      return e;
+    } else {
+      constexpr int M = (1 << B) - 1;
+      const int n = e / W, col = e - n * W;
+      return (n ^ (((n >> B) ^ (n >> (2 * B))) & M)) * W + col;
+    }
+  }
+  static constexpr int size(int n) { return n; }
+  template <int SE, int SPAN>
+  __device__ __forceinline__ static void affine(int first, int& p0, int& step) {
+    p0 = first;
+    step = SE;
+  }
+  template <int SE, int SPAN>
+  static constexpr bool is_affine() { return B == 0; }
+};
+// bits B of the swizzle for W columns of T: 2^B rows per 256 bytes
+template <typename T, int W>
+constexpr int swz_bits() {
+  constexpr int g = 256 / (W * (int)sizeof(T));
+  return g >= 8 ? 3 : (g >= 4 ? 2 : (g >= 2 ? 1 : 0));
+}
+
 // Butterfly b of one DIT pass (radix R, sub-length S, twiddles at TWO):
 // loads its R elements of column b % NCOL, applies the inter-pass twiddles
 // and the radix-R DFT.  Returns the column and the first logical position;

@@ -747,13 +747,17 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
 // full-length column.  The thread with first-pass butterfly n0 owns the
 // physical samples n0 + (NZF / R0) q, q < R0, of every field (the outputs it
 // reads after an inverse transform are the inputs of its first forward
-// pass).  Six inverse pairs, (u_r, u_t), (u_z, d_z u_r), (d_r u_r,
+// pass).  A workgroup takes ROWS (1 or 2) rows, one after the other.  Per
+// row: six inverse pairs, (u_r, u_t), (u_z, d_z u_r), (d_r u_r,
 // d_theta u_r / r), (d_r u_t, d_theta u_t / r), (d_z u_t, d_z u_z), (d_r u_z,
-// d_theta u_z / r), and two forward pairs, (N_r, N_t) and (N_z, 0).  The
-// plan's pass twiddles follow the half-length plan's in tw (NZF/2 - 1
-// entries).
-template <typename T, int NTHR, int PAD, int PFE, typename PF, int BAT>
-__global__ __launch_bounds__(NTHR) void zpass4_ct(Geom g, const cpx<T>* __restrict__ tw,
+// d_theta u_z / r), and the forward pair (N_r, N_t); N_z goes through a
+// forward transform of its own (ROWS 1: 8 complex transforms per row) or
+// together with the other row's N_z (ROWS 2: 7.5 per row).  The plan's pass
+// twiddles follow the half-length plan's in tw (NZF/2 - 1 entries).  The FP32
+// form without prefetch is compiled for 4 waves per SIMD (two 512-thread
+// workgroups per CU).
+template <typename T, int NTHR, int PAD, int PFE, int ROWS, typename PF, int BAT>
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 && !PFE && PF::rad[0] <= 9 ? 4 : 1))) void zpass4_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                   const cpx<T>* __restrict__ twz, const double* __restrict__ r,
                                                   const cpx<T>* __restrict__ W, size_t ws,
                                                   cpx<T>* __restrict__ Y, const NlMember* tab) {
@@ -775,25 +779,34 @@ __global__ __launch_bounds__(NTHR) void zpass4_ct(Geom g, const cpx<T>* __restri
   using LY = Lay<NF / PF::rad[PF::P - 1], PAD>;
   __shared__ cpx<T> buf[LY::size(NF)];
   const cpx<T>* __restrict__ twf = tw + (NH - 1);
-  const int row = blockIdx.x;
-  if (row >= g.nr * g.ntp) return;  // beyond this member's extent
-  const int i = row / g.ntp;
+  const int nrows = g.nr * g.ntp;
+  const int row0 = ROWS * (int)blockIdx.x;
+  if (row0 >= nrows) return;  // beyond this member's extent
+  const int nrow = ROWS == 2 && row0 + 1 < nrows ? 2 : 1;
   const int tid = threadIdx.x;
   const bool own = NB1 == NTHR || tid < NB1;
   const int nz = g.nz;
-  const T rad = T(r[i]);
-  const T irad = T(1) / rad;
-  auto over_r = [&](T x) -> T {
-    if constexpr (sizeof(T) == 4) return x * irad;
-    else return x / rad;
-  };
-  const int th = row - i * g.ntp;
-  const size_t roff = (size_t)i * tp_plane(g) + (size_t)th * 64;
-  const int rbytes = (int)((tp_plane(g) - (size_t)th * 64) * sizeof(cpx<T>));
   const T kz0 = T(g.kz0);
   const cpx<T> zero{T(0), T(0)};
   constexpr int CS = (int)sizeof(cpx<T>);
   constexpr int OOB = 0x7ffffff0;
+  // the current row: level i, radius, row th of the level's tile-major planes
+  T rad = T(1), irad = T(1);
+  size_t roff = 0;
+  int rbytes = 0;
+  auto set_row = [&](int row) {
+    const int i = row / g.ntp;
+    rad = T(r[i]);
+    irad = T(1) / rad;
+    const int th = row - i * g.ntp;
+    roff = (size_t)i * tp_plane(g) + (size_t)th * 64;
+    rbytes = (int)((tp_plane(g) - (size_t)th * 64) * sizeof(cpx<T>));
+  };
+  // x / r: the FP32 form multiplies by the reciprocal, the FP64 form divides
+  auto over_r = [&](T x) -> T {
+    if constexpr (sizeof(T) == 4) return x * irad;
+    else return x / rad;
+  };
   // sample n = n0 + NB1 q reads mode m = n (n <= NZF/2) or NZF - n (conjugated)
   auto conj_of = [](int lt, int q) -> bool {
     if (NB1 * (q + 1) <= NH + 1) return false;  // every lane: n <= NH
@@ -865,8 +878,9 @@ __global__ __launch_bounds__(NTHR) void zpass4_ct(Geom g, const cpx<T>* __restri
     __syncthreads();
     if (!PFE && fa >= 0) load(fa, fb, A, B);
   };
-  // forward pair of the products pa (+ i pb), separated into Y components c0 (and c0 + 1)
-  auto forward = [&](const T* pa, const T* pb, int c0) {
+  // forward pair of the products pa (+ i pb), separated into the Y rows da
+  // (and db, if pb is given)
+  auto forward = [&](const T* pa, const T* pb, cpx<T>* da, cpx<T>* db) {
     const int lt = opaque(tid);
     if (own) {
       cpx<T> v[R0];
@@ -876,8 +890,6 @@ __global__ __launch_bounds__(NTHR) void zpass4_ct(Geom g, const cpx<T>* __restri
     }
     __syncthreads();
     fft_rest<T, NTHR, -1, LY>(PF{}, buf, twf, lt);
-    cpx<T>* da = Y + c0 * ws + roff;
-    cpx<T>* db = Y + (c0 + 1) * ws + roff;
     const int t64 = g.ntp * 64;
     for (int k = lt; k < nz; k += NTHR) {
       cpx<T> xa = zero, xb = zero;
@@ -894,48 +906,62 @@ __global__ __launch_bounds__(NTHR) void zpass4_ct(Geom g, const cpx<T>* __restri
   };
   // field index in W: u_r, u_t, u_z (0-2), d_r u_c (3 + c), d_theta u_c / r (6 + c)
   T ur[R0], ut[R0], uz[R0], hz[R0], n0[R0], n1[R0];
-  load(0, 1, A, B);
-  inverse(false, false, 2, 0);
+  T nza[R0];             // N_z of the first row, transformed with the second row's
+  cpx<T>* ya = nullptr;  // the first row's N_z destination
+#pragma unroll 1
+  for (int rr = 0; rr < nrow; rr++) {
+    set_row(row0 + rr);
+    load(0, 1, A, B);
+    inverse(false, false, 2, 0);
 #pragma unroll
-  for (int q = 0; q < R0; q++) { ur[q] = s[q].x; ut[q] = s[q].y; }
-  inverse(false, true, 3, 6);
+    for (int q = 0; q < R0; q++) { ur[q] = s[q].x; ut[q] = s[q].y; }
+    inverse(false, true, 3, 6);
 #pragma unroll
-  for (int q = 0; q < R0; q++) { uz[q] = s[q].x; hz[q] = s[q].y; }
-  // calcNonlinear_k (mod_io_nonlinear.cu:142-170), summed in its order
-  inverse(false, false, 4, 7);
+    for (int q = 0; q < R0; q++) { uz[q] = s[q].x; hz[q] = s[q].y; }
+    // calcNonlinear_k (mod_io_nonlinear.cu:142-170), summed in its order
+    inverse(false, false, 4, 7);
 #pragma unroll
-  for (int q = 0; q < R0; q++) {
-    T v = ur[q] * s[q].x;
-    v += ut[q] * s[q].y;
-    v = v + uz[q] * hz[q];
-    v = v - ut[q] * over_r(ut[q]);
-    n0[q] = -v;
+    for (int q = 0; q < R0; q++) {
+      T v = ur[q] * s[q].x;
+      v += ut[q] * s[q].y;
+      v = v + uz[q] * hz[q];
+      v = v - ut[q] * over_r(ut[q]);
+      n0[q] = -v;
+    }
+    inverse(false, false, 1, 2);
+#pragma unroll
+    for (int q = 0; q < R0; q++) {
+      T v = ur[q] * s[q].x;
+      v += ut[q] * s[q].y;
+      n1[q] = v;
+    }
+    inverse(true, true, 5, 8);
+#pragma unroll
+    for (int q = 0; q < R0; q++) {
+      T v = n1[q] + uz[q] * s[q].x;
+      v = v + ur[q] * over_r(ut[q]);
+      n1[q] = -v;
+      hz[q] = s[q].y;
+    }
+    forward(n0, n1, Y + roff, Y + ws + roff);
+    inverse(false, false, -1, -1);
+#pragma unroll
+    for (int q = 0; q < R0; q++) {
+      T v = ur[q] * s[q].x;
+      v += ut[q] * s[q].y;
+      v = v + uz[q] * hz[q];
+      n0[q] = -v;
+    }
+    if (rr == 0 && nrow == 2) {  // held for the pair with the second row
+#pragma unroll
+      for (int q = 0; q < R0; q++) nza[q] = n0[q];
+      ya = Y + 2 * ws + roff;
+    } else if (rr == 1) {
+      forward(nza, n0, ya, Y + 2 * ws + roff);
+    } else {
+      forward(n0, nullptr, Y + 2 * ws + roff, nullptr);
+    }
   }
-  inverse(false, false, 1, 2);
-#pragma unroll
-  for (int q = 0; q < R0; q++) {
-    T v = ur[q] * s[q].x;
-    v += ut[q] * s[q].y;
-    n1[q] = v;
-  }
-  inverse(true, true, 5, 8);
-#pragma unroll
-  for (int q = 0; q < R0; q++) {
-    T v = n1[q] + uz[q] * s[q].x;
-    v = v + ur[q] * over_r(ut[q]);
-    n1[q] = -v;
-    hz[q] = s[q].y;
-  }
-  forward(n0, n1, 0);
-  inverse(false, false, -1, -1);
-#pragma unroll
-  for (int q = 0; q < R0; q++) {
-    T v = ur[q] * s[q].x;
-    v += ut[q] * s[q].y;
-    v = v + uz[q] * hz[q];
-    n0[q] = -v;
-  }
-  forward(n0, nullptr, 2);
 }
 
 // ------------------------------------------------------------------ theta forward
@@ -1267,6 +1293,9 @@ bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const do
   if (v.alg != 0) return false;
   using R4 = ThetaRowPad<PL, 4>;
   using R8 = ThetaRowPad<PL, 8>;
+  using S4 = Swz<4, swz_bits<cpx<T>, 4>()>;  // pad 3: swizzled rows
+  using S8 = Swz<8, swz_bits<cpx<T>, 8>()>;
+  using S16 = Swz<16, swz_bits<cpx<T>, 16>()>;
   if (tl == 4 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_inv_ct<T, 4, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 4 && v.pad == 1) { if constexpr (lds_fits<T, Lay<16, 1>>(PL::L * 4)) theta_inv_ct<T, 4, 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 4 && v.pad == 2) { if constexpr (lds_fits<T, R4>(PL::L * 4)) theta_inv_ct<T, 4, 256, R4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
@@ -1274,6 +1303,9 @@ bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const do
   else if (tl == 8 && v.pad == 1) { if constexpr (lds_fits<T, Lay<8, 1>>(PL::L * 8)) theta_inv_ct<T, 8, 256, Lay<8, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 8 && v.pad == 2) { if constexpr (lds_fits<T, R8>(PL::L * 8)) theta_inv_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 16 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_inv_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 4 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_inv_ct<T, 4, 256, S4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 8 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_inv_ct<T, 8, 256, S8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 16 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_inv_ct<T, 16, 256, S16, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else return false;
   return true;
 }
@@ -1296,6 +1328,9 @@ bool theta_fwd_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const cp
   if (v.alg != 0) return false;
   using R4 = ThetaRowPad<PL, 4>;
   using R8 = ThetaRowPad<PL, 8>;
+  using S4 = Swz<4, swz_bits<cpx<T>, 4>()>;  // pad 3: swizzled rows
+  using S8 = Swz<8, swz_bits<cpx<T>, 8>()>;
+  using S16 = Swz<16, swz_bits<cpx<T>, 16>()>;
   if (tl == 4 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_fwd_ct<T, 4, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else if (tl == 4 && v.pad == 1) { if constexpr (lds_fits<T, Lay<16, 1>>(PL::L * 4)) theta_fwd_ct<T, 4, 256, Lay<16, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else if (tl == 4 && v.pad == 2) { if constexpr (lds_fits<T, R4>(PL::L * 4)) theta_fwd_ct<T, 4, 256, R4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
@@ -1303,6 +1338,9 @@ bool theta_fwd_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const cp
   else if (tl == 8 && v.pad == 1) { if constexpr (lds_fits<T, Lay<8, 1>>(PL::L * 8)) theta_fwd_ct<T, 8, 256, Lay<8, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else if (tl == 8 && v.pad == 2) { if constexpr (lds_fits<T, R8>(PL::L * 8)) theta_fwd_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else if (tl == 16 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_fwd_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 4 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_fwd_ct<T, 4, 256, S4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 8 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_fwd_ct<T, 8, 256, S8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 16 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_fwd_ct<T, 16, 256, S16, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else return false;
   return true;
 }
@@ -1352,25 +1390,33 @@ bool zpass_plan(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T
       return false;
     }
   }
-  if (v.z_alg == 3) {  // pairs of real transforms as full-length complex ones; z_pad: row padding 0, 1, R0
+  if (v.z_alg == 3 || v.z_alg == 4) {  // pairs of real transforms as full-length complex ones (4: two rows
+                                       // per workgroup); z_pad: row padding 0, 1, R0
     using PP = typename ZPair<LL>::P;
     if constexpr (!std::is_void_v<PP>) {
       constexpr int NT4 = PP::L / PP::rad[0];
       constexpr int RL4 = PP::L / PP::rad[PP::P - 1];
       if constexpr (NT4 % 64 == 0 && NT4 <= 1024 && lds_fits<T, Lay<RL4, PP::rad[0]>>(PP::L)) {
-        auto go = [&](auto pf) {
+        auto go = [&](auto pf, auto rows) {
           constexpr int PFE = decltype(pf)::value;
+          constexpr int RW = decltype(rows)::value;
+          const int nrw = bt ? bt->max_nr * g.ntp : g.nr * g.ntp;
+          const dim3 nbr = bt ? dim3((nrw + RW - 1) / RW, bt->count) : dim3((nrw + RW - 1) / RW);
           if (v.z_pad == 0)
-            zpass4_ct<T, NT4, 0, PFE, PP, BAT><<<nb, NT4, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+            zpass4_ct<T, NT4, 0, PFE, RW, PP, BAT><<<nbr, NT4, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
           else if (v.z_pad == 1)
-            zpass4_ct<T, NT4, 1, PFE, PP, BAT><<<nb, NT4, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+            zpass4_ct<T, NT4, 1, PFE, RW, PP, BAT><<<nbr, NT4, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
           else if (v.z_pad == 2)
-            zpass4_ct<T, NT4, PP::rad[0], PFE, PP, BAT><<<nb, NT4, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
+            zpass4_ct<T, NT4, PP::rad[0], PFE, RW, PP, BAT><<<nbr, NT4, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);
           else
             return false;
           return true;
         };
-        return v.z_pf ? go(std::integral_constant<int, 1>{}) : go(std::integral_constant<int, 0>{});
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        if (v.z_alg == 3) return v.z_pf ? go(I1{}, I1{}) : go(I0{}, I1{});
+        return v.z_pf ? go(I1{}, I2{}) : go(I0{}, I2{});
       }
     }
     return false;

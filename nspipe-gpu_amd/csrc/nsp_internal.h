@@ -233,6 +233,9 @@ std::vector<double> centred_table(int nr, const std::vector<double>& T);
 // (16, 256)); ord = block
 // order (0: radial level fastest, 1: column tile fastest).
 struct ThetaVar {
+  // alg 0 LDS-scatter form, 1 register form; tl columns per block; nthr threads;
+  // pad (scatter form): 0 dense, 1 one slot per 16 (tl 4) or 8 (tl 8) elements,
+  // 2 row padding, 3 swizzled rows; ord block order
   int alg, tl, nthr, pad, ord;
 };
 // Kernel variant of the compiled-plan path (chosen by the autotuner).
@@ -240,7 +243,7 @@ struct NlVariant {
   ThetaVar inv, fwd;
   int z_alg;   // z pass: 0 LDS-scatter form, 1 register form (256 threads), 2 register form with the
                // inverse transforms in triples, 3 pair form (two real transforms as one full-length
-               // complex transform)
+               // complex transform), 4 pair form with two rows per workgroup
   int z_nthr;  // scatter form: threads per row (256 or 512)
   int z_pad;   // scatter form: 1 = one LDS slot per 16 elements, 2 = row padding as the register form;
                // triples: row padding of the three-column layout 0, 1 or R0 slots; pairs: row

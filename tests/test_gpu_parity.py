@@ -181,10 +181,11 @@ def _variants():
     (the others at their defaults)."""
     th = []
     for ord_ in (0, 1):
-        for pad in (0, 1, 2):
+        for pad in (0, 1, 2, 3):
             for tl in (4, 8):
                 th.append(dict(alg=0, tl=tl, nthr=256, pad=pad, ord=ord_))
         th.append(dict(alg=0, tl=16, nthr=256, pad=0, ord=ord_))
+        th.append(dict(alg=0, tl=16, nthr=256, pad=3, ord=ord_))
         for tl, nthr in ((4, 192), (8, 256), (8, 384), (16, 256)):
             th.append(dict(alg=1, tl=tl, nthr=nthr, pad=0, ord=ord_))
     base = dict(theta_inv=dict(alg=0, tl=8, nthr=256, pad=0, ord=1),
@@ -201,9 +202,10 @@ def _variants():
         out.append(dict(base, z_alg=1, z_pf=pf))
     for pad in (0, 1, 2):
         out.append(dict(base, z_alg=2, z_pad=pad))
-    for pad in (0, 1, 2):
-        for pf in (0, 1):
-            out.append(dict(base, z_alg=3, z_pad=pad, z_pf=pf))
+    for alg in (3, 4):
+        for pad in (0, 1, 2):
+            for pf in (0, 1):
+                out.append(dict(base, z_alg=alg, z_pad=pad, z_pf=pf))
     return out
 
 

@@ -621,6 +621,8 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
       for (int pad = 0; pad < 4; pad++)
         for (int tl : {4, 8}) cands.push_back(ThetaVar{0, tl, 256, pad, ord});
       cands.push_back(ThetaVar{0, 16, 256, 0, ord});
+      cands.push_back(ThetaVar{0, 16, 512, 0, ord});
+      cands.push_back(ThetaVar{0, 16, 1024, 0, ord});
       cands.push_back(ThetaVar{0, 16, 256, 3, ord});
       for (const auto& tr : treg) cands.push_back(ThetaVar{1, tr[0], tr[1], 0, ord});
     }

@@ -1302,6 +1302,8 @@ bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const do
   else if (tl == 8 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_inv_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 8 && v.pad == 1) { if constexpr (lds_fits<T, Lay<8, 1>>(PL::L * 8)) theta_inv_ct<T, 8, 256, Lay<8, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 8 && v.pad == 2) { if constexpr (lds_fits<T, R8>(PL::L * 8)) theta_inv_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 16 && v.pad == 0 && v.nthr == 512) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_inv_ct<T, 16, 512, Dense, PL, BAT><<<nb, 512, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 16 && v.pad == 0 && v.nthr == 1024) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_inv_ct<T, 16, 1024, Dense, PL, BAT><<<nb, 1024, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 16 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_inv_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 4 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_inv_ct<T, 4, 256, S4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 8 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_inv_ct<T, 8, 256, S8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
@@ -1337,6 +1339,8 @@ bool theta_fwd_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const cp
   else if (tl == 8 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_fwd_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else if (tl == 8 && v.pad == 1) { if constexpr (lds_fits<T, Lay<8, 1>>(PL::L * 8)) theta_fwd_ct<T, 8, 256, Lay<8, 1>, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else if (tl == 8 && v.pad == 2) { if constexpr (lds_fits<T, R8>(PL::L * 8)) theta_fwd_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 16 && v.pad == 0 && v.nthr == 512) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_fwd_ct<T, 16, 512, Dense, PL, BAT><<<nb, 512, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
+  else if (tl == 16 && v.pad == 0 && v.nthr == 1024) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_fwd_ct<T, 16, 1024, Dense, PL, BAT><<<nb, 1024, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else if (tl == 16 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_fwd_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else if (tl == 4 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_fwd_ct<T, 4, 256, S4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
   else if (tl == 8 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_fwd_ct<T, 8, 256, S8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }

@@ -185,6 +185,8 @@ def _variants():
             for tl in (4, 8):
                 th.append(dict(alg=0, tl=tl, nthr=256, pad=pad, ord=ord_))
         th.append(dict(alg=0, tl=16, nthr=256, pad=0, ord=ord_))
+        th.append(dict(alg=0, tl=16, nthr=512, pad=0, ord=ord_))
+        th.append(dict(alg=0, tl=16, nthr=1024, pad=0, ord=ord_))
         th.append(dict(alg=0, tl=16, nthr=256, pad=3, ord=ord_))
         for tl, nthr in ((4, 192), (8, 256), (8, 384), (16, 256)):
             th.append(dict(alg=1, tl=tl, nthr=nthr, pad=0, ord=ord_))

@@ -108,6 +108,12 @@ void free_all(nsp_handle* h) {
   h->norm_h = nullptr;
   if (h->stream) (void)hipStreamDestroy(h->stream);
   h->stream = nullptr;
+  if (h->nb.s2) (void)hipStreamDestroy(h->nb.s2);
+  h->nb.s2 = nullptr;
+  for (hipEvent_t& e : h->nb.ev) {
+    if (e) (void)hipEventDestroy(e);
+    e = nullptr;
+  }
 }
 
 nsp_status check_params(const nsp_params* p) {
@@ -229,9 +235,10 @@ std::string variant_json(const NlVariant& v) {
                   t.nthr, t.pad, t.ord);
     return std::string(o);
   };
-  char z[144];
-  std::snprintf(z, sizeof z, ", \"z_alg\": %d, \"z_nthr\": %d, \"z_pad\": %d, \"z_pf\": %d, \"s_ord\": %d}", v.z_alg,
-                v.z_nthr, v.z_pad, v.z_pf, v.s_ord);
+  char z[176];
+  std::snprintf(z, sizeof z,
+                ", \"z_alg\": %d, \"z_nthr\": %d, \"z_pad\": %d, \"z_pf\": %d, \"s_ord\": %d, \"nl_chunks\": %d}",
+                v.z_alg, v.z_nthr, v.z_pad, v.z_pf, v.s_ord, v.nl_chunks);
   return "{\"theta_inv\": " + tvj(v.inv) + ", \"theta_fwd\": " + tvj(v.fwd) + z;
 }
 
@@ -310,6 +317,8 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   g.d_im = p->d_im;
   g.qdes = p->qdes;
   g.skip = nullptr;
+  g.lv0 = 0;
+  g.nlv = g.nr;
 
   auto bail = [&](nsp_status s) {
     free_all(h);
@@ -328,6 +337,8 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   } while (0)
 
   HTRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  HTRY(hipStreamCreateWithFlags(&h->nb.s2, hipStreamNonBlocking));  // default priority, as h->stream
+  for (hipEvent_t& e : h->nb.ev) HTRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 
   // radial tables
   build_radial_tables(g.nr, h->host);
@@ -358,7 +369,7 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   h->nb.wstride = size_t(g.nr) * tp_plane(g);  // tile-major planes (tp_plane)
   h->nb.use_ct = false;
   h->nb.mixed = p->precision != NSP_FP64;
-  h->nb.var = NlVariant{ThetaVar{0, 8, 256, 0, 1}, ThetaVar{0, 8, 256, 0, 1}, 1, 256, 0, 0, 1};
+  h->nb.var = NlVariant{ThetaVar{0, 8, 256, 0, 1}, ThetaVar{0, 8, 256, 0, 1}, 1, 256, 0, 0, 1, 1};
   if (!ct_has_zreg(g.nh)) {  // long axial rows: the 512-thread LDS form
     h->nb.var.z_alg = 0;
     h->nb.var.z_nthr = 512;
@@ -583,6 +594,13 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
                                             nb.wstride, h->t.D, h->u, h->Nl, s)
                       : ct_theta_fwd<double>(v, g, (const cpx<double>*)nb.ct_tw_t, (const cpx<double>*)nb.Y,
                                              nb.wstride, h->t.D, h->u, h->Nl, s);
+    if (which == 4) {  // the whole nonlinear stage with v (chunked pipeline or not)
+      const NlVariant keep = nb.var;
+      nb.var = v;
+      launch_nonlinear(g, h->t, nb, h->u, h->Nl, s, nullptr);
+      nb.var = keep;
+      return true;
+    }
     // a predictor sweep of the current state into the free iterate buffer
     // (E and the iterate are recomputed by the next step)
     Lin2Args a = lin_args(h, g, h->Nl, h->u, h->ui, true, h->u);
@@ -678,6 +696,18 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
         const double t = time_it(1, v, lab);
         if (t >= 0 && t < bt) { bt = t; best.z_alg = alg; best.z_pad = pad; best.z_pf = pf; }
       }
+  // the nonlinear stage in level chunks on two streams (z pass of one chunk
+  // beside the theta inverse of the next) or stage after stage
+  bt = 1e30;
+  for (int kc : {1, 2, 3, 4, 6, 8}) {
+    if (kc > g.nr) continue;
+    NlVariant v = best;
+    v.nl_chunks = kc;
+    char lab[64];
+    std::snprintf(lab, sizeof lab, "nonlinear chunks=%d", kc);
+    const double t = time_it(4, v, lab);
+    if (t >= 0 && t < bt) { bt = t; best.nl_chunks = kc; }
+  }
   // linear sweep: order of its (m, -m, axial chunk) workgroups
   bt = 1e30;
   for (int so = 0; so < 2; so++) {
@@ -726,6 +756,8 @@ nsp_status nsp_set_tuning(nsp_handle* h, const char* json) {
   v.z_pf = int(zf);
   const long so = json_int(t, base, "s_ord");  // absent in older tuning text: keep the current order
   if (so >= 0) v.s_ord = so ? 1 : 0;
+  const long nc = json_int(t, base, "nl_chunks");  // absent: stage after stage
+  v.nl_chunks = nc >= 1 ? int(std::min<long>(nc, kNlMaxChunks)) : 1;
   // accept only variants that are compiled for this size: one nonlinear
   // evaluation of the current state into the scratch of the nonlinear term
   const NlVariant old = h->nb.var;

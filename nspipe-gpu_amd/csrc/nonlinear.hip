@@ -17,6 +17,8 @@
 // pair of mod_fft.cu:21-49 (theta = outer dimension, z = real dimension);
 // at l = 0 only the real part of the theta-transformed column enters the
 // real inverse transform.
+#include <algorithm>
+
 #include "fft_device.h"
 
 namespace nsp {
@@ -372,6 +374,41 @@ void launch_dr3(const Geom& g, const double* D, const double2* u, double2* dR, b
 
 void launch_nonlinear(const Geom& g, const DevTables& t, const NlBufs& nb, const double2* u, double2* N,
                       hipStream_t s, Profiler* pf) {
+  if (nb.use_ct && !pf && nb.var.nl_chunks > 1 && nb.s2) {
+    // Pipelined: the theta inverse of level chunk c + 1 (memory-bound) runs on
+    // s beside the z pass of chunk c (VALU-bound) on s2; each z chunk waits
+    // for its own theta chunk, the theta forward for the last z chunk.  The
+    // chunks' planes of W and Y are disjoint.  Under stream capture the
+    // events become graph edges (s2 joins the capture at its first wait).
+    const NlVariant& v = nb.var;
+    const int K = std::min(std::min(v.nl_chunks, kNlMaxChunks), g.nr);
+    launch_dr3(g, t.D, u, nb.dR, nb.mixed, s);
+    for (int c = 0; c < K; c++) {
+      Geom gc = g;
+      gc.lv0 = (int)((long long)c * g.nr / K);
+      gc.nlv = (int)((long long)(c + 1) * g.nr / K) - gc.lv0;
+      if (nb.mixed)
+        ct_theta_inv<float>(v, gc, (const cpx<float>*)nb.ct_tw_t32, t.r, u, nb.dR, (cpx<float>*)nb.W32, nb.wstride, s);
+      else
+        ct_theta_inv<double>(v, gc, (const cpx<double>*)nb.ct_tw_t, t.r, u, nb.dR, (cpx<double>*)nb.W, nb.wstride,
+                             s);
+      (void)hipEventRecord(nb.ev[c], s);
+      (void)hipStreamWaitEvent(nb.s2, nb.ev[c], 0);
+      if (nb.mixed)
+        ct_zpass<float>(v, gc, (const cpx<float>*)nb.ct_tw_h32, (const cpx<float>*)nb.tw_z32, t.r,
+                        (const cpx<float>*)nb.W32, nb.wstride, (cpx<float>*)nb.Y32, nb.s2);
+      else
+        ct_zpass<double>(v, gc, (const cpx<double>*)nb.ct_tw_h, (const cpx<double>*)nb.tw_z, t.r,
+                         (const cpx<double>*)nb.W, nb.wstride, (cpx<double>*)nb.Y, nb.s2);
+    }
+    (void)hipEventRecord(nb.ev[K], nb.s2);
+    (void)hipStreamWaitEvent(s, nb.ev[K], 0);
+    if (nb.mixed)
+      ct_theta_fwd<float>(v, g, (const cpx<float>*)nb.ct_tw_t32, (const cpx<float>*)nb.Y32, nb.wstride, t.D, u, N, s);
+    else
+      ct_theta_fwd<double>(v, g, (const cpx<double>*)nb.ct_tw_t, (const cpx<double>*)nb.Y, nb.wstride, t.D, u, N, s);
+    return;
+  }
   if (nb.use_ct) {
     const NlVariant& v = nb.var;
     bool ok = true;

@@ -49,17 +49,18 @@ __device__ __forceinline__ cpx<T> buf_load(nsp_i4v rs, int off) {
 }
 
 // ------------------------------------------------------------------ block order
-// (level i, first column l0) of a theta block.  ord 0: level fastest;
+// (level i, first column l0) of a theta block, levels counted from g.lv0.
+// ord 0: level fastest;
 // ord 1: column tile fastest, so the blocks an XCD runs together cover whole
 // rows of one level (contiguous HBM pages, write lines filled in one L2).
 __device__ __forceinline__ void theta_block(const Geom& g, int TL, int ord, int& i, int& l0) {
   const int lb = xcd_block(blockIdx.x, gridDim.x);
   if (ord == 0) {
-    i = lb % g.nr;
-    l0 = (lb / g.nr) * TL;
+    i = g.lv0 + lb % g.nlv;
+    l0 = (lb / g.nlv) * TL;
   } else {
     const int nt = (g.nz + TL - 1) / TL;
-    i = lb / nt;
+    i = g.lv0 + lb / nt;
     l0 = (lb % nt) * TL;
   }
 }
@@ -94,7 +95,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
   const int nz = g.nz;
   int i, l0;
   theta_block(g, TL, ord, i, l0);
-  if (i >= g.nr || l0 >= nz) return;  // beyond this member's extent
+  if (i >= g.lv0 + g.nlv || l0 >= nz) return;  // beyond this member's (or level chunk's) extent
   const int tid = threadIdx.x;
   const double ri = r[i];
   // m u / r: the FP32 form multiplies by the reciprocal (the result is rounded
@@ -212,8 +213,8 @@ __global__ __launch_bounds__(NTHR) void zpass_ct(Geom g, const cpx<T>* __restric
   constexpr int NH = PL::L;
   constexpr int NZF = 2 * NH;
   __shared__ cpx<T> buf[LY::size(NH)];
-  const int row = blockIdx.x;
-  if (row >= g.nr * g.ntp) return;  // beyond this member's extent
+  const int row = g.lv0 * g.ntp + (int)blockIdx.x;
+  if (row >= (g.lv0 + g.nlv) * g.ntp) return;  // beyond this member's (or level chunk's) extent
   const int i = row / g.ntp;
   const int tid = threadIdx.x;
   const int nz = g.nz;
@@ -355,8 +356,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   static_assert(NB1 <= NTHR, "one first-pass butterfly per thread");
   using LY = Lay<NH / PL::rad[PL::P - 1], PAD>;
   __shared__ cpx<T> buf[LY::size(NH)];
-  const int row = blockIdx.x;
-  if (row >= g.nr * g.ntp) return;  // beyond this member's extent
+  const int row = g.lv0 * g.ntp + (int)blockIdx.x;
+  if (row >= (g.lv0 + g.nlv) * g.ntp) return;  // beyond this member's (or level chunk's) extent
   const int i = row / g.ntp;
   const int tid = threadIdx.x;
   const bool own = NB1 == NTHR || tid < NB1;  // thread owns a first-pass butterfly
@@ -590,8 +591,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   using LY3 = Lay<3 * RL, PAD3>;  // three interleaved columns (inverse triples)
   constexpr int NBUF = LY3::size(3 * NH) > LY::size(NH) ? LY3::size(3 * NH) : LY::size(NH);
   __shared__ cpx<T> buf[NBUF];
-  const int row = blockIdx.x;
-  if (row >= g.nr * g.ntp) return;  // beyond this member's extent
+  const int row = g.lv0 * g.ntp + (int)blockIdx.x;
+  if (row >= (g.lv0 + g.nlv) * g.ntp) return;  // beyond this member's (or level chunk's) extent
   const int i = row / g.ntp;
   const int tid = threadIdx.x;
   const bool own = NB1 == NTHR || tid < NB1;
@@ -779,9 +780,9 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   using LY = Lay<NF / PF::rad[PF::P - 1], PAD>;
   __shared__ cpx<T> buf[LY::size(NF)];
   const cpx<T>* __restrict__ twf = tw + (NH - 1);
-  const int nrows = g.nr * g.ntp;
-  const int row0 = ROWS * (int)blockIdx.x;
-  if (row0 >= nrows) return;  // beyond this member's extent
+  const int nrows = (g.lv0 + g.nlv) * g.ntp;  // end of this member's (or level chunk's) rows
+  const int row0 = g.lv0 * g.ntp + ROWS * (int)blockIdx.x;
+  if (row0 >= nrows) return;
   const int nrow = ROWS == 2 && row0 + 1 < nrows ? 2 : 1;
   const int tid = threadIdx.x;
   const bool own = NB1 == NTHR || tid < NB1;
@@ -986,7 +987,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd_ct(Geom g, const cpx<T>* __res
   __shared__ cpx<T> buf[LY::size(NTP * TL)];
   int i, l0;
   theta_block(g, TL, ord, i, l0);
-  if (i >= g.nr || l0 >= g.nz) return;  // beyond this member's extent
+  if (i >= g.lv0 + g.nlv || l0 >= g.nz) return;  // beyond this member's (or level chunk's) extent
   const int tid = threadIdx.x;
   const int nt = g.nt, half = nt / 2, nz = g.nz;
   const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
@@ -1063,7 +1064,7 @@ __global__ __launch_bounds__(NTHR) void theta_inv2_ct(Geom g, const cpx<T>* __re
   const int nz = g.nz;
   int i, l0;
   theta_block(g, TL, ord, i, l0);
-  if (i >= g.nr || l0 >= g.nz) return;  // beyond this member's extent
+  if (i >= g.lv0 + g.nlv || l0 >= g.nz) return;  // beyond this member's (or level chunk's) extent
   const int tid = threadIdx.x;
   const double ri = r[i];
   // m u / r: the FP32 form multiplies by the reciprocal (the result is rounded
@@ -1149,7 +1150,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
   __shared__ cpx<T> buf[LY::size(NTP * TL)];
   int i, l0;
   theta_block(g, TL, ord, i, l0);
-  if (i >= g.nr || l0 >= g.nz) return;  // beyond this member's extent
+  if (i >= g.lv0 + g.nlv || l0 >= g.nz) return;  // beyond this member's (or level chunk's) extent
   const int tid = threadIdx.x;
   const int nz = g.nz;
   const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
@@ -1281,7 +1282,7 @@ bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const do
   const int tl = v.tl;
   if (PL::L * tl * (int)sizeof(cpx<T>) > 163840) return false;
   const NlMember* tab = bt ? bt->tab : nullptr;
-  const dim3 nb = bt ? dim3(((bt->max_nz + tl - 1) / tl) * bt->max_nr, bt->count) : dim3(((g.nz + tl - 1) / tl) * g.nr);
+  const dim3 nb = bt ? dim3(((bt->max_nz + tl - 1) / tl) * bt->max_nr, bt->count) : dim3(((g.nz + tl - 1) / tl) * g.nlv);
   if (v.alg == 1) {
     if (tl == 4 && v.nthr == 192) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_inv2_ct<T, 4, 192, Dense, PL, BAT><<<nb, 192, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
     else if (tl == 8 && v.nthr == 256) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_inv2_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
@@ -1318,7 +1319,7 @@ bool theta_fwd_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const cp
   const int tl = v.tl;
   if (PL::L * tl * (int)sizeof(cpx<T>) > 163840) return false;
   const NlMember* tab = bt ? bt->tab : nullptr;
-  const dim3 nb = bt ? dim3(((bt->max_nz + tl - 1) / tl) * bt->max_nr, bt->count) : dim3(((g.nz + tl - 1) / tl) * g.nr);
+  const dim3 nb = bt ? dim3(((bt->max_nz + tl - 1) / tl) * bt->max_nr, bt->count) : dim3(((g.nz + tl - 1) / tl) * g.nlv);
   if (v.alg == 1) {
     if (tl == 4 && v.nthr == 192) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_fwd2_ct<T, 4, 192, Dense, PL, BAT><<<nb, 192, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
     else if (tl == 8 && v.nthr == 256) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_fwd2_ct<T, 8, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, Y, ws, D, u, N, v.ord, tab); else return false; }
@@ -1361,7 +1362,7 @@ bool zpass_plan(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T
                 const cpx<T>* W, size_t ws, cpx<T>* Y, hipStream_t s, const Batch* bt) {
   constexpr int LL = PL::L;
   const NlMember* tab = bt ? bt->tab : nullptr;
-  const dim3 nb = bt ? dim3(bt->max_nr * g.ntp, bt->count) : dim3(g.nr * g.ntp);
+  const dim3 nb = bt ? dim3(bt->max_nr * g.ntp, bt->count) : dim3(g.nlv * g.ntp);
   // register forms: one first-pass butterfly per thread, 256 threads (512
   // when the first pass has 512 butterflies, as for 4608)
   constexpr int NB1 = LL / PL::rad[0];
@@ -1404,7 +1405,7 @@ bool zpass_plan(const NlVariant& v, const Geom& g, const cpx<T>* tw, const cpx<T
         auto go = [&](auto pf, auto rows) {
           constexpr int PFE = decltype(pf)::value;
           constexpr int RW = decltype(rows)::value;
-          const int nrw = bt ? bt->max_nr * g.ntp : g.nr * g.ntp;
+          const int nrw = bt ? bt->max_nr * g.ntp : g.nlv * g.ntp;
           const dim3 nbr = bt ? dim3((nrw + RW - 1) / RW, bt->count) : dim3((nrw + RW - 1) / RW);
           if (v.z_pad == 0)
             zpass4_ct<T, NT4, 0, PFE, RW, PP, BAT><<<nbr, NT4, 0, s>>>(g, tw, twz, r, W, ws, Y, tab);

@@ -50,6 +50,9 @@ struct Geom {
   // Corrector launches of a captured time step: nonzero *skip means the step
   // has already converged and the launch returns at once.  Null elsewhere.
   const int* skip;
+  // Levels a theta or z launch covers: [lv0, lv0 + nlv).  The whole column
+  // (0, NR) except for the level chunks of the pipelined nonlinear stage.
+  int lv0, nlv;
 };
 
 __device__ __forceinline__ bool step_done(const Geom& g) { return g.skip != nullptr && *g.skip != 0; }
@@ -239,6 +242,7 @@ struct ThetaVar {
   int alg, tl, nthr, pad, ord;
 };
 // Kernel variant of the compiled-plan path (chosen by the autotuner).
+constexpr int kNlMaxChunks = 8;
 struct NlVariant {
   ThetaVar inv, fwd;
   int z_alg;   // z pass: 0 LDS-scatter form, 1 register form (256 threads), 2 register form with the
@@ -250,6 +254,9 @@ struct NlVariant {
                // padding 0, 1 or R0 slots
   int z_pf;    // register and pair forms: load the next transform's inputs before the LDS passes
   int s_ord;   // linear sweep: 1 = the chunks of an azimuthal row XCD-contiguous, 0 = dispatch order
+  int nl_chunks;  // nonlinear stage: 1 = stage after stage; K > 1 = the theta inverse and the z pass
+                  // in K level chunks on two streams, the z pass of chunk c overlapping the theta
+                  // inverse of chunk c + 1
 };
 // Nonlinear-phase buffers and FFT data.
 struct NlBufs {
@@ -275,6 +282,9 @@ struct NlBufs {
   float2* ct_tw_h32;
   float2* tw_z32;
   NlVariant var;
+  // the second stream and the events of the pipelined nonlinear stage
+  hipStream_t s2;
+  hipEvent_t ev[kNlMaxChunks + 1];
 };
 
 // ----------------------------------------------------------------- profiler

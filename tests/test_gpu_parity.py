@@ -415,6 +415,57 @@ def test_graph_steps_match_eager_headline_multi_corrector(monkeypatch):
         assert np.array_equal(a[c], b[c])
 
 
+# a kernel-variant choice compiled for the test shapes below (pair z pass, two rows)
+PIPE_BASE = dict(theta_inv=dict(alg=0, tl=8, nthr=256, pad=0, ord=1),
+                 theta_fwd=dict(alg=1, tl=8, nthr=256, pad=0, ord=1),
+                 z_alg=4, z_nthr=512, z_pad=1, z_pf=0, s_ord=1)
+
+
+@pytest.mark.parametrize("prec", [nsp.FP64, nsp.MIXED], ids=["fp64", "mixed"])
+def test_pipelined_nonlinear_matches_stage_order(prec):
+    """The nonlinear stage in K level chunks on two streams (the theta inverse
+    of chunk c + 1 beside the z pass of chunk c) runs the same kernels on the
+    same data: N(u) equal bit for bit to the stage-after-stage order, for chunk
+    counts that divide NR, that do not, and one level per chunk."""
+    shape = dict(COMPILED[3], nr=8)  # plans 576 / 2304
+    print(f"[gpu] pipelined nonlinear precision {prec}", flush=True)
+    p = nsp.Pipe(**shape, precision=prec)
+    p.seeded_ic(SEED1, 6, 6, 1, 0.1)
+    u = p.get_state()
+    p.set_tuning(dict(PIPE_BASE, nl_chunks=1))
+    ref = p.nonlinear(*u)
+    for k in (2, 3, 8):
+        p.set_tuning(dict(PIPE_BASE, nl_chunks=k))
+        a = p.nonlinear(*u)
+        for c in range(3):
+            assert np.array_equal(a[c], ref[c]), (k, c)
+    p.close()
+
+
+def test_pipelined_nonlinear_graph_steps(monkeypatch):
+    """Captured steps with the pipelined nonlinear stage (its events become
+    graph edges between the two streams) give the bits, counts and flags of
+    stage-after-stage steps launched one by one, and of pipelined ones."""
+    print("[gpu] pipelined nonlinear in captured steps", flush=True)
+    p1 = nsp.Pipe(**CFG1)
+    p1.set_tuning(dict(PIPE_BASE, nl_chunks=3))
+    monkeypatch.setenv("NSP_NO_GRAPH", "1")
+    p2 = nsp.Pipe(**CFG1)
+    p2.set_tuning(dict(PIPE_BASE, nl_chunks=1))
+    p3 = nsp.Pipe(**CFG1)
+    p3.set_tuning(dict(PIPE_BASE, nl_chunks=3))
+    out = []
+    for p in (p1, p2, p3):
+        p.seeded_ic(SEED1, 4, 4, 0, 1e-3)
+        out.append(p.step(3) + (p.get_state(),))
+        p.close()
+    for q in (1, 2):
+        for j in range(3):
+            assert np.array_equal(out[0][j], out[q][j]), (q, j)
+        for c in range(3):
+            assert np.array_equal(out[0][3][c], out[q][3][c]), (q, c)
+
+
 def test_profile_counts_executed_launches():
     """Corrector iterations skipped inside a captured step are not counted."""
     p = nsp.Pipe(**CFG1)

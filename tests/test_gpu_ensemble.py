@@ -43,6 +43,40 @@ def test_ensemble_members_match_single_pipes():
     ens.close()
 
 
+PAIR_VARIANT = dict(theta_inv=dict(alg=0, tl=8, nthr=256, pad=0, ord=1),
+                    theta_fwd=dict(alg=1, tl=8, nthr=256, pad=0, ord=1),
+                    z_alg=4, z_nthr=512, z_pad=1, z_pf=0, s_ord=1)
+
+
+@pytest.mark.parametrize("z_alg", [3, 4])
+def test_ensemble_pair_form_matches_single_pipes(z_alg):
+    """The batched pair-form z pass (one or two rows per workgroup, rows of
+    uneven members beyond a member's extent returning at entry): every member
+    gives the bits and corrector counts of the same pipe stepped alone with the
+    same kernel variants."""
+    ms = members()
+    var = dict(PAIR_VARIANT, z_alg=z_alg)
+    print(f"[gpu] ensemble pair form z_alg {z_alg}", flush=True)
+    ens = nsp.Ensemble(ms)
+    for j in range(len(ms)):
+        ens[j].set_tuning(var)  # the class's first member's choice is the class's
+        seed_member(ens[j], j)
+    n_e, err_e, fl_e = ens.step(3)
+    assert np.all(fl_e == 0)
+    for j, kw in enumerate(ms):
+        p = nsp.Pipe(**kw)
+        p.set_tuning(var)
+        seed_member(p, j)
+        n_p, err_p, _ = p.step(3)
+        assert np.array_equal(n_p, n_e[:, j]), (j, n_p, n_e[:, j])
+        assert np.array_equal(err_p, err_e[:, j]), j
+        a, b = ens[j].get_state(), p.get_state()
+        for c in range(3):
+            assert np.array_equal(a[c], b[c]), (j, c)
+        p.close()
+    ens.close()
+
+
 def test_ensemble_members_converge_independently():
     """A member with a tight tolerance would need more correctors; with a
     shared tol each member still stops on its own test (counts may differ)."""

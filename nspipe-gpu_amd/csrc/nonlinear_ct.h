@@ -907,7 +907,12 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   };
   // field index in W: u_r, u_t, u_z (0-2), d_r u_c (3 + c), d_theta u_c / r (6 + c)
   T ur[R0], ut[R0], uz[R0], hz[R0], n0[R0], n1[R0];
-  T nza[R0];             // N_z of the first row, transformed with the second row's
+  // N_z of the first row, transformed with the second row's: held in LDS
+  // beside the transform buffer where both fit (each thread its own values,
+  // so no barrier), which keeps it out of the registers the second row needs
+  constexpr bool NZ_LDS = ROWS == 2 && LY::size(NF) * (int)sizeof(cpx<T>) + R0 * NTHR * (int)sizeof(T) <= 163840;
+  __shared__ T nza[NZ_LDS ? R0 * NTHR : 1];
+  T nzr[NZ_LDS ? 1 : R0];
   cpx<T>* ya = nullptr;  // the first row's N_z destination
 #pragma unroll 1
   for (int rr = 0; rr < nrow; rr++) {
@@ -955,10 +960,19 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
     }
     if (rr == 0 && nrow == 2) {  // held for the pair with the second row
 #pragma unroll
-      for (int q = 0; q < R0; q++) nza[q] = n0[q];
+      for (int q = 0; q < R0; q++) {
+        if constexpr (NZ_LDS) nza[q * NTHR + tid] = n0[q];
+        else nzr[q] = n0[q];
+      }
       ya = Y + 2 * ws + roff;
     } else if (rr == 1) {
-      forward(nza, n0, ya, Y + 2 * ws + roff);
+      T na[R0];
+#pragma unroll
+      for (int q = 0; q < R0; q++) {
+        if constexpr (NZ_LDS) na[q] = nza[q * NTHR + tid];
+        else na[q] = nzr[q];
+      }
+      forward(na, n0, ya, Y + 2 * ws + roff);
     } else {
       forward(n0, nullptr, Y + 2 * ws + roff, nullptr);
     }

@@ -779,6 +779,12 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   static_assert(NB1 <= NTHR, "one first-pass butterfly per thread");
   using LY = Lay<NF / PF::rad[PF::P - 1], PAD>;
   __shared__ cpx<T> buf[LY::size(NF)];
+  // LDS beside the transform buffer, where a workgroup's total stays within
+  // a CU's 160 KB: the first row's N_z (two-row form) and the load offsets
+  constexpr int BUFB = LY::size(NF) * (int)sizeof(cpx<T>);
+  constexpr bool NZ_LDS = ROWS == 2 && BUFB + R0 * NTHR * (int)sizeof(T) <= 163840;
+  constexpr bool OFF_LDS = BUFB + (NZ_LDS ? R0 * NTHR * (int)sizeof(T) : 0) + R0 * NTHR * 4 <= 163840;
+  __shared__ int offs[OFF_LDS ? R0 * NTHR : 1];
   const cpx<T>* __restrict__ twf = tw + (NH - 1);
   const int nrows = (g.lv0 + g.nlv) * g.ntp;  // end of this member's (or level chunk's) rows
   const int row0 = g.lv0 * g.ntp + ROWS * (int)blockIdx.x;
@@ -817,26 +823,36 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   // the stored modes m of fields fa, fb at the owned samples, zero past NZ
   // (range-checked loads).  With NB1 and NZF multiples of 64, the tile-major
   // offsets of m = n and m = NZF - n are affine in q.
+  auto offset = [&](int lt, int q) -> int {
+    const int n = lt + NB1 * q;
+    const int t64 = g.ntp * 64;
+    if constexpr (NB1 % 64 == 0 && NF % 64 == 0) {
+      const int dq = (NB1 / 64) * t64 * CS;
+      const int ok = ((lt >> 6) * t64 + (lt & 63)) * CS + q * dq;
+      const int oc = (((NF >> 6) - ((lt + 63) >> 6)) * t64 + ((-lt) & 63)) * CS - q * dq;
+      const bool cj = conj_of(lt, q);
+      return cj ? (NF - n < nz ? oc : OOB) : (n < nz ? ok : OOB);
+    } else {
+      const int m = conj_of(lt, q) ? NF - n : n;
+      return m < nz ? (int)tp_col(g, m) * CS : OOB;
+    }
+  };
+  // The offsets are the same for every pair and row of the workgroup: where
+  // LDS has room they are computed once and read back per load.
+  if constexpr (OFF_LDS) {
+    if (own) {
+#pragma unroll
+      for (int q = 0; q < R0; q++) offs[q * NTHR + tid] = offset(tid, q);
+    }
+  }
   auto load = [&](int fa, int fb, cpx<T>* A, cpx<T>* B) {
     const int lt = opaque(tid);
     if (own) {
       const nsp_i4v ra = buf_rsrc(W + fa * ws + roff, rbytes);
       const nsp_i4v rb = buf_rsrc(W + fb * ws + roff, rbytes);
-      const int t64 = g.ntp * 64;
 #pragma unroll
       for (int q = 0; q < R0; q++) {
-        const int n = lt + NB1 * q;
-        int o;
-        if constexpr (NB1 % 64 == 0 && NF % 64 == 0) {
-          const int dq = (NB1 / 64) * t64 * CS;
-          const int ok = ((lt >> 6) * t64 + (lt & 63)) * CS + q * dq;
-          const int oc = (((NF >> 6) - ((lt + 63) >> 6)) * t64 + ((-lt) & 63)) * CS - q * dq;
-          const bool cj = conj_of(lt, q);
-          o = cj ? (NF - n < nz ? oc : OOB) : (n < nz ? ok : OOB);
-        } else {
-          const int m = conj_of(lt, q) ? NF - n : n;
-          o = m < nz ? (int)tp_col(g, m) * CS : OOB;
-        }
+        const int o = OFF_LDS ? offs[q * NTHR + lt] : offset(lt, q);
         A[q] = buf_load<T>(ra, o);
         B[q] = buf_load<T>(rb, o);
       }
@@ -910,7 +926,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   // N_z of the first row, transformed with the second row's: held in LDS
   // beside the transform buffer where both fit (each thread its own values,
   // so no barrier), which keeps it out of the registers the second row needs
-  constexpr bool NZ_LDS = ROWS == 2 && LY::size(NF) * (int)sizeof(cpx<T>) + R0 * NTHR * (int)sizeof(T) <= 163840;
   __shared__ T nza[NZ_LDS ? R0 * NTHR : 1];
   T nzr[NZ_LDS ? 1 : R0];
   cpx<T>* ya = nullptr;  // the first row's N_z destination

@@ -191,15 +191,17 @@ struct Plan {
   static constexpr int rad[P] = {R...};
   static constexpr int L = (1 * ... * R);
   // LDS position of input sample n (mixed-radix digit reversal)
-  __device__ __forceinline__ static int digitrev(int n) {
-    int pos = 0, len = L;
+  // (unsigned arithmetic: n >= 0, and the divisions by constants need no
+  // sign corrections)
+  __device__ __forceinline__ static int digitrev(int n_) {
+    unsigned n = (unsigned)n_, pos = 0, len = L;
 #pragma unroll
     for (int q = P - 1; q >= 0; --q) {
       len /= rad[q];
       pos += (n % rad[q]) * len;
       n /= rad[q];
     }
-    return pos;
+    return (int)pos;
   }
 };
 
@@ -235,6 +237,24 @@ struct Lay {
   }
   template <int SE, int SPAN>
   static constexpr bool is_affine() { return PAD == 0 || W % SPAN == 0 || SE % W == 0; }
+  // affine() for first = j SPAN + c with 0 <= c < SE: the row of first follows
+  // from the group index j alone where a group lies in one row (j / (W /
+  // SPAN)) or its SE-spaced elements start in row j (SPAN / W) (SE == W).
+  template <int SE, int SPAN>
+  __device__ __forceinline__ static void affine_g(int first, int j, int& p0, int& step) {
+    if constexpr (PAD == 0) {
+      p0 = first;
+      step = SE;
+    } else if constexpr (W % SPAN == 0) {
+      p0 = first + (int)((unsigned)j / (unsigned)(W / SPAN)) * PAD;
+      step = SE;
+    } else if constexpr (SE == W) {
+      p0 = first + j * (SPAN / W) * PAD;
+      step = SE + PAD;
+    } else {
+      affine<SE, SPAN>(first, p0, step);
+    }
+  }
 };
 using Dense = Lay<1, 0>;
 
@@ -264,6 +284,11 @@ struct Swz {
   }
   template <int SE, int SPAN>
   static constexpr bool is_affine() { return B == 0; }
+  template <int SE, int SPAN>
+  __device__ __forceinline__ static void affine_g(int first, int, int& p0, int& step) {
+    p0 = first;
+    step = SE;
+  }
 };
 // bits B of the swizzle for W columns of T: 2^B rows per 256 bytes
 template <typename T, int W>
@@ -272,30 +297,51 @@ constexpr int swz_bits() {
   return g >= 8 ? 3 : (g >= 4 ? 2 : (g >= 2 ? 1 : 0));
 }
 
-// Butterfly b of one DIT pass (radix R, sub-length S, twiddles at TWO):
-// loads its R elements of column b % NCOL, applies the inter-pass twiddles
-// and the radix-R DFT.  Returns the column and the first logical position;
-// element q belongs at position base + q*S.
-template <typename T, int L, int NCOL, int SIGN, typename LY, int S, int TWO, int R>
-__device__ __forceinline__ void dit_butterfly(const cpx<T>* buf, const cpx<T>* __restrict__ tw, int b, cpx<T>* v,
-                                              int& col, int& base) {
-  col = b % NCOL;
-  const int bb = b / NCOL;
-  const int k = bb % S;
-  base = (bb - k) * R + k;
-  if constexpr (LY::template is_affine<S * NCOL, S * R * NCOL>()) {
-    int p0, st;
-    LY::template affine<S * NCOL, S * R * NCOL>(col + base * NCOL, p0, st);
-#pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = buf[p0 + q * st];
-  } else {
-#pragma unroll
-    for (int q = 0; q < R; ++q) v[q] = buf[LY::pos(col + (base + q * S) * NCOL)];
+// Butterfly b of one DIT pass (radix R, sub-length S) over NCOL interleaved
+// columns: column col = b % NCOL, group j and offset k < S of b / NCOL = j S
+// + k; its elements are n = base + q S, base = j R S + k.  The index
+// arithmetic is unsigned (b >= 0): divisions by the constants need no sign
+// corrections.
+template <typename LY, int NCOL, int S, int R>
+struct Group {
+  int col, base, p0, st;  // p0 = -1: positions not affine in q
+  unsigned k;
+  __device__ __forceinline__ explicit Group(int b) {
+    const unsigned ub = (unsigned)b;
+    const unsigned bb = ub / NCOL;
+    const unsigned j = bb / S;
+    init((int)(ub % NCOL), j, bb - j * S);
   }
+  // group j, offset k of column c given directly
+  __device__ __forceinline__ Group(int c, unsigned j, unsigned k_) { init(c, j, k_); }
+  __device__ __forceinline__ void init(int c, unsigned j, unsigned k_) {
+    col = c;
+    k = k_;
+    __builtin_assume(k < (unsigned)S);
+    base = (int)(j * (R * S) + k);
+    if constexpr (LY::template is_affine<S * NCOL, S * R * NCOL>())
+      LY::template affine_g<S * NCOL, S * R * NCOL>(col + base * NCOL, (int)j, p0, st);
+    else
+      p0 = -1, st = 0;
+  }
+  __device__ __forceinline__ int pos(int q) const {
+    if constexpr (LY::template is_affine<S * NCOL, S * R * NCOL>()) return p0 + q * st;
+    else return LY::pos(col + (base + q * S) * NCOL);
+  }
+};
+
+// Loads the R elements of a butterfly, applies the inter-pass twiddles
+// (at TWO) and the radix-R DFT; element q belongs at position gr.pos(q).
+template <typename T, int L, int NCOL, int SIGN, typename LY, int S, int TWO, int R>
+__device__ __forceinline__ void dit_butterfly(const cpx<T>* buf, const cpx<T>* __restrict__ tw,
+                                              const Group<LY, NCOL, S, R>& gr, cpx<T>* v) {
+#pragma unroll
+  for (int q = 0; q < R; ++q) v[q] = buf[gr.pos(q)];
   if constexpr (S > 1) {
+    const cpx<T>* __restrict__ twk = tw + (TWO + (int)gr.k * (R - 1));
 #pragma unroll
     for (int q = 1; q < R; ++q) {
-      const cpx<T> w = tw[TWO + k * (R - 1) + (q - 1)];
+      const cpx<T> w = twk[q - 1];
       v[q] = (SIGN < 0) ? cx_mul(v[q], w) : cx_mulc(v[q], w);
     }
   }
@@ -304,16 +350,9 @@ __device__ __forceinline__ void dit_butterfly(const cpx<T>* buf, const cpx<T>* _
 
 // Stores the R outputs of a butterfly back to its positions.
 template <typename T, int NCOL, typename LY, int S, int R>
-__device__ __forceinline__ void dit_store(cpx<T>* buf, const cpx<T>* v, int col, int base) {
-  if constexpr (LY::template is_affine<S * NCOL, S * R * NCOL>()) {
-    int p0, st;
-    LY::template affine<S * NCOL, S * R * NCOL>(col + base * NCOL, p0, st);
+__device__ __forceinline__ void dit_store(cpx<T>* buf, const cpx<T>* v, const Group<LY, NCOL, S, R>& gr) {
 #pragma unroll
-    for (int q = 0; q < R; ++q) buf[p0 + q * st] = v[q];
-  } else {
-#pragma unroll
-    for (int q = 0; q < R; ++q) buf[LY::pos(col + (base + q * S) * NCOL)] = v[q];
-  }
+  for (int q = 0; q < R; ++q) buf[gr.pos(q)] = v[q];
 }
 
 // One DIT pass and the remaining ones.  NCOL interleaved columns (element n of
@@ -323,14 +362,15 @@ template <typename T, int L, int NCOL, int NTHR, int SIGN, typename LY, int S, i
 __device__ __forceinline__ void dit_passes(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
   constexpr int TOT = (L / R) * NCOL;
   constexpr int ITER = (TOT + NTHR - 1) / NTHR;
+  __builtin_assume((unsigned)tid < (unsigned)NTHR);
 #pragma unroll
   for (int it = 0; it < ITER; ++it) {
     const int b = tid + it * NTHR;
     if ((TOT % NTHR) == 0 || b < TOT) {
       cpx<T> v[R];
-      int col, base;
-      dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, b, v, col, base);
-      dit_store<T, NCOL, LY, S, R>(buf, v, col, base);
+      const Group<LY, NCOL, S, R> gr(b);
+      dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, gr, v);
+      dit_store<T, NCOL, LY, S, R>(buf, v, gr);
     }
   }
   __syncthreads();
@@ -343,6 +383,7 @@ __device__ __forceinline__ void dit_passes(cpx<T>* buf, const cpx<T>* __restrict
 template <typename T, int L, int NCOL, int NTHR, int SIGN, typename LY, int S, int TWO, int R, int... Rest,
           typename F>
 __device__ __forceinline__ void dit_passes_emit(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid, F&& emit) {
+  __builtin_assume((unsigned)tid < (unsigned)NTHR);
   if constexpr (sizeof...(Rest) > 0) {
     constexpr int TOT = (L / R) * NCOL;
     constexpr int ITER = (TOT + NTHR - 1) / NTHR;
@@ -351,9 +392,9 @@ __device__ __forceinline__ void dit_passes_emit(cpx<T>* buf, const cpx<T>* __res
       const int b = tid + it * NTHR;
       if ((TOT % NTHR) == 0 || b < TOT) {
         cpx<T> v[R];
-        int col, base;
-        dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, b, v, col, base);
-        dit_store<T, NCOL, LY, S, R>(buf, v, col, base);
+        const Group<LY, NCOL, S, R> gr(b);
+        dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, gr, v);
+        dit_store<T, NCOL, LY, S, R>(buf, v, gr);
       }
     }
     __syncthreads();
@@ -367,10 +408,10 @@ __device__ __forceinline__ void dit_passes_emit(cpx<T>* buf, const cpx<T>* __res
       const int b = tid + it * NTHR;
       if ((TOT % NTHR) == 0 || b < TOT) {
         cpx<T> v[R];
-        int col, base;
-        dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, b, v, col, base);
+        const Group<LY, NCOL, S, R> gr(b);
+        dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, gr, v);
 #pragma unroll
-        for (int q = 0; q < R; ++q) emit(col, base + q * S, v[q]);
+        for (int q = 0; q < R; ++q) emit(gr.col, gr.base + q * S, v[q]);
       }
     }
   }
@@ -391,8 +432,10 @@ __device__ __forceinline__ void fft_run(Plan<R...>, cpx<T>* buf, const cpx<T>* _
 template <typename T, int SIGN, typename LY, int NCOL, int R0, int... Rest>
 __device__ __forceinline__ void first_pass_store(Plan<R0, Rest...>, cpx<T>* v, cpx<T>* buf, int n0, int col = 0) {
   DFT<T, R0, SIGN>::run(v);
-  const int b = Plan<R0, Rest...>::digitrev(n0);
-  dit_store<T, NCOL, LY, 1, R0>(buf, v, col, b);
+  // digitrev(n0) = R0 * (n0's position in the plan of the remaining radices)
+  unsigned j = 0;
+  if constexpr (sizeof...(Rest) > 0) j = (unsigned)Plan<Rest...>::digitrev(n0);
+  dit_store<T, NCOL, LY, 1, R0>(buf, v, Group<LY, NCOL, 1, R0>(col, j, 0u));
 }
 // Remaining passes after first_pass_store (and a barrier).
 template <typename T, int NTHR, int SIGN, typename LY, int R0, int... Rest>

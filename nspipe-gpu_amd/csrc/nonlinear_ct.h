@@ -17,6 +17,7 @@ __device__ __forceinline__ int m_of_ct(int it, int nt) { return it < nt / 2 ? it
 // transform loops and kept live in registers (which cost occupancy).
 __device__ __forceinline__ int opaque(int x) {
   asm volatile("" : "+v"(x));
+  __builtin_assume((unsigned)x < 1024u);
   return x;
 }
 

@@ -379,12 +379,15 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
     return bail(fail(NSP_ERR_INVALID, "the mixed form needs compiled transform plans for 3NT/2 and 3(NZ-1)/2"));
   if (have_ct && (h->nb.mixed || !getenv("NSP_GENERIC_FFT"))) {
     const auto twt = pass_twiddles(ct_radices(g.ntp, true));
-    // the half-length plan's pass twiddles, then those of the pair form's full-length plan
+    // the half-length plan's pass twiddles, then those of the pair form's
+    // full-length plan and the compact table of its last pass
     auto twh = pass_twiddles(ct_radices(g.nh, false));
     const auto rpair = ct_pair_radices(g.nh);
     if (!rpair.empty()) {
       const auto twp = pass_twiddles(rpair);
       twh.insert(twh.end(), twp.begin(), twp.end());
+      const auto twc = compact_twiddles(rpair);
+      twh.insert(twh.end(), twc.begin(), twc.end());
     }
     if (h->nb.mixed) {
       TRY(upload(h, &h->nb.ct_tw_t32, to_float2(twt)));

@@ -332,16 +332,38 @@ struct Group {
 
 // Loads the R elements of a butterfly, applies the inter-pass twiddles
 // (at TWO) and the radix-R DFT; element q belongs at position gr.pos(q).
-template <typename T, int L, int NCOL, int SIGN, typename LY, int S, int TWO, int R>
-__device__ __forceinline__ void dit_butterfly(const cpx<T>* buf, const cpx<T>* __restrict__ tw,
-                                              const Group<LY, NCOL, S, R>& gr, cpx<T>* v) {
+// The twiddles of a pass that ends within the first TWL entries are read
+// from twl, an LDS copy of them (LDS reads instead of per-lane vector-cache
+// reads, whose bandwidth the twiddle traffic of the transforms otherwise
+// fills), the others from tw.  With TWC, a last pass of radix 8 reads only
+// w^1, w^2, w^4 (w = exp(-2 pi i k / L)) from the compact table twc
+// (3 entries per k, compact_twiddles) and forms w^3 = w w^2, w^5 = w w^4,
+// w^6 = w^2 w^4, w^7 = w^3 w^4: 24 instead of 56 bytes per butterfly in
+// FP32, for 4 complex products.
+template <typename T, int L, int NCOL, int SIGN, typename LY, int TWL, int TWC, int S, int TWO, int R>
+__device__ __forceinline__ void dit_butterfly(const cpx<T>* buf, const cpx<T>* __restrict__ tw, const cpx<T>* twl,
+                                              const cpx<T>* twc, const Group<LY, NCOL, S, R>& gr, cpx<T>* v) {
 #pragma unroll
   for (int q = 0; q < R; ++q) v[q] = buf[gr.pos(q)];
-  if constexpr (S > 1) {
-    const cpx<T>* __restrict__ twk = tw + (TWO + (int)gr.k * (R - 1));
+  if constexpr (TWC != 0 && S * R == L && R == 8) {
+    const int o = 3 * (int)gr.k;
+    cpx<T> w[8];
+    w[1] = twc[o];
+    w[2] = twc[o + 1];
+    w[4] = twc[o + 2];
+    w[3] = cx_mul(w[1], w[2]);
+    w[5] = cx_mul(w[1], w[4]);
+    w[6] = cx_mul(w[2], w[4]);
+    w[7] = cx_mul(w[3], w[4]);
+#pragma unroll
+    for (int q = 1; q < R; ++q) v[q] = (SIGN < 0) ? cx_mul(v[q], w[q]) : cx_mulc(v[q], w[q]);
+  } else if constexpr (S > 1) {
+    const int o = TWO + (int)gr.k * (R - 1);
 #pragma unroll
     for (int q = 1; q < R; ++q) {
-      const cpx<T> w = twk[q - 1];
+      cpx<T> w;
+      if constexpr (TWO + S * (R - 1) <= TWL) w = twl[o + q - 1];
+      else w = tw[o + q - 1];
       v[q] = (SIGN < 0) ? cx_mul(v[q], w) : cx_mulc(v[q], w);
     }
   }
@@ -358,8 +380,10 @@ __device__ __forceinline__ void dit_store(cpx<T>* buf, const cpx<T>* v, const Gr
 // One DIT pass and the remaining ones.  NCOL interleaved columns (element n of
 // column c at logical index c + n*NCOL, stored at LY::pos of it); NTHR
 // threads cooperate; ends with a barrier.
-template <typename T, int L, int NCOL, int NTHR, int SIGN, typename LY, int S, int TWO, int R, int... Rest>
-__device__ __forceinline__ void dit_passes(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
+template <typename T, int L, int NCOL, int NTHR, int SIGN, typename LY, int TWL, int TWC, int S, int TWO, int R,
+          int... Rest>
+__device__ __forceinline__ void dit_passes(cpx<T>* buf, const cpx<T>* __restrict__ tw, const cpx<T>* twl,
+                                           const cpx<T>* twc, int tid) {
   constexpr int TOT = (L / R) * NCOL;
   constexpr int ITER = (TOT + NTHR - 1) / NTHR;
   __builtin_assume((unsigned)tid < (unsigned)NTHR);
@@ -369,20 +393,21 @@ __device__ __forceinline__ void dit_passes(cpx<T>* buf, const cpx<T>* __restrict
     if ((TOT % NTHR) == 0 || b < TOT) {
       cpx<T> v[R];
       const Group<LY, NCOL, S, R> gr(b);
-      dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, gr, v);
+      dit_butterfly<T, L, NCOL, SIGN, LY, TWL, TWC, S, TWO, R>(buf, tw, twl, twc, gr, v);
       dit_store<T, NCOL, LY, S, R>(buf, v, gr);
     }
   }
   __syncthreads();
   if constexpr (sizeof...(Rest) > 0)
-    dit_passes<T, L, NCOL, NTHR, SIGN, LY, S * R, TWO + S * (R - 1), Rest...>(buf, tw, tid);
+    dit_passes<T, L, NCOL, NTHR, SIGN, LY, TWL, TWC, S * R, TWO + S * (R - 1), Rest...>(buf, tw, twl, twc, tid);
 }
 
 // Like dit_passes, but the last pass hands its outputs to emit(col, k, value)
 // (k = natural output index) instead of storing them; no barrier after it.
-template <typename T, int L, int NCOL, int NTHR, int SIGN, typename LY, int S, int TWO, int R, int... Rest,
+template <typename T, int L, int NCOL, int NTHR, int SIGN, typename LY, int TWL, int S, int TWO, int R, int... Rest,
           typename F>
-__device__ __forceinline__ void dit_passes_emit(cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid, F&& emit) {
+__device__ __forceinline__ void dit_passes_emit(cpx<T>* buf, const cpx<T>* __restrict__ tw, const cpx<T>* twl, int tid,
+                                                F&& emit) {
   __builtin_assume((unsigned)tid < (unsigned)NTHR);
   if constexpr (sizeof...(Rest) > 0) {
     constexpr int TOT = (L / R) * NCOL;
@@ -393,12 +418,12 @@ __device__ __forceinline__ void dit_passes_emit(cpx<T>* buf, const cpx<T>* __res
       if ((TOT % NTHR) == 0 || b < TOT) {
         cpx<T> v[R];
         const Group<LY, NCOL, S, R> gr(b);
-        dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, gr, v);
+        dit_butterfly<T, L, NCOL, SIGN, LY, TWL, 0, S, TWO, R>(buf, tw, twl, nullptr, gr, v);
         dit_store<T, NCOL, LY, S, R>(buf, v, gr);
       }
     }
     __syncthreads();
-    dit_passes_emit<T, L, NCOL, NTHR, SIGN, LY, S * R, TWO + S * (R - 1), Rest...>(buf, tw, tid, emit);
+    dit_passes_emit<T, L, NCOL, NTHR, SIGN, LY, TWL, S * R, TWO + S * (R - 1), Rest...>(buf, tw, twl, tid, emit);
   } else {
     static_assert(S * R == L, "last pass");
     constexpr int TOT = (L / R) * NCOL;
@@ -409,7 +434,7 @@ __device__ __forceinline__ void dit_passes_emit(cpx<T>* buf, const cpx<T>* __res
       if ((TOT % NTHR) == 0 || b < TOT) {
         cpx<T> v[R];
         const Group<LY, NCOL, S, R> gr(b);
-        dit_butterfly<T, L, NCOL, SIGN, LY, S, TWO, R>(buf, tw, gr, v);
+        dit_butterfly<T, L, NCOL, SIGN, LY, TWL, 0, S, TWO, R>(buf, tw, twl, nullptr, gr, v);
 #pragma unroll
         for (int q = 0; q < R; ++q) emit(gr.col, gr.base + q * S, v[q]);
       }
@@ -421,7 +446,7 @@ __device__ __forceinline__ void dit_passes_emit(cpx<T>* buf, const cpx<T>* __res
 // natural order.
 template <typename T, int NCOL, int NTHR, int SIGN, typename LY, int... R>
 __device__ __forceinline__ void fft_run(Plan<R...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
-  dit_passes<T, Plan<R...>::L, NCOL, NTHR, SIGN, LY, 1, 0, R...>(buf, tw, tid);
+  dit_passes<T, Plan<R...>::L, NCOL, NTHR, SIGN, LY, 0, 0, 1, 0, R...>(buf, tw, nullptr, nullptr, tid);
 }
 
 // First pass in registers: the thread with butterfly index n0 < L/R0 of
@@ -440,19 +465,26 @@ __device__ __forceinline__ void first_pass_store(Plan<R0, Rest...>, cpx<T>* v, c
 // Remaining passes after first_pass_store (and a barrier).
 template <typename T, int NTHR, int SIGN, typename LY, int R0, int... Rest>
 __device__ __forceinline__ void fft_rest(Plan<R0, Rest...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
-  dit_passes<T, Plan<R0, Rest...>::L, 1, NTHR, SIGN, LY, R0, R0 - 1, Rest...>(buf, tw, tid);
+  dit_passes<T, Plan<R0, Rest...>::L, 1, NTHR, SIGN, LY, 0, 0, R0, R0 - 1, Rest...>(buf, tw, nullptr, nullptr, tid);
+}
+// fft_rest with the first TWL twiddle entries read from their LDS copy twl
+// and (TWC) the last radix-8 pass's from the compact table twc.
+template <typename T, int NTHR, int SIGN, typename LY, int TWL, int TWC, int R0, int... Rest>
+__device__ __forceinline__ void fft_rest_tw(Plan<R0, Rest...>, cpx<T>* buf, const cpx<T>* __restrict__ tw,
+                                            const cpx<T>* twl, const cpx<T>* twc, int tid) {
+  dit_passes<T, Plan<R0, Rest...>::L, 1, NTHR, SIGN, LY, TWL, TWC, R0, R0 - 1, Rest...>(buf, tw, twl, twc, tid);
 }
 // Remaining passes after first_pass_store of NCOL interleaved columns (and a
 // barrier): the butterflies of all columns are spread over the threads.
 template <typename T, int NCOL, int NTHR, int SIGN, typename LY, int R0, int... Rest>
 __device__ __forceinline__ void fft_rest_cols(Plan<R0, Rest...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
-  dit_passes<T, Plan<R0, Rest...>::L, NCOL, NTHR, SIGN, LY, R0, R0 - 1, Rest...>(buf, tw, tid);
+  dit_passes<T, Plan<R0, Rest...>::L, NCOL, NTHR, SIGN, LY, 0, 0, R0, R0 - 1, Rest...>(buf, tw, nullptr, nullptr, tid);
 }
 // Remaining passes after first_pass_store, the last one emitting.
 template <typename T, int NCOL, int NTHR, int SIGN, typename LY, int R0, int... Rest, typename F>
 __device__ __forceinline__ void fft_rest_emit(Plan<R0, Rest...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid,
                                               F&& emit) {
-  dit_passes_emit<T, Plan<R0, Rest...>::L, NCOL, NTHR, SIGN, LY, R0, R0 - 1, Rest...>(buf, tw, tid, emit);
+  dit_passes_emit<T, Plan<R0, Rest...>::L, NCOL, NTHR, SIGN, LY, 0, R0, R0 - 1, Rest...>(buf, tw, nullptr, tid, emit);
 }
 
 }  // namespace nsp

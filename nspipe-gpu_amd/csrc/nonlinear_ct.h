@@ -780,13 +780,30 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   static_assert(NB1 <= NTHR, "one first-pass butterfly per thread");
   using LY = Lay<NF / PF::rad[PF::P - 1], PAD>;
   __shared__ cpx<T> buf[LY::size(NF)];
-  // LDS beside the transform buffer, where a workgroup's total stays within
-  // a CU's 160 KB: the first row's N_z (two-row form) and the load offsets
+  // LDS beside the transform buffer, in this order where a workgroup's total
+  // stays within its share of a CU's 160 KB (half of it for the FP32 form
+  // compiled for two workgroups per CU): the pass twiddles of every pass but
+  // the last (TWL entries), the first row's N_z (two-row form), the compact
+  // twiddles of a last radix-8 pass (TWN entries) and the load offsets.
+  constexpr int CAP = sizeof(T) == 4 && !PFE && R0 <= 9 ? 81920 : 163840;
   constexpr int BUFB = LY::size(NF) * (int)sizeof(cpx<T>);
-  constexpr bool NZ_LDS = ROWS == 2 && BUFB + R0 * NTHR * (int)sizeof(T) <= 163840;
-  constexpr bool OFF_LDS = BUFB + (NZ_LDS ? R0 * NTHR * (int)sizeof(T) : 0) + R0 * NTHR * 4 <= 163840;
+  constexpr int TWL = NF / PF::rad[PF::P - 1] - 1;
+  constexpr bool TW_LDS = BUFB + TWL * (int)sizeof(cpx<T>) <= CAP;
+  constexpr int B1 = BUFB + (TW_LDS ? TWL * (int)sizeof(cpx<T>) : 0);
+  constexpr bool NZ_LDS = ROWS == 2 && B1 + R0 * NTHR * (int)sizeof(T) <= CAP;
+  constexpr int B2 = B1 + (NZ_LDS ? R0 * NTHR * (int)sizeof(T) : 0);
+  // a last pass of radix 8 reads the compact table that follows the plan's
+  // NF - 1 pass twiddles
+  constexpr int TWC = PF::rad[PF::P - 1] == 8 ? 1 : 0;
+  constexpr int TWN = TWC ? 3 * (NF / 8) : 0;
+  constexpr bool TWC_LDS = TWC && B2 + TWN * (int)sizeof(cpx<T>) <= CAP;
+  constexpr int B3 = B2 + (TWC_LDS ? TWN * (int)sizeof(cpx<T>) : 0);
+  constexpr bool OFF_LDS = B3 + R0 * NTHR * 4 <= CAP;
   __shared__ int offs[OFF_LDS ? R0 * NTHR : 1];
+  __shared__ cpx<T> twl[TW_LDS ? TWL : 1];
+  __shared__ cpx<T> twcl[TWC_LDS ? TWN : 1];
   const cpx<T>* __restrict__ twf = tw + (NH - 1);
+  const cpx<T>* __restrict__ twc = twf + (NF - 1);
   const int nrows = (g.lv0 + g.nlv) * g.ntp;  // end of this member's (or level chunk's) rows
   const int row0 = g.lv0 * g.ntp + ROWS * (int)blockIdx.x;
   if (row0 >= nrows) return;
@@ -838,6 +855,13 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
       return m < nz ? (int)tp_col(g, m) * CS : OOB;
     }
   };
+  // (read after the barrier that follows the first pass of the first pair)
+  if constexpr (TW_LDS) {
+    for (int e = tid; e < TWL; e += NTHR) twl[e] = twf[e];
+  }
+  if constexpr (TWC_LDS) {
+    for (int e = tid; e < TWN; e += NTHR) twcl[e] = twc[e];
+  }
   // The offsets are the same for every pair and row of the workgroup: where
   // LDS has room they are computed once and read back per load.
   if constexpr (OFF_LDS) {
@@ -890,7 +914,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
     if (PFE && fa >= 0) load(fa, fb, A, B);
     __syncthreads();
     const int lt = opaque(tid);
-    fft_rest<T, NTHR, +1, LY>(PF{}, buf, twf, lt);
+    if constexpr (TWC_LDS) fft_rest_tw<T, NTHR, +1, LY, TW_LDS ? TWL : 0, TWC>(PF{}, buf, twf, twl, twcl, lt);
+    else fft_rest_tw<T, NTHR, +1, LY, TW_LDS ? TWL : 0, TWC>(PF{}, buf, twf, twl, twc, lt);
 #pragma unroll
     for (int q = 0; q < R0; q++) s[q] = own ? buf[LY::pos(lt + NB1 * q)] : zero;
     __syncthreads();
@@ -907,7 +932,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
       first_pass_store<T, -1, LY, 1>(PF{}, v, buf, lt);
     }
     __syncthreads();
-    fft_rest<T, NTHR, -1, LY>(PF{}, buf, twf, lt);
+    if constexpr (TWC_LDS) fft_rest_tw<T, NTHR, -1, LY, TW_LDS ? TWL : 0, TWC>(PF{}, buf, twf, twl, twcl, lt);
+    else fft_rest_tw<T, NTHR, -1, LY, TW_LDS ? TWL : 0, TWC>(PF{}, buf, twf, twl, twc, lt);
     const int t64 = g.ntp * 64;
     for (int k = lt; k < nz; k += NTHR) {
       cpx<T> xa = zero, xb = zero;

@@ -360,6 +360,7 @@ std::vector<int> ct_pair_radices(int nh);
 // per-pass twiddles of a plan: for each pass (radix R, sub-length S) S*(R-1)
 // entries exp(-2 pi i q k / (S R)) at [k*(R-1) + q-1]
 std::vector<double2> pass_twiddles(const std::vector<int>& radices);
+std::vector<double2> compact_twiddles(const std::vector<int>& radices);
 template <typename T>
 bool ct_theta_inv(const NlVariant& v, const Geom& g, const cpx<T>* tw, const double* r, const double2* u,
                   const double2* dR, cpx<T>* W, size_t ws, hipStream_t s, const Batch* bt = nullptr);

@@ -305,4 +305,22 @@ std::vector<double2> pass_twiddles(const std::vector<int>& radices) {
   return tw;
 }
 
+// For a plan whose last pass has radix 8 (sub-length S = L/8): w^1, w^2, w^4
+// of w = exp(-2 pi i k / L) for each k < S, the entries the compact form of
+// that pass reads (the other powers are products of these).  Empty for any
+// other last radix.
+std::vector<double2> compact_twiddles(const std::vector<int>& radices) {
+  std::vector<double2> tw;
+  if (radices.empty() || radices.back() != 8) return tw;
+  long long L = 1;
+  for (int R : radices) L *= R;
+  const long long S = L / 8;
+  for (long long k = 0; k < S; k++)
+    for (int q : {1, 2, 4}) {
+      const double a = -PI2 * double((q * k) % L) / double(L);
+      tw.push_back(make_double2(std::cos(a), std::sin(a)));
+    }
+  return tw;
+}
+
 }  // namespace nsp

@@ -486,5 +486,11 @@ __device__ __forceinline__ void fft_rest_emit(Plan<R0, Rest...>, cpx<T>* buf, co
                                               F&& emit) {
   dit_passes_emit<T, Plan<R0, Rest...>::L, NCOL, NTHR, SIGN, LY, 0, R0, R0 - 1, Rest...>(buf, tw, nullptr, tid, emit);
 }
+// fft_rest_emit with the first TWL twiddle entries read from their LDS copy twl.
+template <typename T, int NCOL, int NTHR, int SIGN, typename LY, int TWL, int R0, int... Rest, typename F>
+__device__ __forceinline__ void fft_rest_emit_tw(Plan<R0, Rest...>, cpx<T>* buf, const cpx<T>* __restrict__ tw,
+                                                 const cpx<T>* twl, int tid, F&& emit) {
+  dit_passes_emit<T, Plan<R0, Rest...>::L, NCOL, NTHR, SIGN, LY, TWL, R0, R0 - 1, Rest...>(buf, tw, twl, tid, emit);
+}
 
 }  // namespace nsp

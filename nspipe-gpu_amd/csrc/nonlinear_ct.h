@@ -1204,10 +1204,19 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
   constexpr int PAIRS = NB1 * TL;
   constexpr int PPT = (PAIRS + NTHR - 1) / NTHR;
   __shared__ cpx<T> buf[LY::size(NTP * TL)];
+  // The pass twiddles (NTP - 1 entries) copied into LDS where that keeps
+  // the workgroups per CU that the transform buffer allows (two up to 80 KB)
+  constexpr int BUFB = LY::size(NTP * TL) * (int)sizeof(cpx<T>);
+  constexpr int TWB = (NTP - 1) * (int)sizeof(cpx<T>);
+  constexpr bool TW_LDS = BUFB <= 81920 ? BUFB + TWB <= 81920 : BUFB + TWB <= 163840;
+  __shared__ cpx<T> twl[TW_LDS ? NTP - 1 : 1];
   int i, l0;
   theta_block(g, TL, ord, i, l0);
   if (i >= g.lv0 + g.nlv || l0 >= g.nz) return;  // beyond this member's (or level chunk's) extent
   const int tid = threadIdx.x;
+  if constexpr (TW_LDS) {  // read after the barrier that follows the first pass
+    for (int e = tid; e < NTP - 1; e += NTHR) twl[e] = tw[e];
+  }
   const int nz = g.nz;
   const double norm = double(NTP) * double(g.nzf);  // mod_fft.cu:41
   // the FP32 form scales by the reciprocal (its result is rounded to FP32),
@@ -1257,7 +1266,7 @@ __global__ __launch_bounds__(NTHR) void theta_fwd2_ct(Geom g, const cpx<T>* __re
     // N(u) is stored at the transform precision (FP32 in the mixed form)
     // tile-major N(u): the TL columns from l0 lie in one 64-column tile
     cpx<T>* dst = reinterpret_cast<cpx<T>*>(N) + c * dr_cs(g) + (size_t)i * dr_plane(g) + dr_col(g, l0);
-    fft_rest_emit<T, TL, NTHR, -1, LY>(PL{}, buf, tw, lt, [&](int col, int j, cpx<T> w) {
+    fft_rest_emit_tw<T, TL, NTHR, -1, LY, TW_LDS ? NTP - 1 : 0>(PL{}, buf, tw, twl, lt, [&](int col, int j, cpx<T> w) {
       const int l = l0 + col;
       if (l < nz && (j < half || j >= NT)) {
         const int it = j < half ? j : j - half;

@@ -70,9 +70,14 @@ __device__ __forceinline__ void theta_block(const Geom& g, int TL, int ord, int&
 // W: 9 intermediates [f][NR][NTP][NZ] in precision T: u, d_r u, d_theta u / r.
 // The radial derivative (derivr_k, mod_radialFD.cu:10-30, flags 1,1,0) comes
 // from dR (dr3_kernel, stored at precision T); forming it here from seven
-// levels was slower.
+// levels was slower.  Per component the transforms run in the order u,
+// d_theta u / r (both from the u inputs), d_r u; one set of inputs is held
+// in registers at a time (u at precision T, then d_r u), the next set's
+// loads issued as soon as the previous one has been scattered into LDS, so
+// that they are in flight during the transform passes.  The FP32 form is
+// compiled for 4 waves per SIMD (two 512-thread workgroups per CU).
 template <typename T, int TL, int NTHR, typename LY, typename PL, int BAT>
-__global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __restrict__ tw,
+__global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 1))) void theta_inv_ct(Geom g, const cpx<T>* __restrict__ tw,
                                                      const double* __restrict__ r,
                                                      const double2* __restrict__ u,
                                                      const cpx<T>* __restrict__ dR,
@@ -107,23 +112,20 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
     else return kt * x / ri;
   };
   constexpr int half = NT / 2;
-  // digit-reversed LDS positions of this thread's elements (padForward_k row
-  // map included): the same for all nine transforms of the block
-  int pos[EPT];
-#pragma unroll
-  for (int q = 0; q < EPT; q++) {
-    const int e = tid + q * NTHR;
-    const int it = e / TL, tl = e - it * TL;
-    const int j = it < half ? it : it + half;
-    pos[q] = e < NT * TL ? LY::pos(PL::digitrev(j) * TL + tl) : 0;
-  }
+  // digit-reversed LDS positions of element e of the inputs (padForward_k
+  // row map included) and of element e of the zero padding rows; computed
+  // where used (unsigned arithmetic, constant divisors) rather than held in
+  // registers across the transforms
+  auto pos = [&](int e) -> int {
+    const unsigned it = (unsigned)e / TL, tl = (unsigned)e - it * TL;
+    const unsigned j = it < (unsigned)half ? it : it + half;
+    return LY::pos(PL::digitrev((int)j) * TL + (int)tl);
+  };
+  auto zpos = [&](int e) -> int {
+    const unsigned it = (unsigned)e / TL, tl = (unsigned)e - it * TL;
+    return LY::pos(PL::digitrev((int)(half + it)) * TL + (int)tl);
+  };
   constexpr int EPZ = ((NTP - NT) * TL + NTHR - 1) / NTHR;  // padding-row elements per thread
-  int zpos[EPZ];
-#pragma unroll
-  for (int q = 0; q < EPZ; q++) {
-    const int e = tid + q * NTHR;
-    zpos[q] = e < (NTP - NT) * TL ? LY::pos(PL::digitrev(half + e / TL) * TL + e % TL) : -1;
-  }
   // A thread's elements share its column tl = tid % TL (NTHR is a multiple
   // of TL) and step by NTHR / TL rows: one column test, affine offsets, and
   // the loads of a component issued together without per-element branches.
@@ -136,48 +138,49 @@ __global__ __launch_bounds__(NTHR) void theta_inv_ct(Geom g, const cpx<T>* __res
   // the same elements of the tile-major d_r u
   const size_t d0 = (size_t)i * dr_plane(g) + (size_t)(tid / TL) * 64 + dr_col(g, lok ? l0 + tcol : 0);
   constexpr int dstep = (NTHR / TL) * 64;
-  for (int c = 0; c < 3; c++) {
+  // inputs of the current transforms: u of component c (rounded to T, which
+  // is how every transform of the FP32 form takes them), or d_r u
+  cpx<T> X[EPT];
+  auto load = [&](int c, bool dr) {
     const double2* src = u + c * g.n;
-    double2 U[EPT];
-    cpx<T> DR[EPT];
 #pragma unroll
     for (int q = 0; q < EPT; q++) {
       const bool in = q < FULL || tid + q * NTHR < NT * TL;
-      const size_t off = in ? o0 + q * ostep : o0;
-      U[q] = src[off];
-      DR[q] = dR[c * dr_cs(g) + (in ? d0 + q * dstep : d0)];
-      if (q >= FULL && !in) {
-        U[q] = make_double2(0.0, 0.0);
-        DR[q] = cpx<T>{T(0), T(0)};
+      if (dr) {
+        X[q] = dR[c * dr_cs(g) + (in ? d0 + q * dstep : d0)];
+      } else {
+        const double2 v = src[in ? o0 + q * ostep : o0];
+        X[q] = cpx<T>{T(v.x), T(v.y)};
       }
+      if ((q >= FULL && !in) || !lok) X[q] = cpx<T>{T(0), T(0)};  // rows past NT, columns past NZ
     }
-    if (!lok) {  // columns past NZ (last tile only)
-#pragma unroll
-      for (int q = 0; q < EPT; q++) {
-        U[q] = make_double2(0.0, 0.0);
-        DR[q] = cpx<T>{T(0), T(0)};
-      }
-    }
+  };
+  load(0, false);
+  for (int c = 0; c < 3; c++) {
 #pragma unroll 1
-    for (int kind = 0; kind < 3; kind++) {  // 0: u, 1: d_r u, 2: d_theta u / r
+    for (int kk = 0; kk < 3; kk++) {
+      const int kind = kk == 0 ? 0 : (kk == 1 ? 2 : 1);  // 0: u, 1: d_r u, 2: d_theta u / r
       const int lt = opaque(tid);
 #pragma unroll
-      for (int q = 0; q < EPZ; q++)  // padding rows
-        if (zpos[q] >= 0) buf[zpos[q]] = cpx<T>{T(0), T(0)};
+      for (int q = 0; q < EPZ; q++) {  // padding rows
+        const int e = lt + q * NTHR;
+        if (e < (NTP - NT) * TL) buf[zpos(e)] = cpx<T>{T(0), T(0)};
+      }
 #pragma unroll
       for (int q = 0; q < EPT; q++) {
         const int e = lt + q * NTHR;
         if (q < FULL || e < NT * TL) {
-          const int it = e / TL;
-          const double2 v = U[q];
-          cpx<T> w = (kind == 1) ? DR[q] : cpx<T>{T(v.x), T(v.y)};
+          const int it = (int)((unsigned)e / TL);
+          cpx<T> w = X[q];
           if (kind == 2) {  // derivt_k (mod_deriv.cu:10-21)
             const double kt = double(it < half ? it : it - NT);
-            w = cpx<T>{T(m_over_r(-kt, v.y)), T(m_over_r(kt, v.x))};
+            w = cpx<T>{T(m_over_r(-kt, double(X[q].y))), T(m_over_r(kt, double(X[q].x)))};
           }
-          buf[pos[q]] = w;
+          buf[pos(e)] = w;
         }
       }
+      if (kind == 2) load(c, true);
+      else if (kind == 1 && c < 2) load(c + 1, false);
       __syncthreads();
       fft_run<T, TL, NTHR, +1, LY>(PL{}, buf, tw, lt);
       // rows lt / TL + q NTHR / TL of column tcol (tile-major W: rows of a

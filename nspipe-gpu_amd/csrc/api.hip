@@ -644,6 +644,7 @@ nsp_status nsp_autotune(nsp_handle* h, int reps) {
       cands.push_back(ThetaVar{0, 16, 256, 0, ord});
       cands.push_back(ThetaVar{0, 16, 512, 0, ord});
       cands.push_back(ThetaVar{0, 16, 1024, 0, ord});
+      if (stage == 0) cands.push_back(ThetaVar{0, 32, 1024, 0, ord});
       cands.push_back(ThetaVar{0, 16, 256, 3, ord});
       for (const auto& tr : treg) cands.push_back(ThetaVar{1, tr[0], tr[1], 0, ord});
     }

@@ -886,8 +886,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
       }
     }
   };
-  // z = a + i b at the owned samples (d_z applied first where asked) and the first pass
-  auto first = [&](const cpx<T>* A, const cpx<T>* B, bool dza, bool dzb) {
+  // z = a + i b at the owned samples (d_z applied first where asked) and the
+  // first pass; sub0: field a enters without its l = 0 mode (added back to
+  // the samples by the caller)
+  auto first = [&](const cpx<T>* A, const cpx<T>* B, bool dza, bool dzb, bool sub0) {
     const int lt = opaque(tid);
     if (own) {
       cpx<T> v[R0];
@@ -896,7 +898,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
         const int n = lt + NB1 * q;
         const bool cj = conj_of(lt, q);
         cpx<T> a = A[q], b = B[q];
-        if (q == 0 && n == 0) { a.y = T(0); b.y = T(0); }
+        if (q == 0 && n == 0) {
+          a = cpx<T>{sub0 ? T(0) : a.x, T(0)};
+          b.y = T(0);
+        }
         if (dza || dzb) {  // derivz_k (mod_deriv.cu:24-35) on the stored mode
           const T km = kz0 * T(cj ? NF - n : n);
           if (dza) a = cpx<T>{-km * a.y, km * a.x};
@@ -912,8 +917,8 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   // inverse pair from A, B, then the loads of the next pair (if any): PFE 1
   // issues them right after the first pass (in flight during the LDS
   // passes), PFE 0 after the outputs are read (fewer live registers)
-  auto inverse = [&](bool dza, bool dzb, int fa, int fb) {
-    first(A, B, dza, dzb);
+  auto inverse = [&](bool dza, bool dzb, int fa, int fb, bool sub0 = false) {
+    first(A, B, dza, dzb, sub0);
     if (PFE && fa >= 0) load(fa, fb, A, B);
     __syncthreads();
     const int lt = opaque(tid);
@@ -953,6 +958,17 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   };
   // field index in W: u_r, u_t, u_z (0-2), d_r u_c (3 + c), d_theta u_c / r (6 + c)
   T ur[R0], ut[R0], uz[R0], hz[R0], n0[R0], n1[R0];
+  // Pairing.  A pair z = a + i b is rounded at the scale of the larger of
+  // a and b, so the fields are paired by size: near the wall (distance d,
+  // no slip, zero divergence) u_r, d_theta u_r / r and d_z u_r fall off as
+  // d^2, u_t, u_z and their theta and z derivatives and d_r u_r as d, and
+  // d_r u_t, d_r u_z stay finite.  The pairs (u_z, u_t), (u_r, d_z u_r),
+  // (d_theta u_r / r, d_r u_r), (d_theta u_t / r, d_z u_t), (d_r u_z,
+  // d_r u_t), (d_theta u_z / r, d_z u_z) mismatch only in the third, whose
+  // d^2 member enters N_r alone, where u_t^2 / r (order d^2) dominates.  The
+  // mean flow and its shear (the l = 0 modes of u_z and d_r u_z, constant
+  // along the row and an order of magnitude above the fluctuations) are kept
+  // out of their transforms and added to the samples afterwards.
   // N_z of the first row, transformed with the second row's: held in LDS
   // beside the transform buffer where both fit (each thread its own values,
   // so no barrier), which keeps it out of the registers the second row needs
@@ -962,45 +978,47 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
 #pragma unroll 1
   for (int rr = 0; rr < nrow; rr++) {
     set_row(row0 + rr);
-    load(0, 1, A, B);
-    inverse(false, false, 2, 0);
+    const T m0z = W[2 * ws + roff].x, m0d = W[5 * ws + roff].x;  // l = 0 modes of u_z, d_r u_z
+    load(2, 1, A, B);
+    inverse(false, false, 0, 0, true);  // (u_z, u_t)
 #pragma unroll
-    for (int q = 0; q < R0; q++) { ur[q] = s[q].x; ut[q] = s[q].y; }
-    inverse(false, true, 3, 6);
+    for (int q = 0; q < R0; q++) { uz[q] = s[q].x + m0z; ut[q] = s[q].y; }
+    inverse(false, true, 6, 3);  // (u_r, d_z u_r)
 #pragma unroll
-    for (int q = 0; q < R0; q++) { uz[q] = s[q].x; hz[q] = s[q].y; }
+    for (int q = 0; q < R0; q++) { ur[q] = s[q].x; hz[q] = s[q].y; }
     // calcNonlinear_k (mod_io_nonlinear.cu:142-170), summed in its order
-    inverse(false, false, 4, 7);
+    inverse(false, false, 7, 1);  // (d_theta u_r / r, d_r u_r)
 #pragma unroll
     for (int q = 0; q < R0; q++) {
-      T v = ur[q] * s[q].x;
-      v += ut[q] * s[q].y;
+      T v = ur[q] * s[q].y;
+      v += ut[q] * s[q].x;
       v = v + uz[q] * hz[q];
       v = v - ut[q] * over_r(ut[q]);
       n0[q] = -v;
     }
-    inverse(false, false, 1, 2);
+    inverse(false, true, 5, 4);  // (d_theta u_t / r, d_z u_t)
 #pragma unroll
     for (int q = 0; q < R0; q++) {
-      T v = ur[q] * s[q].x;
-      v += ut[q] * s[q].y;
-      n1[q] = v;
-    }
-    inverse(true, true, 5, 8);
-#pragma unroll
-    for (int q = 0; q < R0; q++) {
-      T v = n1[q] + uz[q] * s[q].x;
-      v = v + ur[q] * over_r(ut[q]);
-      n1[q] = -v;
+      n1[q] = ut[q] * s[q].x;
       hz[q] = s[q].y;
     }
-    forward(n0, n1, Y + roff, Y + ws + roff);
-    inverse(false, false, -1, -1);
+    inverse(false, false, 8, 2, true);  // (d_r u_z, d_r u_t)
 #pragma unroll
     for (int q = 0; q < R0; q++) {
-      T v = ur[q] * s[q].x;
-      v += ut[q] * s[q].y;
+      T v = ur[q] * s[q].y;
+      v = v + n1[q];
       v = v + uz[q] * hz[q];
+      v = v + ur[q] * over_r(ut[q]);
+      n1[q] = -v;
+      hz[q] = ur[q] * (s[q].x + m0d);  // the first term of N_z
+    }
+    forward(n0, n1, Y + roff, Y + ws + roff);
+    inverse(false, true, -1, -1);  // (d_theta u_z / r, d_z u_z)
+#pragma unroll
+    for (int q = 0; q < R0; q++) {
+      T v = hz[q];
+      v += ut[q] * s[q].x;
+      v = v + uz[q] * s[q].y;
       n0[q] = -v;
     }
     if (rr == 0 && nrow == 2) {  // held for the pair with the second row
@@ -1373,6 +1391,7 @@ bool theta_inv_plan(const ThetaVar& v, const Geom& g, const cpx<T>* tw, const do
   else if (tl == 8 && v.pad == 2) { if constexpr (lds_fits<T, R8>(PL::L * 8)) theta_inv_ct<T, 8, 256, R8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 16 && v.pad == 0 && v.nthr == 512) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_inv_ct<T, 16, 512, Dense, PL, BAT><<<nb, 512, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 16 && v.pad == 0 && v.nthr == 1024) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_inv_ct<T, 16, 1024, Dense, PL, BAT><<<nb, 1024, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
+  else if (tl == 32 && v.pad == 0 && v.nthr == 1024) { if constexpr (lds_fits<T, Dense>(PL::L * 32)) theta_inv_ct<T, 32, 1024, Dense, PL, BAT><<<nb, 1024, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 16 && v.pad == 0) { if constexpr (lds_fits<T, Dense>(PL::L * 16)) theta_inv_ct<T, 16, 256, Dense, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 4 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 4)) theta_inv_ct<T, 4, 256, S4, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }
   else if (tl == 8 && v.pad == 3) { if constexpr (lds_fits<T, Dense>(PL::L * 8)) theta_inv_ct<T, 8, 256, S8, PL, BAT><<<nb, 256, 0, s>>>(g, tw, r, u, dR, W, ws, v.ord, tab); else return false; }

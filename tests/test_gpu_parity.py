@@ -228,6 +228,44 @@ def test_every_kernel_variant_keeps_parity(shape):
     p.close()
 
 
+def test_mixed_forms_keep_fp32_accuracy():
+    """The forms the mixed form is tuned to at the headline transform sizes
+    (pair z pass with one or two rows per workgroup, the 16-column theta
+    inverse on 256 to 1024 threads, and the 32-column theta inverse that only
+    the FP32 buffer fits in LDS) keep N(u) at FP32 accuracy against the
+    oracle: relative L2 below 1.5e-6 per component (the default forms: about
+    2e-7).  A pair z = a + ib rounds at the scale of the larger field; with
+    fields paired regardless of their size near the wall the pair form's N_r
+    error was 4.4e-6 here."""
+    shape = COMPILED[3]
+    print(f"[gpu] mixed theta forms {shape}", flush=True)
+    p = nsp.Pipe(**shape, precision=nsp.MIXED)
+    o = Oracle(**shape)
+    p.seeded_ic(SEED1, 6, 6, 1, 0.1)
+    u = p.get_state()
+    b = o.nonlinear(*u)
+    ref = p.nonlinear(*u)
+    base = dict(theta_inv=dict(alg=0, tl=16, nthr=1024, pad=0, ord=1),
+                theta_fwd=dict(alg=1, tl=16, nthr=256, pad=0, ord=1),
+                z_alg=4, z_nthr=512, z_pad=1, z_pf=0)
+    e0 = [rel_l2(ref[c], b[c]) for c in range(3)]
+    print("[gpu] default forms vs oracle", e0, flush=True)
+    forms = [dict(base, theta_inv=dict(alg=0, tl=32, nthr=1024, pad=0, ord=o_)) for o_ in (0, 1)]
+    forms += [dict(base, theta_inv=dict(alg=0, tl=16, nthr=n, pad=0, ord=1)) for n in (256, 512, 1024)]
+    forms += [dict(base, z_alg=za, z_pad=zp) for za in (3, 4) for zp in (0, 1, 2)]
+    bad = []
+    for t in forms:
+        p.set_tuning(t)
+        a = p.nonlinear(*u)
+        e = [rel_l2(a[c], b[c]) for c in range(3)]
+        d = [rel_l2(a[c], ref[c]) for c in range(3)]
+        print("[gpu]", t, "vs oracle", e, "vs default", d, flush=True)
+        if not all(e[c] < 1.5e-6 for c in range(3)):
+            bad.append((t, e))
+    p.close()
+    assert not bad, bad
+
+
 # the longest compiled plans: theta 1152 (NT = 768) and half-z 4608 (NZ = 3073)
 LONG = [dict(nr=8, nt=768, nz=65, lz=4 * np.pi, re=19000.0, dt=0.0005),
         dict(nr=8, nt=64, nz=3073, lz=4 * np.pi, re=19000.0, dt=0.0005)]

@@ -5,6 +5,7 @@ Bar (FP64 form): <= 1e-12 relative L2 on every spectral field after one
 step and after one nonlinear evaluation, <= 1e-9 after 100 steps, identical
 corrector counts, equal divergence and wall residuals.
 """
+import json
 import os
 
 import numpy as np
@@ -680,18 +681,27 @@ def fluct_ke(pipe):
     return tot
 
 
+# the kernel forms the autotuner picks for the headline workload on MI355X
+# (the forms bench.py times)
+TUNED3 = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "nspipe-gpu_amd", "tuning",
+                      "mi355x_96x384x1537.json")
+
+
 def test_config3_mixed_against_fp64_hundred_steps():
     """Headline configuration (NR=96, NT=384, NZ=1537, Re 11700, L 10 pi R,
-    dt 0.001, seeded noise of rms 0.15 U_bulk): the mixed form against the FP64
-    form on the GPU after 100 steps.  Kinetic energy, fluctuation kinetic
-    energy, wall shear (u_tau), centre-line velocity within 1e-5 relative;
-    divergence and wall residuals at the FP64 form's level; identical
-    corrector counts."""
+    dt 0.001, seeded noise of rms 0.15 U_bulk), each form with its tuned
+    kernel forms: the mixed form against the FP64 form on the GPU after 100
+    steps.  Kinetic energy, fluctuation kinetic energy, wall shear (u_tau),
+    centre-line velocity within 1e-5 relative; divergence and wall residuals
+    at the FP64 form's level; identical corrector counts."""
     cfg = dict(nr=96, nt=384, nz=1537, lz=10 * np.pi, re=11700.0, dt=0.001)
+    with open(TUNED3) as fh:
+        tuned = json.load(fh)
     out = {}
     for prec in (nsp.FP64, nsp.MIXED, nsp.MIXED_REFINED):
         print(f"[gpu] config 3, 100 steps, precision {prec}", flush=True)
         p = nsp.Pipe(**cfg, precision=prec)
+        p.set_tuning(tuned["fp64" if prec == nsp.FP64 else "mixed"])
         p.seeded_ic(20261202 + 3 * 1000, 4, 4, 0, 0.15 * 0.5)
         n_it, _, flags = p.step(100)
         assert np.all(flags == 0)

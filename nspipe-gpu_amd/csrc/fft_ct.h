@@ -448,6 +448,12 @@ template <typename T, int NCOL, int NTHR, int SIGN, typename LY, int... R>
 __device__ __forceinline__ void fft_run(Plan<R...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, int tid) {
   dit_passes<T, Plan<R...>::L, NCOL, NTHR, SIGN, LY, 0, 0, 1, 0, R...>(buf, tw, nullptr, nullptr, tid);
 }
+// fft_run with the first TWL twiddle entries read from their LDS copy twl.
+template <typename T, int NCOL, int NTHR, int SIGN, typename LY, int TWL, int... R>
+__device__ __forceinline__ void fft_run_tw(Plan<R...>, cpx<T>* buf, const cpx<T>* __restrict__ tw, const cpx<T>* twl,
+                                           int tid) {
+  dit_passes<T, Plan<R...>::L, NCOL, NTHR, SIGN, LY, TWL, 0, 1, 0, R...>(buf, tw, twl, nullptr, tid);
+}
 
 // First pass in registers: the thread with butterfly index n0 < L/R0 of
 // column col holds the samples x[n0 + (L/R0) q], q < R0.  Their

@@ -98,11 +98,20 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   constexpr int NT = 2 * NTP / 3;
   constexpr int EPT = (NT * TL + NTHR - 1) / NTHR;
   __shared__ cpx<T> buf[LY::size(NTP * TL)];
+  // the pass twiddles (NTP - 1 entries) copied into LDS where that keeps the
+  // workgroups per CU that the transform buffer allows (two up to 80 KB)
+  constexpr int BUFB = LY::size(NTP * TL) * (int)sizeof(cpx<T>);
+  constexpr int TWB = (NTP - 1) * (int)sizeof(cpx<T>);
+  constexpr bool TW_LDS = BUFB <= 81920 ? BUFB + TWB <= 81920 : BUFB + TWB <= 163840;
+  __shared__ cpx<T> twl[TW_LDS ? NTP - 1 : 1];
   const int nz = g.nz;
   int i, l0;
   theta_block(g, TL, ord, i, l0);
   if (i >= g.lv0 + g.nlv || l0 >= nz) return;  // beyond this member's (or level chunk's) extent
   const int tid = threadIdx.x;
+  if constexpr (TW_LDS) {  // read after the barrier that follows the first scatter
+    for (int e = tid; e < NTP - 1; e += NTHR) twl[e] = tw[e];
+  }
   const double ri = r[i];
   // m u / r: the FP32 form multiplies by the reciprocal (the result is rounded
   // to FP32 anyway), the FP64 form divides as derivt_k does
@@ -182,7 +191,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
       if (kind == 2) load(c, true);
       else if (kind == 1 && c < 2) load(c + 1, false);
       __syncthreads();
-      fft_run<T, TL, NTHR, +1, LY>(PL{}, buf, tw, lt);
+      fft_run_tw<T, TL, NTHR, +1, LY, TW_LDS ? NTP - 1 : 0>(PL{}, buf, tw, twl, lt);
       // rows lt / TL + q NTHR / TL of column tcol (tile-major W: rows of a
       // tile are 64 values apart): one pointer step per store
       constexpr int WFULL = (NTP * TL) / NTHR, WREM = (NTP * TL) % NTHR;

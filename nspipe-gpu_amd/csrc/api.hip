@@ -480,13 +480,17 @@ static nsp_status create_handle(const nsp_params* p, int device, const nsp_handl
   // transform intermediates after the setup buffers are freed (lower peak)
   // W carries nh spare elements: the z pass's buffer loads of a row stop at
   // its nz stored modes by the resource's range check, and the spare tail
-  // keeps even the last row's unchecked extent (nh elements) inside the block
+  // keeps even the last row's unchecked extent (nh elements) inside the block.
+  // W starts zeroed and no kernel writes its columns l >= nz (the padding of
+  // the last 64-column tile), which the pair z pass reads as zero modes.
   if (h->nb.mixed) {
     TRY(dalloc(h, &h->nb.W32, 9 * h->nb.wstride + g.nh));
     TRY(dalloc(h, &h->nb.Y32, 3 * h->nb.wstride));
+    HTRY(hipMemsetAsync(h->nb.W32, 0, (9 * h->nb.wstride + g.nh) * sizeof(*h->nb.W32), h->stream));
   } else {
     TRY(dalloc(h, &h->nb.W, 9 * h->nb.wstride + g.nh));
     TRY(dalloc(h, &h->nb.Y, 3 * h->nb.wstride));
+    HTRY(hipMemsetAsync(h->nb.W, 0, (9 * h->nb.wstride + g.nh) * sizeof(*h->nb.W), h->stream));
   }
 #undef TRY
 #undef HTRY

@@ -826,7 +826,6 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   const T kz0 = T(g.kz0);
   const cpx<T> zero{T(0), T(0)};
   constexpr int CS = (int)sizeof(cpx<T>);
-  constexpr int OOB = 0x7ffffff0;
   // the current row: level i, radius, row th of the level's tile-major planes
   T rad = T(1), irad = T(1);
   size_t roff = 0;
@@ -850,9 +849,12 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
     if (NB1 * q > NH) return true;              // every lane: n > NH
     return lt + NB1 * q > NH;
   };
-  // the stored modes m of fields fa, fb at the owned samples, zero past NZ
-  // (range-checked loads).  With NB1 and NZF multiples of 64, the tile-major
-  // offsets of m = n and m = NZF - n are affine in q.
+  // the stored modes m of fields fa, fb at the owned samples, zero past NZ:
+  // columns nz <= m < 64 ceil(nz / 64) are W's zero padding (W starts zeroed
+  // and is written at m < nz only), columns beyond lie past the end of the
+  // row's plane, where the buffer range check returns zero.  With NB1 and
+  // NZF multiples of 64, the tile-major offsets of m = n and m = NZF - n are
+  // affine in q.
   auto offset = [&](int lt, int q) -> int {
     const int n = lt + NB1 * q;
     const int t64 = g.ntp * 64;
@@ -860,11 +862,10 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
       const int dq = (NB1 / 64) * t64 * CS;
       const int ok = ((lt >> 6) * t64 + (lt & 63)) * CS + q * dq;
       const int oc = (((NF >> 6) - ((lt + 63) >> 6)) * t64 + ((-lt) & 63)) * CS - q * dq;
-      const bool cj = conj_of(lt, q);
-      return cj ? (NF - n < nz ? oc : OOB) : (n < nz ? ok : OOB);
+      return conj_of(lt, q) ? oc : ok;
     } else {
       const int m = conj_of(lt, q) ? NF - n : n;
-      return m < nz ? (int)tp_col(g, m) * CS : OOB;
+      return (int)tp_col(g, m) * CS;
     }
   };
   // (read after the barrier that follows the first pass of the first pair)

@@ -924,6 +924,18 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
   };
   cpx<T> A[R0], B[R0];
   cpx<T> s[R0];
+  // LDS position of output n = lt + NB1 q (0 <= lt < NTHR): with a padded
+  // layout of rows of RW elements and NTHR <= RW, the row of n is that of
+  // NB1 q or the next one, so one comparison replaces the division
+  auto out_pos = [&](int lt, int q) -> int {
+    constexpr int RW = NF / PF::rad[PF::P - 1];
+    if constexpr (PAD != 0 && NTHR <= RW) {
+      const int c = NB1 * q;
+      return lt + c + (c / RW + (lt >= RW - c % RW ? 1 : 0)) * PAD;
+    } else {
+      return LY::pos(lt + NB1 * q);
+    }
+  };
   // inverse pair from A, B, then the loads of the next pair (if any): PFE 1
   // issues them right after the first pass (in flight during the LDS
   // passes), PFE 0 after the outputs are read (fewer live registers)
@@ -935,7 +947,7 @@ __global__ __launch_bounds__(NTHR) __attribute__((amdgpu_waves_per_eu(sizeof(T) 
     if constexpr (TWC_LDS) fft_rest_tw<T, NTHR, +1, LY, TW_LDS ? TWL : 0, TWC>(PF{}, buf, twf, twl, twcl, lt);
     else fft_rest_tw<T, NTHR, +1, LY, TW_LDS ? TWL : 0, TWC>(PF{}, buf, twf, twl, twc, lt);
 #pragma unroll
-    for (int q = 0; q < R0; q++) s[q] = own ? buf[LY::pos(lt + NB1 * q)] : zero;
+    for (int q = 0; q < R0; q++) s[q] = own ? buf[out_pos(lt, q)] : zero;
     __syncthreads();
     if (!PFE && fa >= 0) load(fa, fb, A, B);
   };
